@@ -1,0 +1,222 @@
+/*
+ * vren_hip/capi.h -- the C ABI of the MI355X (gfx950) clustered-shading path.
+ *
+ * Every entry point below replaces one operator of the reference (loryruta/vren). The reference records
+ * Vulkan dispatches into a caller-supplied VkCommandBuffer and returns void; here each call enqueues HIP
+ * kernels on the caller's stream and returns a status code. Conventions:
+ *   - all buffers are device pointers owned by the caller (the reference's vk_utils::buffer + byte offset);
+ *   - calls are asynchronous on `stream` (a hipStream_t; NULL = the legacy default stream);
+ *   - return 0 (VREN_OK) or a negative vren_status; arguments are validated on the host before any HIP call,
+ *     so invalid lengths / capacities are reported without touching the device;
+ *   - a vren_ctx is not thread-safe (one per host thread), mirroring single-threaded command recording;
+ *   - capacity overflows that only the device can see (unique clusters, assigned lights) never write out of
+ *     bounds: they raise a device-side flag that vren_sync_status() reports as VREN_ECAPACITY
+ *     (the reference writes past its buffers silently, config.hpp:23-24).
+ */
+#ifndef VREN_HIP_CAPI_H_
+#define VREN_HIP_CAPI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VREN_EXPORT __attribute__((visibility("default")))
+
+typedef void* vren_stream_t; /* hipStream_t */
+
+typedef enum vren_status {
+    VREN_OK = 0,
+    VREN_EINVAL = -1,    /* precondition violated (reference: assert / std::invalid_argument) */
+    VREN_ECAPACITY = -2, /* an output exceeded its capacity (reference: silent out-of-bounds write) */
+    VREN_EHIP = -3,      /* a HIP runtime call failed (reference: VREN_CHECK -> std::runtime_error) */
+    VREN_ENOMEM = -4,    /* device allocation failed */
+} vren_status;
+
+enum { VREN_REDUCE_ADD = 0, VREN_REDUCE_MIN = 1, VREN_REDUCE_MAX = 2 };
+
+/* bvh_node (build_bvh.hpp:8-18): 32 bytes */
+#define VREN_BVH_LEAF_NODE 0xFFFFFFFFu
+#define VREN_BVH_INVALID_NODE 0xFFFFFFFEu
+typedef struct vren_bvh_node {
+    float min[3];
+    uint32_t next; /* first child index, VREN_BVH_LEAF_NODE or VREN_BVH_INVALID_NODE */
+    float max[3];
+    uint32_t pad;
+} vren_bvh_node;
+
+/* camera (camera.hpp fields used by camera.cpp:9-50) */
+typedef struct vren_camera {
+    float position[3];
+    float fov_y;
+    float aspect_ratio;
+    float near_plane;
+    float far_plane;
+    float yaw;
+    float pitch;
+} vren_camera;
+
+/* cluster-key encoders */
+enum {
+    VREN_KEYS_REFERENCE = 0, /* 8b tile x | 8b tile y | 10b log slice | 6b cube-face normal code (find_unique_clusters.comp:72-76) */
+    VREN_KEYS_SLICED = 1,    /* 8b tile x | 8b tile y | 8b log slice over [near, far] in `depth_slices` | 3b face (0..5, 7 = null) */
+};
+/* cluster AABB used by light assignment */
+enum {
+    VREN_CLUSTER_AABB_REFERENCE = 0, /* two corners as written in clustered_shading.glsl:193-231 (y range inverted) */
+    VREN_CLUSTER_AABB_TIGHT = 1,     /* min/max over the 8 corners of the frustum slice */
+};
+/* Morton code domain */
+enum {
+    VREN_MORTON_LIGHTS_AABB = 0, /* AABB of the view-space light positions (clustered_shading.cpp:137-224) */
+    VREN_MORTON_GIVEN_AABB = 1,  /* caller-given view-space AABB (e.g. the scene bounds) */
+};
+/* point-light attenuation in shade */
+enum {
+    VREN_ATTEN_REFERENCE = 0, /* 1 - min(d / r, 1)^1.5 (pbr.glsl:89) */
+    VREN_ATTEN_SMOOTH = 1,    /* (1 - (d / r)^4)_+^2, differentiable everywhere */
+};
+
+typedef struct vren_cluster_cfg {
+    uint32_t key_mode;          /* VREN_KEYS_* */
+    uint32_t depth_slices;      /* VREN_KEYS_SLICED only (1..256) */
+    uint32_t cluster_aabb_mode; /* VREN_CLUSTER_AABB_* */
+    uint32_t morton_bits;       /* bits per axis: 5 (reference, 15-bit codes) .. 10 (30-bit codes) */
+    uint32_t morton_aabb_mode;  /* VREN_MORTON_* */
+    float morton_aabb_min[3];   /* view space, VREN_MORTON_GIVEN_AABB only */
+    float morton_aabb_max[3];
+    uint32_t attenuation;       /* VREN_ATTEN_* */
+} vren_cluster_cfg;
+
+typedef struct vren_ctx vren_ctx;
+
+VREN_EXPORT int vren_create(int hip_device, vren_ctx** out);
+VREN_EXPORT void vren_destroy(vren_ctx* ctx);
+VREN_EXPORT const char* vren_status_string(int status);
+/* last host-side error message of this context (argument validation / HIP error text) */
+VREN_EXPORT const char* vren_last_error(vren_ctx* ctx);
+/* synchronise `stream` and report (then clear) device-side capacity flags raised by calls on this ctx */
+VREN_EXPORT int vren_sync_status(vren_ctx* ctx, vren_stream_t stream);
+/* the reference's default configuration (reference keys / AABB / 15-bit Morton / reference attenuation) */
+VREN_EXPORT void vren_cluster_cfg_default(vren_cluster_cfg* cfg);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Parallel primitives (vren/vren/primitives)
+ * ------------------------------------------------------------------------------------------------------- */
+
+/* reduce<uint, op>::operator() (reduce.hpp:15-49, reduce.cpp:32-114): writes the in-place Blelloch up-sweep
+ * tree of each of `blocks` segments. Segment b reads in[b * n ...], padded with the op identity to
+ * p = next_pow2(n), and writes out_tree[b * p ... b * p + p); the segment total is at out_tree[b * p + p - 1].
+ * in == out_tree (in-place overload, reduce.cpp:116-128) is allowed when blocks == 1 or n is a power of 2. */
+VREN_EXPORT int vren_reduce_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in, uint32_t n,
+                                uint32_t* out_tree, int op, uint32_t blocks);
+/* reduce<vec4, op>: same over float4 elements (16 bytes); identities 0, +1e35, -1e35 (VRen.cmake:70-75) */
+VREN_EXPORT int vren_reduce_vec4(vren_ctx* ctx, vren_stream_t stream, const float* in, uint32_t n,
+                                 float* out_tree, int op, uint32_t blocks);
+/* scalar reduction of n uint32 (no tree): *out = op over in[0..n) (mod 2^32 for add) */
+VREN_EXPORT int vren_reduce_total_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in, uint64_t n,
+                                      uint32_t* out, int op);
+
+/* blelloch_scan::operator() (blelloch_scan.hpp:9-49, blelloch_scan.cpp:141-166): exclusive prefix sum
+ * mod 2^32 of each of `blocks` contiguous segments of n elements. in == out (in place) is allowed.
+ * Any n >= 1 (the reference requires a power of 2). */
+VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in, uint32_t* out,
+                                        uint64_t n, uint32_t blocks);
+
+/* radix_sort (radix_sort.hpp:9-53, radix_sort.cpp:149-338): stable LSD sort of uint32 keys (and, if vals is
+ * not NULL, uint32 values carried with them) over key bits [begin_bit, end_bit). The result is left in
+ * keys/vals, as the reference leaves it in `buffer`. scratch: vren_radix_sort_scratch_bytes(n, vals != NULL). */
+VREN_EXPORT size_t vren_radix_sort_scratch_bytes(uint64_t n, int with_values);
+VREN_EXPORT int vren_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint32_t* keys, uint32_t* vals, uint64_t n,
+                                uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes);
+
+/* Radix-sort autotuner. No reference counterpart (the reference's digit width is fixed at 4 bits,
+ * radix_sort.hpp:15-16). Configurations are (digit bits, keys per thread, waves per workgroup); every
+ * configuration gives the same (stable) output. vren_autotune_radix_sort times them all on pseudo-random keys of
+ * n elements and remembers the fastest for that size class in the context. */
+VREN_EXPORT int vren_radix_sort_config_count(void);
+VREN_EXPORT int vren_radix_sort_config_info(int config, uint32_t* radix_bits, uint32_t* keys_per_thread,
+                                            uint32_t* waves_per_block);
+VREN_EXPORT int vren_radix_sort_force_config(vren_ctx* ctx, int config); /* -1 = automatic */
+VREN_EXPORT int vren_autotune_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint64_t n, int with_values,
+                                         uint32_t key_bits, int* best_config, float* best_ms);
+VREN_EXPORT int vren_autotune_save(vren_ctx* ctx, const char* path);
+VREN_EXPORT int vren_autotune_load(vren_ctx* ctx, const char* path);
+
+/* bucket_sort::operator() (bucket_sort.hpp:9-45, bucket_sort.cpp:72-161): counting sort of (key, value)
+ * uint32 pairs into n_buckets (a power of 2, <= 65536) buckets by (key >> key_shift) & (n_buckets - 1).
+ * Stable (the reference's atomic scatter is not). bucket_offsets[n_buckets] receives the exclusive bucket
+ * starts (the reference appends them to its output buffer, bucket_sort.cpp:67-70). */
+VREN_EXPORT int vren_bucket_sort(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in_pairs, uint32_t n,
+                                 uint32_t* out_pairs, uint32_t* bucket_offsets, uint32_t n_buckets,
+                                 uint32_t key_shift);
+
+/* calc_bvh_* (build_bvh.hpp:52-56, build_bvh.cpp:101-136), host only */
+VREN_EXPORT uint32_t vren_calc_bvh_padded_leaf_count(uint32_t leaf_count);
+VREN_EXPORT uint32_t vren_calc_bvh_buffer_length(uint32_t leaf_count);
+VREN_EXPORT size_t vren_calc_bvh_buffer_size(uint32_t leaf_count);
+VREN_EXPORT uint32_t vren_calc_bvh_root_index(uint32_t leaf_count);
+VREN_EXPORT uint32_t vren_calc_bvh_level_count(uint32_t leaf_count);
+
+/* build_bvh::operator() (build_bvh.cpp:38-99): builds the parent levels of a 32-ary BVH whose
+ * padded_leaf_count (a power of 32, >= 32) leaves are already in nodes[0 .. padded_leaf_count). A parent's
+ * AABB is the min/max over its valid children, next = its first child, or VREN_BVH_INVALID_NODE with the
+ * empty AABB (min = +inf, max = -inf) when no child is valid. */
+VREN_EXPORT int vren_build_bvh(vren_ctx* ctx, vren_stream_t stream, vren_bvh_node* nodes,
+                               uint32_t padded_leaf_count);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Clustered shading (vren/vren/pipeline/clustered_shading.{hpp,cpp})
+ * ------------------------------------------------------------------------------------------------------- */
+
+/* camera::get_view / get_projection (camera.cpp:32-50), column-major float[16] */
+VREN_EXPORT void vren_camera_matrices(const vren_camera* camera, float view[16], float proj[16]);
+
+/* construct_point_light_bvh::operator() (clustered_shading.cpp:60-346):
+ *   view_positions[L] (float4) = view * (p, 1);  light_codes[L] (uint2) = (Morton code, light index) stably
+ *   sorted by code;  bvh[vren_calc_bvh_buffer_length(L)] = 32-ary BVH over the sorted lights (leaf AABB =
+ *   view position -+ radius; padding leaves invalid). point_lights = {float color[3]; float intensity}[L]
+ *   where intensity is the influence radius (init_light_array_bvh.comp:52-53). */
+VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stream, const float* light_positions,
+                                               const float* point_lights, uint32_t light_count,
+                                               const vren_camera* camera, const vren_cluster_cfg* cfg,
+                                               float* view_positions, uint32_t* light_codes, vren_bvh_node* bvh);
+
+/* find_unique_cluster_list::operator() (clustered_shading.cpp:363-448): per-pixel cluster keys from the D32
+ * depth and RGBA16F normals, unique keys per 32x32 tile in ascending order, tiles in row-major order.
+ * cluster_keys[key_capacity] receives the unique keys, *cluster_count (device) their number, and
+ * cluster_refs[W*H] each pixel's index into cluster_keys. */
+VREN_EXPORT int vren_find_unique_cluster_list(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                              const vren_camera* camera, const vren_cluster_cfg* cfg,
+                                              const float* depth, const uint16_t* normals, uint32_t* cluster_keys,
+                                              uint32_t key_capacity, uint32_t* cluster_count,
+                                              uint32_t* cluster_refs);
+
+/* assign_lights::operator() (clustered_shading.cpp:473-690): per unique cluster, traverse the light BVH and
+ * write the indices of the lights whose sphere meets the cluster AABB, each list in ascending light index.
+ * light_counts / light_offsets [key_capacity], light_indices[index_capacity], *assigned_total (device). */
+VREN_EXPORT int vren_assign_lights(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                   const vren_camera* camera, const vren_cluster_cfg* cfg,
+                                   const uint32_t* cluster_keys, const uint32_t* cluster_count, uint32_t key_capacity,
+                                   const vren_bvh_node* bvh, uint32_t light_count, const uint32_t* light_codes,
+                                   const float* view_positions, uint32_t* light_counts, uint32_t* light_offsets,
+                                   uint32_t* light_indices, uint32_t index_capacity, uint32_t* assigned_total);
+
+/* shade::operator() (clustered_shading.cpp:709-811, shade.comp:69-141) with resolved per-pixel materials:
+ * albedo_roughness[W*H] = (r, g, b, roughness), metallic[W*H]. Writes RGBA32F output for every pixel whose
+ * view depth is below far - 0.001 (background pixels are left untouched, as in the reference). */
+VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                           const vren_camera* camera, const vren_cluster_cfg* cfg, const float* depth,
+                           const uint16_t* normals, const float* albedo_roughness, const float* metallic,
+                           const uint32_t* cluster_refs, uint32_t key_capacity, const uint32_t* light_counts,
+                           const uint32_t* light_offsets, const uint32_t* light_indices, uint32_t index_capacity,
+                           const float* light_positions, const float* point_lights, uint32_t light_count,
+                           const float* directional_lights, uint32_t directional_light_count, float* output);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VREN_HIP_CAPI_H_ */

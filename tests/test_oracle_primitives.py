@@ -1,0 +1,231 @@
+"""Pins of the CPU oracle's primitives, independent of the oracle's own code.
+
+Each check uses something the oracle does not: the published PCG32 demo vector, numpy's own sort/scan/min/max,
+the closed form of the Blelloch up-sweep tree, the build_bvh.utils known answers of the reference's test
+(vren_test/vren_test/primitives/build_bvh.cpp:223-254) and its point-query property test (:175-210).
+"""
+import math
+
+import numpy as np
+import pytest
+
+BVH_DTYPE = np.dtype([("min", "<f4", 3), ("next", "<u4"), ("max", "<f4", 3), ("pad", "<u4")])
+
+
+def test_pcg32_published_vector(oracle):
+    # pcg32-demo, initstate 42, initseq 54 (O'Neill's reference implementation output)
+    expect = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+    assert list(oracle.pcg32(42, 54, 6)) == expect
+
+
+def test_pcg32_jump_ahead_matches_sequential(oracle):
+    seq = oracle.pcg32(1042, 55, 5000)
+    for start in (0, 1, 17, 4095):
+        jmp = oracle.pcg32(1042, 55, 64, start=start, jump=True)
+        assert np.array_equal(jmp, seq[start:start + 64])
+
+
+def _ulp_err(got, ref):
+    ref32 = np.float32(ref)
+    return abs(float(got) - ref) / float(np.spacing(np.abs(ref32)))
+
+
+def test_vmath_logf_error_bound(oracle):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(1e-3, 1e3, 4000), 1.0 + rng.uniform(-1e-3, 1e-3, 2000),
+                         [1e-40, 1e-45, 3.4e38, 2.0, 0.5, 1.0000001]]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        r = math.log(float(x))
+        g = oracle.logf(x)
+        if r == 0.0:
+            assert g == 0.0
+            continue
+        worst = max(worst, _ulp_err(g, r))
+    assert worst <= 2.0, worst
+    assert oracle.logf(1.0) == 0.0
+    assert math.isinf(oracle.logf(0.0)) and oracle.logf(0.0) < 0
+    assert math.isnan(oracle.logf(-1.0))
+
+
+def test_vmath_expf_and_powi(oracle):
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for x in rng.uniform(-30, 30, 4000).astype(np.float32):
+        worst = max(worst, _ulp_err(oracle.expf(x), math.exp(float(x))))
+    assert worst <= 2.0, worst
+    # a^k by square-and-multiply: exact for powers of two, within k ulp-ish otherwise
+    assert oracle.powi(2.0, 10) == 1024.0
+    assert oracle.powi(1.5, 0) == 1.0
+    a = np.float32(1.0 + 2.0 / 34.0)
+    for k in (1, 7, 33, 121, 400):
+        rel = abs(oracle.powi(a, k) - float(a) ** k) / float(a) ** k
+        assert rel < 1e-6 * max(1, math.log2(k)), (k, rel)
+
+
+def _tree_closed_form(x, op, ident):
+    """tree[i] = op over the aligned block of 2^ctz(i+1) elements ending at i (input padded with ident)."""
+    n = x.size
+    p = 1 << max(0, (n - 1).bit_length())
+    xp = np.full(p, ident, dtype=x.dtype)
+    xp[:n] = x
+    out = np.empty(p, dtype=x.dtype)
+    for i in range(p):
+        k = ((i + 1) & -(i + 1)).bit_length() - 1
+        blk = xp[i + 1 - (1 << k): i + 1]
+        out[i] = op(blk)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 10, 1000, 10000])
+def test_reduce_tree_u32_closed_form(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    for op, fn, ident in [(0, lambda b: np.uint32(np.sum(b, dtype=np.uint64) & 0xFFFFFFFF), 0),
+                          (1, np.min, 0xFFFFFFFF), (2, np.max, 0)]:
+        got = oracle.reduce_tree_u32(x, op)
+        want = _tree_closed_form(x, fn, ident) if n <= 1000 else None
+        if want is not None:
+            assert np.array_equal(got, want), op
+        assert got[-1] == fn(x), op
+        assert got[-1] == oracle.reduce_total_u32(x, op)
+
+
+def test_reduce_tree_u32_all_ones_reference_case(oracle):
+    # reduce.type_uint: N = 10000 ones, padded to 16384 (vren_test/.../reduce.cpp:256-282)
+    x = np.ones(10000, np.uint32)
+    t = oracle.reduce_tree_u32(x, 0)
+    assert t.size == 16384 and t[-1] == 10000
+    i = np.arange(16384)
+    block = (i + 1) & -(i + 1)
+    lo = i + 1 - block
+    want = np.clip(i + 1, 0, 10000) - np.clip(lo, 0, 10000)
+    assert np.array_equal(t, want.astype(np.uint32))
+
+
+def test_reduce_tree_multi_block(oracle):
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 1000, 3 * 100, dtype=np.uint32)
+    t = oracle.reduce_tree_u32(x, 0, blocks=3)
+    add = lambda b: np.uint32(np.sum(b, dtype=np.uint64) & 0xFFFFFFFF)
+    for b in range(3):
+        assert np.array_equal(t[b * 128:(b + 1) * 128], _tree_closed_form(x[b * 100:(b + 1) * 100], add, 0))
+
+
+def test_reduce_tree_vec4_integer_valued(oracle):
+    # integer-valued floats make the sum order-independent (as in reduce.type_vec4, reduce.cpp:174-196)
+    rng = np.random.default_rng(7)
+    x = rng.integers(0, 100, (1000, 4)).astype(np.float32)
+    for op, fn in [(0, np.sum), (1, np.min), (2, np.max)]:
+        t = oracle.reduce_tree_vec4(x, op)
+        assert t.shape == (1024, 4)
+        assert np.array_equal(t[-1], fn(x, axis=0).astype(np.float32)), op
+    # identities of the padding (VRen.cmake:70-75)
+    one = oracle.reduce_tree_vec4(np.array([[1, 2, 3, 4]], np.float32), 1)
+    assert np.array_equal(one[0], [1, 2, 3, 4])
+    t = oracle.reduce_tree_vec4(np.zeros((5, 4), np.float32), 1)
+    assert t[6, 0] == np.float32(1e35) and t[5, 0] == 0.0 and t[7, 0] == 0.0
+    t = oracle.reduce_tree_vec4(np.zeros((5, 4), np.float32), 2)
+    assert t[6, 0] == np.float32(-1e35)
+
+
+def test_exclusive_scan_vs_numpy(oracle):
+    rng = np.random.default_rng(3)
+    for n in (1, 7, 4096, 100003):
+        x = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+        want = np.concatenate([[0], np.cumsum(x.astype(np.uint64))[:-1]]).astype(np.uint64) & 0xFFFFFFFF
+        assert np.array_equal(oracle.exclusive_scan_u32(x), want.astype(np.uint32))
+    ones = np.ones(1 << 12, np.uint32)
+    assert np.array_equal(oracle.exclusive_scan_u32(ones), np.arange(1 << 12, dtype=np.uint32))
+
+
+def test_stable_sort_vs_numpy(oracle):
+    rng = np.random.default_rng(4)
+    k = rng.integers(0, 64, 5000, dtype=np.uint32)  # many ties
+    v = np.arange(5000, dtype=np.uint32)
+    ks, vs = oracle.stable_sort_kv(k, v)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(ks, k[order]) and np.array_equal(vs, v[order])
+    # bit range [4, 12)
+    k2 = rng.integers(0, 2 ** 32, 3000, dtype=np.uint64).astype(np.uint32)
+    ks, vs = oracle.stable_sort_kv(k2, np.arange(3000, dtype=np.uint32), 4, 12)
+    order = np.argsort((k2 >> 4) & 0xFF, kind="stable")
+    assert np.array_equal(vs, order.astype(np.uint32))
+
+
+def test_bucket_sort_vs_numpy(oracle):
+    rng = np.random.default_rng(6)
+    pairs = np.stack([rng.integers(0, 2 ** 32, 7000, dtype=np.uint64).astype(np.uint32),
+                      np.arange(7000, dtype=np.uint32)], axis=1)
+    for nb, shift in [(65536, 0), (64, 24)]:
+        out, offs = oracle.bucket_sort(pairs, nb, shift)
+        b = (pairs[:, 0] >> shift) & (nb - 1)
+        order = np.argsort(b, kind="stable")
+        assert np.array_equal(out, pairs[order])
+        cnt = np.bincount(b, minlength=nb)
+        assert np.array_equal(offs, np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.uint32))
+
+
+def test_calc_bvh_known_answers(oracle):
+    # build_bvh.utils (vren_test/vren_test/primitives/build_bvh.cpp:223-254)
+    for n, v in [(0, 32), (1, 32), (17, 32), (32, 32), (129, 1024), (582, 1024), (1024, 1024), (2193819, 33554432)]:
+        assert oracle.calc("padded_leaf_count", n) == v
+    L3 = 32 * 32 + 32 + 1
+    L5 = 32 ** 5 + 32 ** 4 + 32 ** 3 + 32 ** 2 + 32 + 1
+    for n, v in [(0, 33), (1, 33), (17, 33), (32, 33), (129, L3), (582, L3), (1024, L3), (2193819, L5)]:
+        assert oracle.calc("buffer_length", n) == v
+    for n, v in [(0, 32), (1, 32), (129, L3 - 1), (582, L3 - 1), (2193819, L5 - 1)]:
+        assert oracle.calc("root_index", n) == v
+    for n, v in [(0, 1), (1, 1), (129, 2), (582, 2), (2193819, 5)]:
+        assert oracle.calc("level_count", n) == v
+
+
+def random_leaves(n, padded, length, rng):
+    nodes = np.zeros(length, BVH_DTYPE)
+    pos = rng.uniform(0, 100, (n, 3)).astype(np.float32)
+    ext = rng.uniform(0, 100, (n, 3)).astype(np.float32)
+    nodes["min"][:n] = pos
+    nodes["max"][:n] = pos + ext
+    nodes["next"][:n] = 0xFFFFFFFF
+    nodes["min"][n:padded] = np.inf
+    nodes["max"][n:padded] = -np.inf
+    nodes["next"][n:padded] = 0xFFFFFFFE
+    return nodes
+
+
+@pytest.mark.parametrize("n", [1, 10, 100, 1000, 10000])
+def test_build_bvh_levels_and_point_queries(oracle, n):
+    rng = np.random.default_rng(n)
+    padded = oracle.calc("padded_leaf_count", n)
+    nodes = random_leaves(n, padded, oracle.calc("buffer_length", n), rng)
+    built = oracle.build_bvh(nodes, padded)
+    # every parent = min/max over its valid children; next = first child or invalid
+    start, size = 0, padded
+    while size > 1:
+        ch = built[start:start + size].reshape(-1, 32)
+        par = built[start + size:start + size + size // 32]
+        valid = ch["next"] != 0xFFFFFFFE
+        anyv = valid.any(axis=1)
+        mn = np.where(valid[..., None], ch["min"], np.inf).min(axis=1)
+        mx = np.where(valid[..., None], ch["max"], -np.inf).max(axis=1)
+        assert np.array_equal(par["min"][anyv], mn[anyv]) and np.array_equal(par["max"][anyv], mx[anyv])
+        assert np.all(par["next"][~anyv] == 0xFFFFFFFE)
+        assert np.array_equal(par["next"][anyv], (start + 32 * np.arange(size // 32))[anyv].astype(np.uint32))
+        start, size = start + size, size // 32
+    # the reference's property test: BVH-accelerated point query == linear scan over all nodes
+    root = oracle.calc("root_index", n)
+    for _ in range(8):
+        p = rng.uniform(0, 100, 3).astype(np.float32)
+        inside = np.all((built["min"] <= p) & (p <= built["max"]), axis=1)
+        want = sorted(np.nonzero(inside)[0].tolist())
+        got = []
+        stack = [root]
+        while stack:
+            i = stack.pop()
+            nd = built[i]
+            if not (np.all(nd["min"] <= p) and np.all(p <= nd["max"])):
+                continue
+            got.append(i)
+            if nd["next"] not in (0xFFFFFFFF, 0xFFFFFFFE):
+                stack.extend(range(int(nd["next"]), int(nd["next"]) + 32))
+        assert sorted(got) == want

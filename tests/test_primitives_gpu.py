@@ -1,0 +1,153 @@
+"""GPU parity of the primitives through the C ABI, against the CPU oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+from gpu_util import empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+N_C1 = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def c1_keys(oracle):
+    # config 1: 2^20 uint32 keys from pcg32(initstate 42, initseq 54)
+    return oracle.pcg32(42, 54, N_C1)
+
+
+@pytest.mark.parametrize("n", [1, 10, 1000, 10000, 100000, N_C1])
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_reduce_u32_tree(vctx, oracle, c1_keys, n, op):
+    x = c1_keys[:n] if op != 0 or n != 10000 else np.ones(n, np.uint32)
+    p = 1 << max(0, (n - 1).bit_length())
+    out = empty_dev(4 * p, fill=0xAB)
+    vctx.reduce_u32(to_dev(x), n, out, op)
+    got = from_dev(out, np.uint32, (p,))
+    assert np.array_equal(got, oracle.reduce_tree_u32(x, op))
+
+
+def test_reduce_u32_multi_block_and_in_place(vctx, oracle, c1_keys):
+    x = c1_keys[:3 * 1000]
+    out = empty_dev(4 * 3 * 1024)
+    vctx.reduce_u32(to_dev(x), 1000, out, 0, blocks=3)
+    assert np.array_equal(from_dev(out, np.uint32, (3 * 1024,)), oracle.reduce_tree_u32(x, 0, blocks=3))
+    y = c1_keys[:4096]
+    d = to_dev(y)
+    vctx.reduce_u32(d, 4096, d, 0)
+    assert np.array_equal(from_dev(d, np.uint32, (4096,)), oracle.reduce_tree_u32(y, 0))
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_reduce_vec4_tree_bit_exact(vctx, oracle, op):
+    # random (non-integer) floats: bit-exact only if the combination tree is the reference's
+    rng = np.random.default_rng(11)
+    for n in (1, 100, 10000):
+        x = rng.uniform(-100, 100, (n, 4)).astype(np.float32)
+        p = 1 << max(0, (n - 1).bit_length())
+        out = empty_dev(16 * p)
+        vctx.reduce_vec4(to_dev(x), n, out, op)
+        got = from_dev(out, np.float32, (p, 4))
+        assert np.array_equal(got.view(np.uint32), oracle.reduce_tree_vec4(x, op).view(np.uint32)), n
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_reduce_total(vctx, oracle, c1_keys, op):
+    for n in (1, 5, 4099, N_C1):
+        out = empty_dev(4)
+        vctx.reduce_total_u32(to_dev(c1_keys[:n]), n, out, op)
+        assert from_dev(out, np.uint32)[0] == oracle.reduce_total_u32(c1_keys[:n], op)
+
+
+@pytest.mark.parametrize("n", [1, 7, 4096, 4097, 100003, N_C1])
+def test_exclusive_scan(vctx, oracle, c1_keys, n):
+    x = c1_keys[:n]
+    out = empty_dev(4 * n)
+    vctx.exclusive_scan_u32(to_dev(x), out, n)
+    assert np.array_equal(from_dev(out, np.uint32, (n,)), oracle.exclusive_scan_u32(x))
+
+
+def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, c1_keys):
+    x = c1_keys[:3 * 5000]
+    d = to_dev(x)
+    vctx.exclusive_scan_u32(d, d, 5000, blocks=3)
+    assert np.array_equal(from_dev(d, np.uint32, (15000,)), oracle.exclusive_scan_u32(x, blocks=3))
+    for e in range(0, 20):  # blelloch_scan.main: all ones, N = 2^0 .. 2^19
+        n = 1 << e
+        o = empty_dev(4 * n)
+        vctx.exclusive_scan_u32(to_dev(np.ones(n, np.uint32)), o, n)
+        assert np.array_equal(from_dev(o, np.uint32, (n,)), np.arange(n, dtype=np.uint32))
+
+
+def _sort_gpu(vctx, keys, vals, begin=0, end=32):
+    n = keys.size
+    dk = to_dev(keys)
+    dv = to_dev(vals) if vals is not None else None
+    scratch = empty_dev(vctx.radix_sort_scratch_bytes(n, vals is not None))
+    vctx.radix_sort(dk, dv, n, scratch, begin, end)
+    k = from_dev(dk, np.uint32, (n,))
+    v = from_dev(dv, np.uint32, (n,)) if dv is not None else None
+    return k, v
+
+
+def test_radix_sort_c1_kv(vctx, oracle, c1_keys):
+    vals = np.arange(N_C1, dtype=np.uint32)
+    k, v = _sort_gpu(vctx, c1_keys, vals)
+    ok, ov = oracle.stable_sort_kv(c1_keys, vals)
+    assert np.array_equal(k, ok) and np.array_equal(v, ov)
+
+
+def test_radix_sort_reference_case_and_edges(vctx, oracle, c1_keys):
+    # radix_sort.main: N = 1024, reversed iota, vs std::sort
+    k, _ = _sort_gpu(vctx, np.arange(1024, dtype=np.uint32)[::-1].copy(), None)
+    assert np.array_equal(k, np.arange(1024, dtype=np.uint32))
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 33, 5000, N_C1 + 7):
+        keys = rng.integers(0, 50, n, dtype=np.uint32) if n == 5000 else rng.integers(0, 2 ** 32, n,
+                                                                                        dtype=np.uint64).astype(np.uint32)
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = _sort_gpu(vctx, keys, vals)
+        ok, ov = oracle.stable_sort_kv(keys, vals)
+        assert np.array_equal(k, ok) and np.array_equal(v, ov), n
+    keys = c1_keys[:70000]
+    for b, e in ((0, 15), (4, 12), (16, 32), (0, 30)):
+        k, v = _sort_gpu(vctx, keys, np.arange(keys.size, dtype=np.uint32), b, e)
+        ok, ov = oracle.stable_sort_kv(keys, np.arange(keys.size, dtype=np.uint32), b, e)
+        assert np.array_equal(v, ov), (b, e)
+
+
+def test_radix_sort_every_config_identical(vctx, oracle, c1_keys):
+    keys = c1_keys[:100003]
+    vals = np.arange(keys.size, dtype=np.uint32)
+    ok, ov = oracle.stable_sort_kv(keys, vals)
+    configs = vctx.radix_configs()
+    try:
+        for i, cfg in enumerate(configs):
+            vctx.radix_force_config(i)
+            k, v = _sort_gpu(vctx, keys, vals)
+            assert np.array_equal(k, ok) and np.array_equal(v, ov), cfg
+            k, _ = _sort_gpu(vctx, keys, None)
+            assert np.array_equal(k, ok), cfg
+    finally:
+        vctx.radix_force_config(-1)
+
+
+def test_radix_sort_headline_size_properties(vctx, oracle):
+    # 2^26 pairs (the headline size): checked through size-independent properties on the device
+    import torch
+    n = 1 << 26
+    keys = oracle.pcg32(42, 54, n)
+    dk = to_dev(keys).view(torch.int32)
+    orig = dk.clone()
+    dv = torch.arange(n, dtype=torch.int32, device="cuda")
+    scratch = empty_dev(vctx.radix_sort_scratch_bytes(n, True))
+    vctx.radix_sort(dk, dv, n, scratch)
+    torch.cuda.synchronize()
+    ku = dk.to(torch.int64) & 0xFFFFFFFF
+    assert bool((ku[1:] >= ku[:-1]).all())                                   # sorted
+    seen = torch.zeros(n, dtype=torch.int32, device="cuda")
+    seen.index_add_(0, dv.long(), torch.ones(n, dtype=torch.int32, device="cuda"))
+    assert bool((seen == 1).all())                                           # values are a permutation
+    assert bool((orig[dv.long()] == dk).all())                               # each key travelled with its value
+    eq = ku[1:] == ku[:-1]
+    assert bool((dv[1:][eq] > dv[:-1][eq]).all())                            # stable: ties keep input order
+    del seen, orig

@@ -1,0 +1,123 @@
+"""vren_amd -- MI355X (gfx950) implementation of vren's clustered-shading compute path.
+
+The product is the C ABI in include/vren_hip/capi.h, implemented by libvren_amd.so (hand-written HIP kernels),
+with the drop-in C++ operator classes in include/vren/. This package loads that library with ctypes and wraps
+each entry point so that device buffers can be passed as torch tensors (torch provides device memory and
+streams only). Every call goes to the library; there is no other implementation to fall back to.
+"""
+import ctypes
+
+from . import _lib
+from ._lib import (ATTEN_REFERENCE, ATTEN_SMOOTH, BVH_INVALID_NODE, BVH_LEAF_NODE, CLUSTER_AABB_REFERENCE,
+                   CLUSTER_AABB_TIGHT, KEYS_REFERENCE, KEYS_SLICED, MORTON_GIVEN_AABB, MORTON_LIGHTS_AABB,
+                   REDUCE_ADD, REDUCE_MAX, REDUCE_MIN, Camera, ClusterCfg, FrameDesc, FrameOutputs, VrenError)
+
+__all__ = ["Context", "Camera", "ClusterCfg", "FrameDesc", "FrameOutputs", "VrenError", "lib", "default_cfg",
+           "REDUCE_ADD", "REDUCE_MIN", "REDUCE_MAX", "KEYS_REFERENCE", "KEYS_SLICED", "CLUSTER_AABB_REFERENCE",
+           "CLUSTER_AABB_TIGHT", "MORTON_LIGHTS_AABB", "MORTON_GIVEN_AABB", "ATTEN_REFERENCE", "ATTEN_SMOOTH",
+           "BVH_LEAF_NODE", "BVH_INVALID_NODE"]
+
+
+def lib():
+    return _lib.load()
+
+
+def default_cfg(**overrides):
+    cfg = ClusterCfg()
+    lib().vren_cluster_cfg_default(ctypes.byref(cfg))
+    for k, v in overrides.items():
+        if k in ("morton_aabb_min", "morton_aabb_max"):
+            for i in range(3):
+                getattr(cfg, k)[i] = v[i]
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """Owns a vren_ctx on one HIP device. Methods enqueue on torch's current stream of that device."""
+
+    def __init__(self, device=0):
+        self.lib = lib()
+        h = ctypes.c_void_p()
+        st = self.lib.vren_create(device, ctypes.byref(h))
+        if st != 0:
+            raise VrenError(st, "vren_create(%d) failed" % device)
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.vren_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def check(self, status):
+        if status != 0:
+            raise VrenError(status, self.lib.vren_last_error(self.h).decode())
+        return status
+
+    def sync_status(self):
+        return self.check(self.lib.vren_sync_status(self.h, self.stream()))
+
+    # ---- primitives -----------------------------------------------------------------------------------------
+    def reduce_u32(self, inp, n, out_tree, op=REDUCE_ADD, blocks=1):
+        return self.check(self.lib.vren_reduce_u32(self.h, self.stream(), _ptr(inp), n, _ptr(out_tree), op, blocks))
+
+    def reduce_vec4(self, inp, n, out_tree, op=REDUCE_ADD, blocks=1):
+        return self.check(self.lib.vren_reduce_vec4(self.h, self.stream(), _ptr(inp), n, _ptr(out_tree), op, blocks))
+
+    def reduce_total_u32(self, inp, n, out, op=REDUCE_ADD):
+        return self.check(self.lib.vren_reduce_total_u32(self.h, self.stream(), _ptr(inp), n, _ptr(out), op))
+
+    def exclusive_scan_u32(self, inp, out, n, blocks=1):
+        return self.check(self.lib.vren_exclusive_scan_u32(self.h, self.stream(), _ptr(inp), _ptr(out), n, blocks))
+
+    def radix_sort_scratch_bytes(self, n, with_values):
+        return self.lib.vren_radix_sort_scratch_bytes(n, 1 if with_values else 0)
+
+    def radix_sort(self, keys, vals, n, scratch, begin_bit=0, end_bit=32):
+        return self.check(self.lib.vren_radix_sort(self.h, self.stream(), _ptr(keys), _ptr(vals), n, begin_bit,
+                                                   end_bit, _ptr(scratch), scratch.numel() * scratch.element_size()))
+
+    def radix_force_config(self, cfg):
+        return self.check(self.lib.vren_radix_sort_force_config(self.h, cfg))
+
+    def autotune_radix_sort(self, n, with_values, key_bits=32):
+        best = ctypes.c_int(-1)
+        ms = ctypes.c_float(0.0)
+        self.check(self.lib.vren_autotune_radix_sort(self.h, self.stream(), n, 1 if with_values else 0, key_bits,
+                                                     ctypes.byref(best), ctypes.byref(ms)))
+        return best.value, ms.value
+
+    @staticmethod
+    def radix_configs():
+        L = lib()
+        out = []
+        for i in range(L.vren_radix_sort_config_count()):
+            b, k, w = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            L.vren_radix_sort_config_info(i, ctypes.byref(b), ctypes.byref(k), ctypes.byref(w))
+            out.append((b.value, k.value, w.value))
+        return out
+
+    def bucket_sort(self, in_pairs, n, out_pairs, offsets, n_buckets=65536, key_shift=0):
+        return self.check(self.lib.vren_bucket_sort(self.h, self.stream(), _ptr(in_pairs), n, _ptr(out_pairs),
+                                                    _ptr(offsets), n_buckets, key_shift))
+
+    def build_bvh(self, nodes, padded_leaf_count):
+        return self.check(self.lib.vren_build_bvh(self.h, self.stream(), _ptr(nodes), padded_leaf_count))

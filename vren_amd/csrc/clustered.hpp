@@ -1,0 +1,67 @@
+// clustered.hpp -- per-frame device descriptor of the clustered-shading chain.
+//
+// Every clustered-shading kernel reads its frame's sizes, camera constants and buffer pointers from a table of
+// FrameDev records in device memory and takes the number of frames in the table, so a single frame (the
+// reference's per-frame call) and a batch of frames run the same code. Constants that the reference recomputes
+// per invocation in GLSL (inverse(P), tan(fov/2), the depth-slice base) are computed once on the host.
+#pragma once
+
+#include "common.hpp"
+
+struct FrameDev {
+    // sizes
+    uint32_t W, H, Sx, Sy;
+    uint32_t tile_base;     // first flattened 32x32 tile of this frame in the table
+    uint32_t light_base;    // first flattened light / BVH-leaf slot (used by batched light kernels)
+    uint32_t leaf_base;
+    uint32_t pad0;
+    // camera constants
+    float view[16];         // camera::get_view()
+    float inv_proj[16];     // inverse(camera::get_projection()), rounded from double
+    double inv_proj_d[16];  // shade reconstruction (double)
+    double inv_view_d[16];
+    double eye[3];
+    float near_plane, far_plane;
+    float slice_a;          // 2 tan(fov/2) / Sy + 1
+    float log_a;            // vren_logf(slice_a)
+    float log_fn;           // vren_logf(far / near)
+    float bg_threshold;     // far - 0.001
+    // configuration
+    uint32_t key_mode, depth_slices, aabb_mode, morton_bits, morton_aabb_mode, attenuation;
+    float morton_min[3], morton_max[3];
+    // lights
+    const float* light_pos;   // float4[L] world
+    const float* lights;      // {rgb, radius}[L]
+    uint32_t L;
+    uint32_t n_dir;
+    const float* dir_lights;  // {dir, pad, rgb, pad}[n_dir]
+    float* view_pos;          // float4[L]
+    uint32_t* codes;          // uint2[L] (code, light index), sorted by code
+    uint32_t* codes_tmp;      // uint2[L] scratch for the sort
+    vren_bvh_node* bvh;
+    uint32_t bvh_padded, bvh_root, bvh_levels, bvh_length;
+    // G-buffer
+    const float* depth;
+    const uint16_t* normals;
+    const float* albedo_rough;
+    const float* metallic;
+    // clusters
+    uint32_t* keys;
+    uint32_t* cluster_count;  // device u32
+    uint32_t* refs;
+    uint32_t key_cap;
+    uint32_t index_cap;
+    // light lists
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint32_t* indices;
+    uint32_t* assigned_total; // device u32
+    uint64_t* offset_status;  // look-back statuses of the offset scan (ceil(key_cap / 4096) words)
+    // output
+    float* out;
+    // error flags
+    uint32_t* flags;
+};
+
+// host: fill the camera/config part of a FrameDev (sizes, matrices, derived constants)
+void vren_frame_setup(FrameDev* f, uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg);

@@ -1,0 +1,181 @@
+// common.hpp -- shared device helpers and host plumbing of the gfx950 library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "../../include/vren_hip/capi.h"
+#include "../../include/vren_hip/vmath.h"
+
+#define VREN_WAVE 64
+
+// ---------------------------------------------------------------------------------------------------------------
+// device: wave64 primitives
+// ---------------------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// popcount of mask bits below this lane
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, VREN_WAVE); }
+
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, unsigned d) { return __shfl_up(v, d, VREN_WAVE); }
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int m) { return __shfl_xor(v, m, VREN_WAVE); }
+
+// inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+    uint32_t lane = lane_id();
+#pragma unroll
+    for (unsigned d = 1; d < VREN_WAVE; d <<= 1) {
+        uint32_t o = shfl_up(v, d);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+    return v;
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 128-bit streaming (non-temporal) load / store of 4 uint32 at a 16-byte aligned address
+__device__ __forceinline__ uint4 ntload4(const void* p)
+{
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void ntstore4(void* p, uint4 q)
+{
+    u32x4 v = {q.x, q.y, q.z, q.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(p));
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// device: agent-scope atomics for cross-workgroup (cross-XCD) publication
+// ---------------------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_agent(uint32_t* p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_or_agent(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// min / max in the total order -0 < +0 (order-independent, so any reduction tree gives the same bits)
+__device__ __host__ __forceinline__ float tmin(float a, float b)
+{
+    if (a < b) return a;
+    if (b < a) return b;
+    return (vren_f32_bits(a) >> 31) ? a : b;
+}
+__device__ __host__ __forceinline__ float tmax(float a, float b)
+{
+    if (a > b) return a;
+    if (b > a) return b;
+    return (vren_f32_bits(a) >> 31) ? b : a;
+}
+
+// look-back status words (64 bit): [63:62] state, [31:0] value
+#define VREN_LB_EMPTY 0ull
+#define VREN_LB_AGG (1ull << 62)
+#define VREN_LB_INCL (2ull << 62)
+#define VREN_LB_STATE(s) ((s) & (3ull << 62))
+
+// device capacity flags (vren_ctx::d_flags)
+#define VREN_FLAG_KEY_CAPACITY 1u
+#define VREN_FLAG_INDEX_CAPACITY 2u
+
+// ---------------------------------------------------------------------------------------------------------------
+// host plumbing
+// ---------------------------------------------------------------------------------------------------------------
+
+struct FrameDev; // clustered.hpp
+
+struct vren_ctx {
+    int device = 0;
+    int num_cus = 256;
+    std::string last_error;
+    uint32_t* d_flags = nullptr;       // device capacity flags
+    // grow-only device scratch for primitives that do not take a caller scratch buffer
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // ring of device frame-descriptor slots for the single-frame stage entry points
+    FrameDev* frame_slots = nullptr;
+    uint32_t frame_slot_next = 0;
+    // radix-sort autotuner: forced configuration (-1 = none) and tuned choices keyed by problem bucket
+    int radix_forced = -1;
+    std::map<uint64_t, int> radix_tuned;
+};
+
+#define VREN_FRAME_SLOTS 256
+
+int vren_fail(vren_ctx* ctx, int status, const char* fmt, ...);
+int vren_hip_fail(vren_ctx* ctx, hipError_t err, const char* what);
+int vren_ensure_scratch(vren_ctx* ctx, size_t bytes);
+
+#define VREN_HIP(ctx, expr)                                                                                            \
+    do {                                                                                                               \
+        hipError_t _e = (expr);                                                                                        \
+        if (_e != hipSuccess) return vren_hip_fail((ctx), _e, #expr);                                                  \
+    } while (0)
+
+#define VREN_LAUNCH_CHECK(ctx) VREN_HIP(ctx, hipGetLastError())
+
+static inline uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+static inline uint32_t next_pow2_u32(uint32_t v)
+{
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    v++;
+    return v;
+}
+static inline uint32_t ilog2_u64(uint64_t v)
+{
+    uint32_t r = 0;
+    while (v > 1) {
+        v >>= 1;
+        r++;
+    }
+    return r;
+}

@@ -1,0 +1,321 @@
+// radix_sort.hip -- radix_sort (radix_sort.hpp:9-53, radix_sort.cpp:149-338) as a onesweep LSD sort for gfx950.
+//
+// The reference runs, per 4-bit digit, six dispatches (fill, local count, 16-block reduce, global offset,
+// 16-block down-sweep, reorder) over N/1024 workgroups, i.e. 8 passes x 3 reads/writes of the keys and ~48
+// dispatches, and its reorder assumes a 32-wide subgroup (radix_sort_reorder.comp:15-17). Here:
+//   1. one histogram kernel reads the keys once and counts every pass's digit (LDS atomics, then agent-scope
+//      global atomics);
+//   2. one kernel per pass: tiles taken from a ticket counter are ranked stably in LDS with wave64 ballot
+//      matching, each tile's per-digit counts are chained across tiles by a per-digit decoupled look-back, and
+//      the tile is scattered through an LDS staging buffer so global writes go out in per-digit runs.
+// Per pass each key (and value) is read once and written once. Digit width, keys per thread and waves per
+// workgroup are template parameters chosen by the autotuner (autotune.hip).
+#include <algorithm>
+#include <cstring>
+
+#include "radix_device.hpp"
+#include "radix_sort.hpp"
+#include "scan_device.hpp"
+
+namespace {
+
+// status word of (tile, digit): [31:30] state, [29:0] count
+#define RS_AGG (1u << 30)
+#define RS_INCL (2u << 30)
+#define RS_MASK ((1u << 30) - 1u)
+
+__global__ void rs_init_kernel(uint32_t* hist, uint32_t n_hist, uint32_t* counters, uint32_t n_counters)
+{
+    for (uint32_t i = threadIdx.x; i < n_hist; i += blockDim.x) hist[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < n_counters; i += blockDim.x) counters[i] = 0u;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint64_t n, uint32_t begin_bit,
+                                                      uint32_t end_bit, uint32_t passes, uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ clear, uint64_t clear_words)
+{
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_h[8 * RADIX];
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) s_h[i] = 0u;
+    // clear the first pass's look-back statuses while we are here
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < clear_words; i += (uint64_t)gridDim.x * 256)
+        clear[i] = 0u;
+    __syncthreads();
+    auto count = [&](uint32_t k) {
+        for (uint32_t p = 0; p < passes; p++) {
+            uint32_t sh = begin_bit + p * BITS;
+            uint32_t w = min((uint32_t)BITS, end_bit - sh);
+            uint32_t d = (k >> sh) & ((1u << w) - 1u);
+            atomicAdd(&s_h[p * RADIX + d], 1u);
+        }
+    };
+    const bool aligned = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
+    const uint64_t n4 = aligned ? n / 4 : 0;
+    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * 256) {
+        uint4 q = k4[i];
+        count(q.x);
+        count(q.y);
+        count(q.z);
+        count(q.w);
+    }
+    for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        count(keys[i]);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) {
+        uint32_t c = s_h[i];
+        if (c) atomic_add_agent(&hist[i], c);
+    }
+}
+
+template <int BITS, int KPT, int WAVES, bool VALS>
+__global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t* __restrict__ keys_in,
+                                                                 const uint32_t* __restrict__ vals_in,
+                                                                 uint32_t* __restrict__ keys_out,
+                                                                 uint32_t* __restrict__ vals_out, uint64_t n,
+                                                                 uint32_t shift, uint32_t width,
+                                                                 const uint32_t* __restrict__ hist,
+                                                                 uint32_t* status, uint32_t* counter,
+                                                                 uint32_t* __restrict__ clear_next)
+{
+    constexpr int THREADS = WAVES * 64;
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = THREADS * KPT;
+    static_assert(THREADS >= RADIX, "one thread per digit");
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[VALS ? TILE : 1];
+    __shared__ uint32_t s_whist[WAVES * RADIX];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+    __shared__ uint32_t s_scan[THREADS / VREN_WAVE];
+    __shared__ uint32_t s_ticket;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = tid / VREN_WAVE;
+    if (tid == 0) s_ticket = atomic_add_agent(counter, 1u);
+    for (uint32_t i = tid; i < (uint32_t)(WAVES * RADIX); i += THREADS) s_whist[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_ticket;
+    const uint64_t tile_start = (uint64_t)tile * TILE;
+    const uint32_t tile_count = (uint32_t)min<uint64_t>((uint64_t)TILE, n - tile_start);
+    if (clear_next) {
+        for (uint32_t i = tid; i < (uint32_t)RADIX; i += THREADS) clear_next[(uint64_t)tile * RADIX + i] = 0u;
+    }
+    const uint32_t mask = (1u << width) - 1u;
+
+    // warp-striped load
+    const uint32_t wbase = wave * KPT * VREN_WAVE;
+    uint32_t key[KPT], val[VALS ? KPT : 1], digit[KPT], local[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        uint32_t pos = wbase + j * VREN_WAVE + lane;
+        if (pos < tile_count) {
+            key[j] = __builtin_nontemporal_load(keys_in + tile_start + pos);
+            if (VALS) val[j] = __builtin_nontemporal_load(vals_in + tile_start + pos);
+        } else {
+            key[j] = 0xFFFFFFFFu;
+            if (VALS) val[j] = 0u;
+        }
+        digit[j] = (key[j] >> shift) & mask;
+    }
+    rank_wave_keys<BITS, KPT>(digit, wbase, tile_count, &s_whist[wave * RADIX], local);
+    __syncthreads();
+    // per digit: exclusive prefix across waves, tile count
+    uint32_t dcount = 0;
+    if (tid < (uint32_t)RADIX) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+            uint32_t c = s_whist[w * RADIX + tid];
+            s_whist[w * RADIX + tid] = run;
+            run += c;
+        }
+        dcount = run;
+    }
+    uint32_t total;
+    uint32_t dstart = block_exclusive_sum<THREADS>(dcount, s_scan, &total);
+    uint32_t prefix = 0;
+    if (tid < (uint32_t)RADIX) {
+        s_start[tid] = dstart;
+        // decoupled look-back on this digit's count
+        uint32_t* my = status + (uint64_t)tile * RADIX + tid;
+        if (tile == 0) {
+            __hip_atomic_store(my, RS_INCL | dcount, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, RS_AGG | dcount, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t t = (int64_t)tile - 1;
+            while (t >= 0) {
+                const uint32_t* p = status + (uint64_t)t * RADIX + tid;
+                uint32_t s = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                while ((s & ~RS_MASK) == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    s = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                prefix += s & RS_MASK;
+                if (s & RS_INCL) break;
+                t--;
+            }
+            __hip_atomic_store(my, RS_INCL | (prefix + dcount), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // global digit base = exclusive scan of this pass's histogram (every tile recomputes it from L2)
+    uint32_t h = tid < (uint32_t)RADIX ? hist[tid] : 0u;
+    uint32_t htot;
+    uint32_t hex = block_exclusive_sum<THREADS>(h, s_scan, &htot);
+    if (tid < (uint32_t)RADIX) s_goff[tid] = hex + prefix - dstart;
+    __syncthreads();
+    // scatter into LDS in tile-local sorted order
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        uint32_t pos = wbase + j * VREN_WAVE + lane;
+        if (pos < tile_count) {
+            uint32_t d = digit[j];
+            uint32_t dst = s_start[d] + s_whist[wave * RADIX + d] + local[j];
+            s_keys[dst] = key[j];
+            if (VALS) s_vals[dst] = val[j];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < tile_count; i += THREADS) {
+        uint32_t k = s_keys[i];
+        uint32_t d = (k >> shift) & mask;
+        uint32_t o = s_goff[d] + i;
+        if (o < n) {
+            keys_out[o] = k;
+            if (VALS) vals_out[o] = s_vals[i];
+        }
+    }
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Layout {
+    size_t tmp_keys, tmp_vals, hist, counters, status_a, status_b, total;
+};
+
+Layout layout_for(uint64_t n, bool vals, uint32_t min_tile, uint32_t max_radix)
+{
+    Layout l;
+    uint64_t tiles = ceil_div_u64(std::max<uint64_t>(n, 1), min_tile);
+    size_t off = 0;
+    l.tmp_keys = off;
+    off += align256(n * 4);
+    l.tmp_vals = off;
+    off += vals ? align256(n * 4) : 0;
+    l.hist = off;
+    off += align256(8 * 256 * 4);
+    l.counters = off;
+    off += 256;
+    l.status_a = off;
+    off += align256(tiles * max_radix * 4);
+    l.status_b = off;
+    off += align256(tiles * max_radix * 4);
+    l.total = off;
+    return l;
+}
+
+// smallest tile among the instantiated configurations (4 waves x 8 keys per thread)
+constexpr uint32_t kMinTile = 4 * 64 * 8;
+
+template <int BITS, int KPT, int WAVES>
+int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t begin_bit,
+             uint32_t end_bit, char* scratch, const Layout& L)
+{
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = WAVES * 64 * KPT;
+    const uint32_t passes = (end_bit - begin_bit + BITS - 1) / BITS;
+    const uint64_t tiles = ceil_div_u64(n, TILE);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + L.hist);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(scratch + L.counters);
+    uint32_t* st[2] = {reinterpret_cast<uint32_t*>(scratch + L.status_a),
+                       reinterpret_cast<uint32_t*>(scratch + L.status_b)};
+    uint32_t* tk = reinterpret_cast<uint32_t*>(scratch + L.tmp_keys);
+    uint32_t* tv = vals ? reinterpret_cast<uint32_t*>(scratch + L.tmp_vals) : nullptr;
+
+    rs_init_kernel<<<1, 256, 0, s>>>(hist, passes * RADIX, counters, passes);
+    VREN_LAUNCH_CHECK(ctx);
+    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
+    rs_hist_kernel<BITS><<<hb, 256, 0, s>>>(keys, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
+    VREN_LAUNCH_CHECK(ctx);
+    uint32_t *ksrc = keys, *vsrc = vals, *kdst = tk, *vdst = tv;
+    for (uint32_t p = 0; p < passes; p++) {
+        uint32_t shift = begin_bit + p * BITS;
+        uint32_t width = std::min<uint32_t>(BITS, end_bit - shift);
+        uint32_t* clear_next = p + 1 < passes ? st[(p + 1) & 1] : nullptr;
+        if (vals) {
+            rs_onesweep_kernel<BITS, KPT, WAVES, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                ksrc, vsrc, kdst, vdst, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next);
+        } else {
+            rs_onesweep_kernel<BITS, KPT, WAVES, false><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                ksrc, nullptr, kdst, nullptr, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next);
+        }
+        VREN_LAUNCH_CHECK(ctx);
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+    }
+    if (ksrc != keys) {
+        VREN_HIP(ctx, hipMemcpyAsync(keys, ksrc, n * 4, hipMemcpyDeviceToDevice, s));
+        if (vals) VREN_HIP(ctx, hipMemcpyAsync(vals, vsrc, n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    return VREN_OK;
+}
+
+typedef int (*sort_fn)(vren_ctx*, hipStream_t, uint32_t*, uint32_t*, uint64_t, uint32_t, uint32_t, char*, const Layout&);
+
+struct SortConfig {
+    int bits, kpt, waves;
+    sort_fn fn;
+};
+
+#define RS_CFG(B, K, W) {B, K, W, &run_sort<B, K, W>}
+const SortConfig kConfigs[] = {
+    RS_CFG(8, 8, 4),  RS_CFG(8, 15, 4), RS_CFG(8, 23, 4), RS_CFG(8, 8, 8),  RS_CFG(8, 15, 8),
+    RS_CFG(8, 23, 8), RS_CFG(8, 8, 16), RS_CFG(6, 8, 4),  RS_CFG(6, 15, 4), RS_CFG(6, 23, 4),
+    RS_CFG(6, 8, 8),  RS_CFG(6, 15, 8), RS_CFG(6, 23, 8), RS_CFG(6, 8, 16), RS_CFG(4, 8, 4),
+    RS_CFG(4, 15, 4), RS_CFG(4, 23, 4), RS_CFG(4, 8, 8),  RS_CFG(4, 15, 8), RS_CFG(4, 23, 8),
+    RS_CFG(4, 8, 16),
+};
+
+} // namespace
+
+int vren_radix_config_count() { return (int)(sizeof(kConfigs) / sizeof(kConfigs[0])); }
+
+void vren_radix_config_info(int i, int* bits, int* kpt, int* waves)
+{
+    *bits = kConfigs[i].bits;
+    *kpt = kConfigs[i].kpt;
+    *waves = kConfigs[i].waves;
+}
+
+int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t* keys, uint32_t* vals, uint64_t n,
+                                uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes)
+{
+    Layout L = layout_for(n, vals != nullptr, kMinTile, 256);
+    if (scratch_bytes < L.total)
+        return vren_fail(ctx, VREN_EINVAL, "radix_sort: scratch of %zu bytes, %zu required", scratch_bytes, L.total);
+    if (begin_bit >= end_bit) return VREN_OK;
+    return kConfigs[cfg].fn(ctx, s, keys, vals, n, begin_bit, end_bit, static_cast<char*>(scratch), L);
+}
+
+extern "C" {
+
+VREN_EXPORT size_t vren_radix_sort_scratch_bytes(uint64_t n, int with_values)
+{
+    return layout_for(n, with_values != 0, kMinTile, 256).total;
+}
+
+VREN_EXPORT int vren_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint32_t* keys, uint32_t* vals, uint64_t n,
+                                uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes)
+{
+    if (!ctx) return VREN_EINVAL;
+    if (!keys || n == 0 || !scratch) return vren_fail(ctx, VREN_EINVAL, "radix_sort: null buffer or n == 0");
+    if (n >= (1ull << 30)) return vren_fail(ctx, VREN_EINVAL, "radix_sort: n must be below 2^30");
+    if (begin_bit > end_bit || end_bit > 32) return vren_fail(ctx, VREN_EINVAL, "radix_sort: bad bit range");
+    int cfg = vren_radix_pick_config(ctx, (hipStream_t)stream, n, vals != nullptr, end_bit - begin_bit);
+    return vren_radix_sort_with_config(ctx, (hipStream_t)stream, cfg, keys, vals, n, begin_bit, end_bit, scratch,
+                                       scratch_bytes);
+}
+
+} // extern "C"
