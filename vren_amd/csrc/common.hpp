@@ -70,24 +70,27 @@ __device__ __forceinline__ void ntstore4(void* p, uint4 q)
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// device: agent-scope atomics for cross-workgroup (cross-XCD) publication
+// device: agent-scope atomics for cross-workgroup (cross-XCD) publication.
+// Look-back status words carry their state and their value in ONE word, so they need no ordering with other
+// memory: relaxed atomics at agent scope (coherent across the XCDs' L2s) are enough, and they avoid the L2
+// write-back / invalidate that release / acquire at agent scope cost on this part.
 // ---------------------------------------------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t ld_acquire_u32(const uint32_t* p)
+__device__ __forceinline__ uint32_t ld_agent_u32(const uint32_t* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint64_t ld_acquire_u64(const uint64_t* p)
+__device__ __forceinline__ uint64_t ld_agent_u64(const uint64_t* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release_u32(uint32_t* p, uint32_t v)
+__device__ __forceinline__ void st_agent_u32(uint32_t* p, uint32_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v)
+__device__ __forceinline__ void st_agent_u64(uint64_t* p, uint64_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t atomic_add_agent(uint32_t* p, uint32_t v)
 {

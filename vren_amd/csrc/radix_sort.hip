@@ -142,22 +142,22 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
         // decoupled look-back on this digit's count
         uint32_t* my = status + (uint64_t)tile * RADIX + tid;
         if (tile == 0) {
-            __hip_atomic_store(my, RS_INCL | dcount, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_agent_u32(my, RS_INCL | dcount);
         } else {
-            __hip_atomic_store(my, RS_AGG | dcount, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_agent_u32(my, RS_AGG | dcount);
             int64_t t = (int64_t)tile - 1;
             while (t >= 0) {
                 const uint32_t* p = status + (uint64_t)t * RADIX + tid;
-                uint32_t s = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t s = ld_agent_u32(p);
                 while ((s & ~RS_MASK) == 0u) {
                     __builtin_amdgcn_s_sleep(1);
-                    s = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    s = ld_agent_u32(p);
                 }
                 prefix += s & RS_MASK;
                 if (s & RS_INCL) break;
                 t--;
             }
-            __hip_atomic_store(my, RS_INCL | (prefix + dcount), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            st_agent_u32(my, RS_INCL | (prefix + dcount));
         }
     }
     // global digit base = exclusive scan of this pass's histogram (every tile recomputes it from L2)

@@ -35,20 +35,20 @@ __device__ __forceinline__ uint32_t wave_lookback(uint64_t* status, uint32_t t, 
 {
     const uint32_t lane = lane_id();
     if (t == 0) {
-        if (lane == 0) st_release_u64(&status[0], VREN_LB_INCL | agg);
+        if (lane == 0) st_agent_u64(&status[0], VREN_LB_INCL | agg);
         return 0u;
     }
-    if (lane == 0) st_release_u64(&status[t], VREN_LB_AGG | agg);
+    if (lane == 0) st_agent_u64(&status[t], VREN_LB_AGG | agg);
     uint32_t prefix = 0;
     int64_t base = (int64_t)t - 1;
     while (true) {
         int64_t idx = base - (int64_t)lane;
         uint64_t s = VREN_LB_INCL; // before the segment start: an inclusive zero
         if (idx >= 0) {
-            s = ld_acquire_u64(&status[idx]);
+            s = ld_agent_u64(&status[idx]);
             while (VREN_LB_STATE(s) == VREN_LB_EMPTY) {
                 __builtin_amdgcn_s_sleep(1);
-                s = ld_acquire_u64(&status[idx]);
+                s = ld_agent_u64(&status[idx]);
             }
         }
         uint64_t incl_mask = ballot(VREN_LB_STATE(s) == VREN_LB_INCL);
@@ -61,6 +61,6 @@ __device__ __forceinline__ uint32_t wave_lookback(uint64_t* status, uint32_t t, 
         prefix += wave_sum(v);
         base -= VREN_WAVE;
     }
-    if (lane == 0) st_release_u64(&status[t], VREN_LB_INCL | (uint32_t)(prefix + agg));
+    if (lane == 0) st_agent_u64(&status[t], VREN_LB_INCL | (uint32_t)(prefix + agg));
     return prefix;
 }
