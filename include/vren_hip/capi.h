@@ -215,6 +215,86 @@ VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint32_t width, 
                            const float* light_positions, const float* point_lights, uint32_t light_count,
                            const float* directional_lights, uint32_t directional_light_count, float* output);
 
+/* shade_backward: adjoint of vren_shade with the light lists held fixed (no reference counterpart; the reference
+ * renders forward only). Given dL/d(output) (RGBA32F, alpha ignored) it ADDS into
+ *   grad_light_positions[L] (float4: d/dx, d/dy, d/dz, 0), grad_point_lights[L] (float4: d/d colour*intensity rgb,
+ *   d/d radius) and grad_albedo_roughness[W*H] (float4: d/d albedo rgb, d/d roughness).
+ * The gradient buffers must be zeroed by the caller before the first accumulation. Use VREN_ATTEN_SMOOTH: the
+ * reference attenuation has a kink at d = r. */
+VREN_EXPORT int vren_shade_backward(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                    const vren_camera* camera, const vren_cluster_cfg* cfg, const float* depth,
+                                    const uint16_t* normals, const float* albedo_roughness, const float* metallic,
+                                    const uint32_t* cluster_refs, uint32_t key_capacity, const uint32_t* light_counts,
+                                    const uint32_t* light_offsets, const uint32_t* light_indices,
+                                    uint32_t index_capacity, const float* light_positions, const float* point_lights,
+                                    uint32_t light_count, const float* grad_output, float* grad_light_positions,
+                                    float* grad_point_lights, float* grad_albedo_roughness);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * cluster_and_shade (clustered_shading.hpp:147-182, clustered_shading.cpp:817-1172) over a batch of frames.
+ * A batch owns every intermediate buffer of its frames (sized per frame, not at the reference's fixed maxima)
+ * and runs the whole chain -- light BVH, unique clusters, light assignment, shade -- either frame by frame
+ * (each frame's launches followed by a stream synchronisation, the reference's submit + fence wait) or for all
+ * frames at once (one launch per stage), optionally replayed from a captured HIP graph. All modes produce the
+ * same bytes per frame.
+ * ------------------------------------------------------------------------------------------------------- */
+typedef struct vren_frame_desc {
+    uint32_t width, height;
+    vren_camera camera;
+    const float* depth;              /* f32[W*H] NDC depth (D32) */
+    const uint16_t* normals;         /* f16x4[W*H] (RGBA16F) */
+    const float* albedo_roughness;   /* f32x4[W*H] */
+    const float* metallic;           /* f32[W*H] */
+    const float* light_positions;    /* f32x4[L] world space */
+    const float* point_lights;       /* {f32 colour[3]; f32 radius}[L] */
+    uint32_t light_count;
+    const float* directional_lights; /* {f32 dir[3]; pad; f32 colour[3]; pad}[D] */
+    uint32_t directional_light_count;
+    float* output;                   /* f32x4[W*H] RGBA32F */
+    uint32_t key_capacity;           /* 0: min(tiles * 1024, 2^20) */
+    uint32_t index_capacity;         /* 0: min(key_capacity * L, 2^24) */
+} vren_frame_desc;
+
+typedef struct vren_frame_outputs {
+    float* view_positions;
+    uint32_t* light_codes;
+    vren_bvh_node* bvh;
+    uint32_t bvh_length;
+    uint32_t* cluster_keys;
+    uint32_t* cluster_count;
+    uint32_t* cluster_refs;
+    uint32_t key_capacity;
+    uint32_t* light_counts;
+    uint32_t* light_offsets;
+    uint32_t* light_indices;
+    uint32_t index_capacity;
+    uint32_t* assigned_total;
+} vren_frame_outputs;
+
+typedef struct vren_batch vren_batch;
+enum {
+    VREN_RUN_PER_FRAME = 0, /* per frame: launch its chain, hipStreamSynchronize */
+    VREN_RUN_BATCHED = 1,   /* every frame in one launch per stage, asynchronous */
+    VREN_RUN_GRAPH = 2,     /* VREN_RUN_BATCHED captured once into a hipGraph, then replayed */
+};
+VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, uint32_t n_frames,
+                                  const vren_cluster_cfg* cfg, vren_batch** out);
+VREN_EXPORT int vren_batch_run(vren_batch* batch, vren_stream_t stream, int mode);
+VREN_EXPORT int vren_batch_frame_outputs(vren_batch* batch, uint32_t frame, vren_frame_outputs* out);
+VREN_EXPORT void vren_batch_destroy(vren_batch* batch);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Synthetic inputs on the device (include/vren_hip/synth.h; no reference counterpart: the reference draws its
+ * inputs from a rasterised glTF scene). initstate = 42 + 1000 * config (+ frame).
+ * ------------------------------------------------------------------------------------------------------- */
+VREN_EXPORT int vren_synth_lights(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t light_count,
+                                  float* light_positions, float* point_lights);
+VREN_EXPORT int vren_synth_frame(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t width,
+                                 uint32_t height, const vren_camera* camera, uint32_t n_boxes, float* depth,
+                                 uint16_t* normals);
+VREN_EXPORT int vren_synth_materials(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint64_t n_pixels,
+                                     float* albedo_roughness, float* metallic);
+
 #ifdef __cplusplus
 }
 #endif

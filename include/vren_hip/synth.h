@@ -99,34 +99,6 @@ VREN_HD float vren_uniform_f32(uint32_t u, float lo, float hi)
     return lo + (hi - lo) * vren_unit_f32(u);
 }
 
-/* exp(x) for |x| < 80: x = n ln2 + r, |r| <= ln2/2, exp(r) by its Taylor series to r^10 / 10! */
-VREN_HD float vren_expf(float x)
-{
-    const float inv_ln2 = 1.44269504088896340736f;
-    const float ln2_hi = 0.693145751953125f;
-    const float ln2_lo = 1.428606765330187e-06f;
-    float n = vren_floorf(x * inv_ln2 + 0.5f);
-    float r = (x - n * ln2_hi) - n * ln2_lo;
-    float p = 2.7557319223985890653e-07f;      /* 1/10! */
-    p = p * r + 2.7557319223985890653e-06f;    /* 1/9! */
-    p = p * r + 2.4801587301587301587e-05f;    /* 1/8! */
-    p = p * r + 1.9841269841269841270e-04f;    /* 1/7! */
-    p = p * r + 1.3888888888888888889e-03f;    /* 1/6! */
-    p = p * r + 8.3333333333333333333e-03f;    /* 1/5! */
-    p = p * r + 4.1666666666666666667e-02f;    /* 1/4! */
-    p = p * r + 1.6666666666666666667e-01f;    /* 1/3! */
-    p = p * r + 0.5f;                          /* 1/2! */
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    int32_t ni = (int32_t)n;
-    /* scale by 2^n in two steps so that 2^n itself never overflows */
-    int32_t n1 = ni / 2;
-    int32_t n2 = ni - n1;
-    float s1 = vren_bits_f32((uint32_t)(n1 + 127) << 23);
-    float s2 = vren_bits_f32((uint32_t)(n2 + 127) << 23);
-    return (p * s1) * s2;
-}
-
 VREN_HD float vren_loguniform_f32(uint32_t u, float lo, float hi)
 {
     float a = vren_logf(lo);

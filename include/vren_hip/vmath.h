@@ -15,7 +15,11 @@
  *   ln(m) = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...),  s = (m - 1) / (m + 1),  |s| <= 0.1716,
  * truncated after s^11 (the first omitted term is below 2^-34 relative). The series coefficients are the
  * exact Taylor coefficients 1/(2k+1); ln 2 is split into a 16-bit head (so e * head is exact) and a tail.
- * Measured error against double-precision log: see tests/test_oracle_math.py (bound asserted there).
+ * Measured error against double-precision log is asserted (<= 2 ulp) by the oracle tests.
+ *
+ * vren_expf: x = n ln2 + r with |r| <= ln2 / 2 (Cody-Waite split of ln 2), exp(r) by its Taylor series to r^10
+ * (exact coefficients 1 / k!), scaled by 2^n in two steps. Used by the sliced cluster-key mode and the synthetic
+ * log-uniform generator.
  *
  * vren_powi: a^k for an unsigned integer k by binary exponentiation (right-to-left square-and-multiply),
  * which is all the path needs (the depth-slice exponent of clustered_shading.glsl:221 is the integer
@@ -38,15 +42,25 @@
 VREN_HD uint32_t vren_f32_bits(float x)
 {
     uint32_t u;
-    memcpy(&u, &x, sizeof(u));
+    __builtin_memcpy(&u, &x, sizeof(u));
     return u;
 }
 
 VREN_HD float vren_bits_f32(uint32_t u)
 {
     float x;
-    memcpy(&x, &u, sizeof(x));
+    __builtin_memcpy(&x, &u, sizeof(x));
     return x;
+}
+
+VREN_HD float vren_sqrtf(float x)
+{
+    return __builtin_sqrtf(x);
+}
+
+VREN_HD float vren_floorf(float x)
+{
+    return __builtin_floorf(x);
 }
 
 /* IEEE binary16 -> binary32, exact (RGBA16F G-buffer normals, gbuffer.cpp:7-13). */
@@ -113,6 +127,34 @@ VREN_HD float vren_logf(float x)
     return fe * ln2_hi + (lnm + fe * ln2_lo);
 }
 
+/* exp(x) for |x| < 80: x = n ln2 + r, |r| <= ln2/2, exp(r) by its Taylor series to r^10 / 10! */
+VREN_HD float vren_expf(float x)
+{
+    const float inv_ln2 = 1.44269504088896340736f;
+    const float ln2_hi = 0.693145751953125f;
+    const float ln2_lo = 1.428606765330187e-06f;
+    float n = vren_floorf(x * inv_ln2 + 0.5f);
+    float r = (x - n * ln2_hi) - n * ln2_lo;
+    float p = 2.7557319223985890653e-07f;      /* 1/10! */
+    p = p * r + 2.7557319223985890653e-06f;    /* 1/9! */
+    p = p * r + 2.4801587301587301587e-05f;    /* 1/8! */
+    p = p * r + 1.9841269841269841270e-04f;    /* 1/7! */
+    p = p * r + 1.3888888888888888889e-03f;    /* 1/6! */
+    p = p * r + 8.3333333333333333333e-03f;    /* 1/5! */
+    p = p * r + 4.1666666666666666667e-02f;    /* 1/4! */
+    p = p * r + 1.6666666666666666667e-01f;    /* 1/3! */
+    p = p * r + 0.5f;                          /* 1/2! */
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int32_t ni = (int32_t)n;
+    /* scale by 2^n in two steps so that 2^n itself never overflows */
+    int32_t n1 = ni / 2;
+    int32_t n2 = ni - n1;
+    float s1 = vren_bits_f32((uint32_t)(n1 + 127) << 23);
+    float s2 = vren_bits_f32((uint32_t)(n2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
 VREN_HD float vren_powi(float a, uint32_t k)
 {
     float result = 1.0f;
@@ -127,16 +169,6 @@ VREN_HD float vren_powi(float a, uint32_t k)
         }
     }
     return result;
-}
-
-VREN_HD float vren_sqrtf(float x)
-{
-    return __builtin_sqrtf(x);
-}
-
-VREN_HD float vren_floorf(float x)
-{
-    return __builtin_floorf(x);
 }
 
 VREN_HD float vren_minf(float a, float b)

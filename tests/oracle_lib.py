@@ -103,3 +103,139 @@ class Oracle:
         nodes = np.array(nodes, copy=True)
         self.lib.o_build_bvh(_p(nodes), c_u32(padded))
         return nodes
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# clustered shading
+# ---------------------------------------------------------------------------------------------------------------
+from vren_amd._lib import Camera, ClusterCfg  # noqa: E402  (struct layouts only; nothing is loaded)
+
+BVH_DTYPE = np.dtype([("min", "<f4", 3), ("next", "<u4"), ("max", "<f4", 3), ("pad", "<u4")])
+
+
+def camera(W, H, fov_deg=90.0, pos=(0.0, 0.0, -50.0), near=0.1, far=100.0, yaw=0.0, pitch=0.0):
+    c = Camera()
+    for i in range(3):
+        c.position[i] = pos[i]
+    c.fov_y = np.float32(np.deg2rad(fov_deg))
+    c.aspect_ratio = np.float32(W) / np.float32(H)
+    c.near_plane = near
+    c.far_plane = far
+    c.yaw = yaw
+    c.pitch = pitch
+    return c
+
+
+def cfg(key_mode=0, depth_slices=32, cluster_aabb_mode=0, morton_bits=5, morton_aabb_mode=0, aabb=None,
+        attenuation=0):
+    c = ClusterCfg()
+    c.key_mode, c.depth_slices, c.cluster_aabb_mode = key_mode, depth_slices, cluster_aabb_mode
+    c.morton_bits, c.morton_aabb_mode, c.attenuation = morton_bits, morton_aabb_mode, attenuation
+    if aabb is not None:
+        for i in range(3):
+            c.morton_aabb_min[i] = aabb[0][i]
+            c.morton_aabb_max[i] = aabb[1][i]
+    return c
+
+
+def _bref(x):
+    return ctypes.byref(x)
+
+
+class ClusteredOracle:
+    """CPU restatement of the clustered-shading stages (oracle/oracle.cpp)."""
+
+    def __init__(self, oracle):
+        self.o = oracle
+        self.lib = oracle.lib
+
+    def camera_matrices(self, cam):
+        view = np.zeros(16, np.float32)
+        proj = np.zeros(16, np.float32)
+        self.lib.o_camera_matrices(_bref(cam), _p(view), _p(proj))
+        return view, proj
+
+    def frame_consts(self, W, H, cam):
+        inv = np.zeros(16, np.float32)
+        sc = np.zeros(4, np.float32)
+        self.lib.o_frame_consts(c_u32(W), c_u32(H), _bref(cam), _p(inv), _p(sc))
+        return inv, sc
+
+    def synth_lights(self, initstate, L):
+        pos = np.zeros((L, 4), np.float32)
+        lig = np.zeros((L, 4), np.float32)
+        self.lib.o_synth_lights(c_u64(initstate), c_u32(L), _p(pos), _p(lig))
+        return pos, lig
+
+    def synth_frame(self, initstate, W, H, cam, n_boxes=64):
+        depth = np.zeros(W * H, np.float32)
+        normals = np.zeros((W * H, 4), np.uint16)
+        self.lib.o_synth_frame(c_u64(initstate), c_u32(W), c_u32(H), _bref(cam), c_u32(n_boxes), _p(depth),
+                               _p(normals))
+        return depth, normals
+
+    def synth_materials(self, initstate, n):
+        ar = np.zeros((n, 4), np.float32)
+        me = np.zeros(n, np.float32)
+        self.lib.o_synth_materials(c_u64(initstate), c_u64(n), _p(ar), _p(me))
+        return ar, me
+
+    def light_bvh(self, pos, lig, cam, cf):
+        L = pos.shape[0]
+        vp = np.zeros((L, 4), np.float32)
+        codes = np.zeros((L, 2), np.uint32)
+        bvh = np.zeros(self.o.calc("buffer_length", L), BVH_DTYPE)
+        self.lib.o_construct_point_light_bvh(_p(pos), _p(lig), c_u32(L), _bref(cam), _bref(cf), _p(vp), _p(codes),
+                                             _p(bvh))
+        return vp, codes, bvh
+
+    def tile_keys(self, W, H, cam, cf, depth, normals):
+        Sx, Sy = (W + 31) // 32, (H + 31) // 32
+        out = np.zeros(Sx * Sy * 1024, np.uint32)
+        self.lib.o_cluster_keys_raw(c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(out))
+        return out.reshape(Sy * Sx, 1024)
+
+    def unique_clusters(self, W, H, cam, cf, depth, normals, key_cap=1 << 20):
+        keys = np.zeros(key_cap, np.uint32)
+        refs = np.zeros(W * H, np.uint32)
+        U = self.lib.o_find_unique_clusters(c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals),
+                                            _p(keys), c_u32(key_cap), _p(refs))
+        return keys[:min(U, key_cap)].copy(), U, refs
+
+    def assign(self, W, H, cam, cf, keys, U, bvh, codes, vp, key_cap=1 << 20, index_cap=1 << 26):
+        n = min(U, key_cap)
+        counts = np.zeros(max(n, 1), np.uint32)
+        offsets = np.zeros(max(n, 1), np.uint32)
+        idx = np.zeros(index_cap, np.uint32)
+        L = codes.shape[0]
+        tot = self.lib.o_assign_lights(c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(keys), c_u32(U), c_u32(key_cap),
+                                       _p(bvh), c_u32(L), _p(codes), _p(vp), _p(counts), _p(offsets), _p(idx),
+                                       c_u32(index_cap))
+        return counts[:n], offsets[:n], idx[:min(tot, index_cap)].copy(), tot
+
+    def assign_brute(self, W, H, cam, cf, keys, bvh, codes, vp, lig, mode, index_cap=1 << 26):
+        n = keys.size
+        counts = np.zeros(max(n, 1), np.uint32)
+        idx = np.zeros(index_cap, np.uint32)
+        L = codes.shape[0]
+        tot = self.lib.o_assign_lights_brute(c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(keys), c_u32(n), _p(bvh),
+                                             c_u32(L), _p(codes), _p(vp), _p(lig), c_int(mode), _p(counts), _p(idx),
+                                             c_u64(index_cap))
+        return counts[:n], idx[:min(tot, index_cap)].copy()
+
+    def cluster_aabb(self, W, H, cam, cf, key):
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        self.lib.o_cluster_aabb(c_u32(W), c_u32(H), _bref(cam), _bref(cf), c_u32(key), _p(mn), _p(mx))
+        return mn, mx
+
+    def shade(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos, lig,
+              dirl=None, f32=False):
+        out = np.full((W * H, 4), np.nan, np.float32 if f32 else np.float64)
+        mask = np.zeros(W * H, np.uint8)
+        nd = 0 if dirl is None else dirl.shape[0]
+        fn = self.lib.o_shade_f32 if f32 else self.lib.o_shade_f64
+        fn(c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(ar), _p(metal), _p(refs),
+           c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(pos), _p(lig),
+           _p(dirl) if dirl is not None else None, c_u32(nd), _p(out), _p(mask))
+        return out, mask.astype(bool)

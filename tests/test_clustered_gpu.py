@@ -1,0 +1,294 @@
+"""GPU parity of the clustered-shading stages through the C ABI against the CPU oracle.
+
+Integer outputs (Morton codes, BVH topology and AABBs, cluster keys and references, light lists) are compared bit
+for bit; the shaded radiance is compared with the FP64 oracle at a relative tolerance of 1e-5 (absolute floor
+1e-6), and the error distribution is printed per output channel.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import empty_dev, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+W3, H3 = 1920, 1080
+L2 = 4096
+SEED_LIGHTS = 42 + 1000 * 1
+SEED_FRAME = 42 + 1000 * 2
+SHADE_RTOL, SHADE_ATOL = 1e-5, 1e-6
+
+
+@pytest.fixture(scope="module")
+def co(oracle):
+    return ol.ClusteredOracle(oracle)
+
+
+@pytest.fixture(scope="module")
+def c3(co):
+    cam = ol.camera(W3, H3)
+    pos, lig = co.synth_lights(SEED_LIGHTS, L2)
+    depth, normals = co.synth_frame(SEED_FRAME, W3, H3, cam)
+    ar, me = co.synth_materials(SEED_FRAME, W3 * H3)
+    return dict(cam=cam, pos=pos, lig=lig, depth=depth, normals=normals, ar=ar, me=me)
+
+
+def gpu_light_bvh(vctx, co, pos, lig, cam, cf):
+    L = pos.shape[0]
+    vp = empty_dev(16 * L)
+    codes = empty_dev(8 * L)
+    nb = co.o.calc("buffer_length", L)
+    bvh = empty_dev(32 * nb, fill=0xCD)
+    vctx.construct_point_light_bvh(to_dev(pos), to_dev(lig), L, cam, cf, vp, codes, bvh)
+    return (from_dev(vp, np.float32, (L, 4)), from_dev(codes, np.uint32, (L, 2)),
+            from_dev(bvh, ol.BVH_DTYPE, (nb,)))
+
+
+def test_synth_generators_match_oracle(vctx, co, c3):
+    cam = c3["cam"]
+    p, l = empty_dev(16 * L2), empty_dev(16 * L2)
+    vctx.synth_lights(SEED_LIGHTS, L2, p, l)
+    assert np.array_equal(from_dev(p, np.uint32, (L2, 4)), c3["pos"].view(np.uint32))
+    assert np.array_equal(from_dev(l, np.uint32, (L2, 4)), c3["lig"].view(np.uint32))
+    d, n = empty_dev(4 * W3 * H3), empty_dev(8 * W3 * H3)
+    vctx.synth_frame(SEED_FRAME, W3, H3, cam, 64, d, n)
+    assert np.array_equal(from_dev(d, np.uint32, (W3 * H3,)), c3["depth"].view(np.uint32))
+    assert np.array_equal(from_dev(n, np.uint16, (W3 * H3, 4)), c3["normals"])
+    a, m = empty_dev(16 * W3 * H3), empty_dev(4 * W3 * H3)
+    vctx.synth_materials(SEED_FRAME, W3 * H3, a, m)
+    assert np.array_equal(from_dev(a, np.uint32, (W3 * H3, 4)), c3["ar"].view(np.uint32))
+    assert np.array_equal(from_dev(m, np.uint32, (W3 * H3,)), c3["me"].view(np.uint32))
+
+
+def test_camera_matrices_match_oracle(vctx, co):
+    for cam in (ol.camera(W3, H3), ol.camera(640, 360, fov_deg=60.0, pos=(3.0, -2.0, 7.5))):
+        v1, p1 = vctx.camera_matrices(cam)
+        v2, p2 = co.camera_matrices(cam)
+        assert np.array_equal(v1, v2) and np.array_equal(p1, p2)
+
+
+@pytest.mark.parametrize("mode", ["reference15", "scene30"])
+def test_light_bvh_c2(vctx, co, c3, mode):
+    cam = c3["cam"]
+    if mode == "reference15":
+        cf = ol.cfg()
+    else:  # config 2: 30-bit codes on the scene AABB ([-50,50]^3 world = view z in [0,100] for this camera)
+        cf = ol.cfg(morton_bits=10, morton_aabb_mode=1, aabb=((-50, -50, 0), (50, 50, 100)))
+    vp, codes, bvh = gpu_light_bvh(vctx, co, c3["pos"], c3["lig"], cam, cf)
+    ovp, ocodes, obvh = co.light_bvh(c3["pos"], c3["lig"], cam, cf)
+    assert np.array_equal(vp.view(np.uint32), ovp.view(np.uint32))
+    assert np.array_equal(codes, ocodes)
+    assert np.array_equal(bvh.view(np.uint32).reshape(-1, 8)[:, :7], obvh.view(np.uint32).reshape(-1, 8)[:, :7])
+
+
+@pytest.mark.parametrize("L", [1, 31, 32, 33, 1000, 4097, 40000])
+def test_light_bvh_sizes(vctx, co, L):
+    cam = ol.camera(640, 360)
+    pos, lig = co.synth_lights(777, L)
+    vp, codes, bvh = gpu_light_bvh(vctx, co, pos, lig, cam, ol.cfg())
+    ovp, ocodes, obvh = co.light_bvh(pos, lig, cam, ol.cfg())
+    assert np.array_equal(codes, ocodes), L
+    assert np.array_equal(bvh.view(np.uint32).reshape(-1, 8)[:, :7], obvh.view(np.uint32).reshape(-1, 8)[:, :7]), L
+
+
+def gpu_clusters(vctx, W, H, cam, cf, depth, normals, key_cap=1 << 20):
+    keys = empty_dev(4 * key_cap)
+    cnt = empty_dev(4)
+    refs = empty_dev(4 * W * H, fill=0xEE)
+    vctx.find_unique_cluster_list(W, H, cam, cf, to_dev(depth), to_dev(normals), keys, key_cap, cnt, refs)
+    U = int(from_dev(cnt, np.uint32)[0])
+    return keys, from_dev(keys, np.uint32, (min(U, key_cap),)), U, from_dev(refs, np.uint32, (W * H,)), cnt, refs
+
+
+@pytest.mark.parametrize("case", ["c3_reference", "odd_sliced", "odd_reference"])
+def test_unique_clusters(vctx, co, c3, case):
+    if case == "c3_reference":
+        W, H, cam, cf = W3, H3, c3["cam"], ol.cfg()
+        depth, normals = c3["depth"], c3["normals"]
+    else:
+        W, H = 1000, 700
+        cam = ol.camera(W, H, fov_deg=75.0)
+        depth, normals = co.synth_frame(555, W, H, cam)
+        cf = ol.cfg(key_mode=1, depth_slices=32) if case == "odd_sliced" else ol.cfg()
+    _, keys, U, refs, _, _ = gpu_clusters(vctx, W, H, cam, cf, depth, normals)
+    okeys, oU, orefs = co.unique_clusters(W, H, cam, cf, depth, normals)
+    assert U == oU
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(refs, orefs)
+
+
+def test_unique_clusters_capacity_flag(vctx, co, c3):
+    from vren_amd import VrenError
+    _, keys, U, refs, _, _ = gpu_clusters(vctx, W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)
+    okeys, oU, orefs = co.unique_clusters(W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)
+    assert U == oU and U > 100 and np.array_equal(keys, okeys) and np.array_equal(refs, orefs)
+    with pytest.raises(VrenError) as e:
+        vctx.sync_status()
+    assert e.value.status == -2
+    vctx.sync_status()  # flags were cleared
+
+
+def run_assign(vctx, co, W, H, cam, cf, depth, normals, pos, lig, key_cap=1 << 20, index_cap=1 << 22):
+    L = pos.shape[0]
+    dpos, dlig = to_dev(pos), to_dev(lig)
+    vp = empty_dev(16 * L)
+    codes = empty_dev(8 * L)
+    nb = co.o.calc("buffer_length", L)
+    bvh = empty_dev(32 * nb)
+    vctx.construct_point_light_bvh(dpos, dlig, L, cam, cf, vp, codes, bvh)
+    dkeys, keys, U, refs, cnt, drefs = gpu_clusters(vctx, W, H, cam, cf, depth, normals, key_cap)
+    counts, offsets = empty_dev(4 * key_cap), empty_dev(4 * key_cap)
+    idx, tot = empty_dev(4 * index_cap), empty_dev(4)
+    vctx.assign_lights(W, H, cam, cf, dkeys, cnt, key_cap, bvh, L, codes, vp, counts, offsets, idx, index_cap, tot)
+    total = int(from_dev(tot, np.uint32)[0])
+    n = min(U, key_cap)
+    return dict(U=U, keys=keys, refs=refs, counts=from_dev(counts, np.uint32, (n,)),
+                offsets=from_dev(offsets, np.uint32, (n,)), idx=from_dev(idx, np.uint32, (min(total, index_cap),)),
+                total=total, dev=dict(pos=dpos, lig=dlig, refs=drefs, counts=counts, offsets=offsets, idx=idx),
+                vp=from_dev(vp, np.float32, (L, 4)), codes=from_dev(codes, np.uint32, (L, 2)),
+                bvh=from_dev(bvh, ol.BVH_DTYPE, (nb,)))
+
+
+@pytest.mark.parametrize("aabb_mode", [0, 1])
+def test_assign_lights_c3(vctx, co, c3, aabb_mode):
+    cam = c3["cam"]
+    cf = ol.cfg(cluster_aabb_mode=aabb_mode)
+    g = run_assign(vctx, co, W3, H3, cam, cf, c3["depth"], c3["normals"], c3["pos"], c3["lig"])
+    ovp, ocodes, obvh = co.light_bvh(c3["pos"], c3["lig"], cam, cf)
+    okeys, oU, orefs = co.unique_clusters(W3, H3, cam, cf, c3["depth"], c3["normals"])
+    oc, oo, oi, ot = co.assign(W3, H3, cam, cf, okeys, oU, obvh, ocodes, ovp)
+    assert g["total"] == ot
+    assert np.array_equal(g["counts"], oc)
+    assert np.array_equal(g["offsets"], oo)
+    assert np.array_equal(g["idx"], oi)
+
+
+def test_assign_lights_many_lights(vctx, co):
+    # a frame with 40 000 lights: BVH of 2^20 leaf slots (4 levels), LDS bitmask of 1250 words per wave
+    W, H = 640, 360
+    cam = ol.camera(W, H)
+    pos, lig = co.synth_lights(4242, 40000)
+    depth, normals = co.synth_frame(4243, W, H, cam)
+    cf = ol.cfg()
+    g = run_assign(vctx, co, W, H, cam, cf, depth, normals, pos, lig)
+    ovp, ocodes, obvh = co.light_bvh(pos, lig, cam, cf)
+    okeys, oU, orefs = co.unique_clusters(W, H, cam, cf, depth, normals)
+    oc, oo, oi, ot = co.assign(W, H, cam, cf, okeys, oU, obvh, ocodes, ovp)
+    assert g["total"] == ot and np.array_equal(g["counts"], oc) and np.array_equal(g["idx"], oi)
+
+
+def shade_error_report(got, ref, mask):
+    g = got[mask, :3].astype(np.float64)
+    r = ref[mask, :3]
+    err = np.abs(g - r) / np.maximum(np.abs(r), SHADE_ATOL / SHADE_RTOL)
+    return err
+
+
+@pytest.mark.parametrize("atten", [0, 1])
+def test_shade_c3_vs_fp64(vctx, co, c3, atten):
+    cam = c3["cam"]
+    cf = ol.cfg(attenuation=atten)
+    g = run_assign(vctx, co, W3, H3, cam, ol.cfg(), c3["depth"], c3["normals"], c3["pos"], c3["lig"])
+    out = empty_dev(16 * W3 * H3, fill=0xFF)  # NaN sentinel: background pixels must stay untouched
+    key_cap = 1 << 20
+    vctx.shade(W3, H3, cam, cf, to_dev(c3["depth"]), to_dev(c3["normals"]), to_dev(c3["ar"]), to_dev(c3["me"]),
+               g["dev"]["refs"], key_cap, g["dev"]["counts"], g["dev"]["offsets"], g["dev"]["idx"], 1 << 22,
+               g["dev"]["pos"], g["dev"]["lig"], L2, out)
+    got = from_dev(out, np.float32, (W3 * H3, 4))
+    cap_counts = np.zeros(key_cap, np.uint32)
+    cap_offs = np.zeros(key_cap, np.uint32)
+    cap_counts[:g["counts"].size] = g["counts"]
+    cap_offs[:g["offsets"].size] = g["offsets"]
+    ref, mask = co.shade(W3, H3, cam, cf, c3["depth"], c3["normals"], c3["ar"], c3["me"], g["refs"], key_cap,
+                         cap_counts, cap_offs, g["idx"], c3["pos"], c3["lig"])
+    assert np.all(np.isnan(got[~mask]))               # background untouched (shade.comp:86)
+    assert np.all(got[mask, 3] == 1.0)
+    err = shade_error_report(got, ref, mask)
+    print("\nshade atten=%d: %d shaded outputs; rel err max %.3e p99.9 %.3e p50 %.3e per channel max %s"
+          % (atten, err.size, err.max(), np.quantile(err, 0.999), np.median(err),
+             ["%.2e" % x for x in err.max(axis=0)]))
+    assert err.max() <= SHADE_RTOL, err.max()
+
+
+def make_frames_batch(vctx, co, specs):
+    """specs: list of (W, H, L, seed). Inputs generated on the device; returns (frame descs, keep-alive tensors)."""
+    import vren_amd
+    keep, descs = [], []
+    for (W, H, L, seed) in specs:
+        cam = ol.camera(W, H)
+        d, n = empty_dev(4 * W * H), empty_dev(8 * W * H)
+        a, m = empty_dev(16 * W * H), empty_dev(4 * W * H)
+        p, l = empty_dev(16 * max(L, 1)), empty_dev(16 * max(L, 1))
+        o = empty_dev(16 * W * H, fill=0xFF)
+        vctx.synth_frame(seed, W, H, cam, 64, d, n)
+        vctx.synth_materials(seed, W * H, a, m)
+        vctx.synth_lights(seed + 7, L, p, l)
+        fd = vren_amd.FrameDesc()
+        fd.width, fd.height, fd.camera = W, H, cam
+        fd.depth, fd.normals, fd.albedo_roughness, fd.metallic = (d.data_ptr(), n.data_ptr(), a.data_ptr(),
+                                                                   m.data_ptr())
+        fd.light_positions, fd.point_lights, fd.light_count = p.data_ptr(), l.data_ptr(), L
+        fd.directional_lights, fd.directional_light_count, fd.output = None, 0, o.data_ptr()
+        fd.key_capacity, fd.index_capacity = 0, 0
+        descs.append(fd)
+        keep.append(dict(d=d, n=n, a=a, m=m, p=p, l=l, o=o, cam=cam, W=W, H=H, L=L))
+    return descs, keep
+
+
+def test_batch_modes_identical_and_match_oracle(vctx, co):
+    import torch
+    import vren_amd
+    specs = [(640, 360, 256, 4000), (1280, 720, 5000, 4001), (333, 200, 1, 4002), (1920, 1080, 4096, 4003)]
+    descs, keep = make_frames_batch(vctx, co, specs)
+    b = vren_amd.Batch(vctx, descs, ol.cfg())
+    results = []
+    for mode in (b.PER_FRAME, b.BATCHED, b.GRAPH, b.GRAPH):
+        for k in keep:
+            k["o"].fill_(0xFF)
+        b.run(mode)
+        torch.cuda.synchronize()
+        results.append([from_dev(k["o"], np.uint32).copy() for k in keep])
+    for r in results[1:]:
+        for a, c in zip(results[0], r):
+            assert np.array_equal(a, c)
+    vctx.sync_status()
+    # frame 0 intermediates and output against the oracle
+    k = keep[0]
+    W, H, L = k["W"], k["H"], k["L"]
+    depth = from_dev(k["d"], np.float32, (W * H,))
+    normals = from_dev(k["n"], np.uint16, (W * H, 4))
+    pos = from_dev(k["p"], np.float32, (L, 4))
+    lig = from_dev(k["l"], np.float32, (L, 4))
+    ar = from_dev(k["a"], np.float32, (W * H, 4))
+    me = from_dev(k["m"], np.float32, (W * H,))
+    okeys, oU, orefs = co.unique_clusters(W, H, k["cam"], ol.cfg(), depth, normals)
+    ovp, ocodes, obvh = co.light_bvh(pos, lig, k["cam"], ol.cfg())
+    oc, oo, oi, ot = co.assign(W, H, k["cam"], ol.cfg(), okeys, oU, obvh, ocodes, ovp)
+    o = b.outputs(0)
+    dev_u = _read_ptr(o.cluster_count, 4, np.uint32)
+    assert int(dev_u[0]) == oU
+    assert np.array_equal(_read_ptr(o.cluster_refs, 4 * W * H, np.uint32), orefs)
+    assert np.array_equal(_read_ptr(o.light_indices, 4 * int(ot), np.uint32), oi)
+    cap_counts = np.zeros(o.key_capacity, np.uint32)
+    cap_offs = np.zeros(o.key_capacity, np.uint32)
+    cap_counts[:oc.size] = oc
+    cap_offs[:oo.size] = oo
+    ref, mask = co.shade(W, H, k["cam"], ol.cfg(), depth, normals, ar, me, orefs, o.key_capacity, cap_counts,
+                         cap_offs, oi, pos, lig)
+    got = results[0][0].view(np.float32).reshape(-1, 4)
+    err = shade_error_report(got, ref, mask)
+    assert err.max() <= SHADE_RTOL
+    b.close()
+
+
+def _read_ptr(ptr, nbytes, dtype):
+    """Copy nbytes from a raw device pointer to the host (hipMemcpy through the HIP runtime torch loaded)."""
+    import ctypes
+    import torch
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.zeros(max(nbytes, 1), np.uint8)
+    rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes),
+                       ctypes.c_int(2))
+    assert rc == 0
+    return out[:nbytes].view(dtype)

@@ -121,3 +121,90 @@ class Context:
 
     def build_bvh(self, nodes, padded_leaf_count):
         return self.check(self.lib.vren_build_bvh(self.h, self.stream(), _ptr(nodes), padded_leaf_count))
+
+    # ---- clustered shading stages (one frame) --------------------------------------------------------------
+    def camera_matrices(self, cam):
+        import numpy as np
+        view = (ctypes.c_float * 16)()
+        proj = (ctypes.c_float * 16)()
+        self.lib.vren_camera_matrices(ctypes.byref(cam), view, proj)
+        return np.array(view, np.float32), np.array(proj, np.float32)
+
+    def construct_point_light_bvh(self, light_pos, lights, L, cam, cfg, view_pos, codes, bvh):
+        return self.check(self.lib.vren_construct_point_light_bvh(
+            self.h, self.stream(), _ptr(light_pos), _ptr(lights), L, ctypes.byref(cam), ctypes.byref(cfg),
+            _ptr(view_pos), _ptr(codes), _ptr(bvh)))
+
+    def find_unique_cluster_list(self, W, H, cam, cfg, depth, normals, keys, key_cap, count, refs):
+        return self.check(self.lib.vren_find_unique_cluster_list(
+            self.h, self.stream(), W, H, ctypes.byref(cam), ctypes.byref(cfg), _ptr(depth), _ptr(normals),
+            _ptr(keys), key_cap, _ptr(count), _ptr(refs)))
+
+    def assign_lights(self, W, H, cam, cfg, keys, count, key_cap, bvh, L, codes, view_pos, counts, offsets, indices,
+                      index_cap, total):
+        return self.check(self.lib.vren_assign_lights(
+            self.h, self.stream(), W, H, ctypes.byref(cam), ctypes.byref(cfg), _ptr(keys), _ptr(count), key_cap,
+            _ptr(bvh), L, _ptr(codes), _ptr(view_pos), _ptr(counts), _ptr(offsets), _ptr(indices), index_cap,
+            _ptr(total)))
+
+    def shade(self, W, H, cam, cfg, depth, normals, albedo_rough, metallic, refs, key_cap, counts, offsets, indices,
+              index_cap, light_pos, lights, L, out, dir_lights=None, n_dir=0):
+        return self.check(self.lib.vren_shade(
+            self.h, self.stream(), W, H, ctypes.byref(cam), ctypes.byref(cfg), _ptr(depth), _ptr(normals),
+            _ptr(albedo_rough), _ptr(metallic), _ptr(refs), key_cap, _ptr(counts), _ptr(offsets), _ptr(indices),
+            index_cap, _ptr(light_pos), _ptr(lights), L, _ptr(dir_lights), n_dir, _ptr(out)))
+
+    def shade_backward(self, W, H, cam, cfg, depth, normals, albedo_rough, metallic, refs, key_cap, counts, offsets,
+                       indices, index_cap, light_pos, lights, L, grad_out, g_pos, g_lights, g_albedo_rough):
+        return self.check(self.lib.vren_shade_backward(
+            self.h, self.stream(), W, H, ctypes.byref(cam), ctypes.byref(cfg), _ptr(depth), _ptr(normals),
+            _ptr(albedo_rough), _ptr(metallic), _ptr(refs), key_cap, _ptr(counts), _ptr(offsets), _ptr(indices),
+            index_cap, _ptr(light_pos), _ptr(lights), L, _ptr(grad_out), _ptr(g_pos), _ptr(g_lights),
+            _ptr(g_albedo_rough)))
+
+    # ---- synthetic inputs ----------------------------------------------------------------------------------
+    def synth_lights(self, initstate, L, light_pos, lights):
+        return self.check(self.lib.vren_synth_lights(self.h, self.stream(), initstate, L, _ptr(light_pos),
+                                                     _ptr(lights)))
+
+    def synth_frame(self, initstate, W, H, cam, n_boxes, depth, normals):
+        return self.check(self.lib.vren_synth_frame(self.h, self.stream(), initstate, W, H, ctypes.byref(cam),
+                                                    n_boxes, _ptr(depth), _ptr(normals)))
+
+    def synth_materials(self, initstate, n, albedo_rough, metallic):
+        return self.check(self.lib.vren_synth_materials(self.h, self.stream(), initstate, n, _ptr(albedo_rough),
+                                                        _ptr(metallic)))
+
+
+class Batch:
+    """cluster_and_shade over a list of frames (vren_batch_*). `frames` is a list of FrameDesc."""
+
+    PER_FRAME, BATCHED, GRAPH = 0, 1, 2
+
+    def __init__(self, ctx, frames, cfg):
+        self.ctx = ctx
+        arr = (FrameDesc * len(frames))(*frames)
+        self._frames = arr
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.vren_batch_create(ctx.h, arr, len(frames), ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.n = len(frames)
+
+    def run(self, mode):
+        return self.ctx.check(self.ctx.lib.vren_batch_run(self.h, self.ctx.stream(), mode))
+
+    def outputs(self, f):
+        o = FrameOutputs()
+        self.ctx.check(self.ctx.lib.vren_batch_frame_outputs(self.h, f, ctypes.byref(o)))
+        return o
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vren_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,256 @@
+// batch.hip -- cluster_and_shade (clustered_shading.cpp:817-1172) over a batch of frames.
+//
+// The reference records one frame's four stages into a render-graph node and submits it; its buffers are sized
+// once at fixed maxima (1920x1080, 2^17 clusters, 2^23 assignments; clustered_shading.cpp:817-922). A batch
+// here owns one device arena with every frame's intermediates sized for that frame, a device table of FrameDev
+// records, and the look-back workspace. The chain is the same kernels for one frame or for all of them:
+//   chain init -> lights -> BVH leaves + first level -> BVH upper levels -> clusters -> light counts ->
+//   light offsets -> light lists -> shade
+// VREN_RUN_PER_FRAME launches it once per frame (with that frame's one-entry table) and synchronises the stream
+// after each frame; VREN_RUN_BATCHED launches it once for all frames; VREN_RUN_GRAPH captures the batched launches
+// into a hipGraph on first use and replays it.
+#include <cstring>
+#include <vector>
+
+#include "chain.hpp"
+
+struct vren_batch {
+    vren_ctx* ctx = nullptr;
+    uint32_t F = 0;
+    std::vector<FrameDev> host;     // batch table (flattened bases)
+    FrameDev* d_table = nullptr;    // F records
+    FrameDev* d_single = nullptr;   // F one-entry tables (bases 0)
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    void* work = nullptr;           // ChainWork memory
+    ChainWork w;                    // batch-wide
+    std::vector<ChainWork> w_frame; // per-frame views of the same memory
+    std::vector<uint32_t> leaf_slots;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture = nullptr;
+};
+
+namespace {
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int run_chain(vren_batch* b, hipStream_t s, const FrameDev* table, uint32_t F, const ChainWork& w,
+              uint32_t leaf_slots)
+{
+    vren_ctx* ctx = b->ctx;
+    int st = launch_chain_init(ctx, s, w);
+    if (!st) st = launch_lights(ctx, s, table, F, w.max_L);
+    if (!st && leaf_slots) st = launch_bvh(ctx, s, table, F, leaf_slots, true);
+    if (!st) st = launch_clusters(ctx, s, table, F, w);
+    if (!st) st = launch_assign(ctx, s, table, F, w);
+    if (!st) st = launch_shade(ctx, s, table, F, w.total_tiles);
+    return st;
+}
+
+} // namespace
+
+extern "C" {
+
+VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, uint32_t n_frames,
+                                  const vren_cluster_cfg* cfg, vren_batch** out)
+{
+    if (!ctx || !out) return VREN_EINVAL;
+    *out = nullptr;
+    int st = vren_validate_cfg(ctx, cfg);
+    if (st) return st;
+    if (!frames || n_frames == 0 || n_frames > 4096) return vren_fail(ctx, VREN_EINVAL, "batch: 1..4096 frames");
+    for (uint32_t i = 0; i < n_frames; i++) {
+        const vren_frame_desc& d = frames[i];
+        st = vren_validate_camera(ctx, &d.camera);
+        if (st) return st;
+        if (d.width == 0 || d.height == 0 || d.width > 8192 || d.height > 8192)
+            return vren_fail(ctx, VREN_EINVAL, "batch: frame %u screen must be in [1, 8192]^2", i);
+        if (!d.depth || !d.normals || !d.albedo_roughness || !d.metallic || !d.output)
+            return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has a null G-buffer or output", i);
+        if (d.light_count > 0 && (!d.light_positions || !d.point_lights))
+            return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has null light buffers", i);
+        if (d.light_count > (1u << 20)) return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has > 2^20 lights", i);
+        if (d.directional_light_count > 0 && !d.directional_lights)
+            return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has null directional lights", i);
+    }
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "batch: context has no device");
+    vren_batch* b = new vren_batch();
+    b->ctx = ctx;
+    b->F = n_frames;
+    b->host.resize(n_frames);
+    b->leaf_slots.resize(n_frames);
+    // sizes and arena layout
+    uint32_t tile_base = 0, leaf_base = 0, offs_base = 0, max_L = 0;
+    std::vector<size_t> off(n_frames * 12);
+    size_t arena = 0;
+    for (uint32_t i = 0; i < n_frames; i++) {
+        const vren_frame_desc& d = frames[i];
+        FrameDev& f = b->host[i];
+        memset(&f, 0, sizeof(f));
+        vren_frame_setup(&f, d.width, d.height, &d.camera, cfg);
+        const uint32_t L = d.light_count;
+        const uint32_t tiles = f.Sx * f.Sy;
+        uint32_t key_cap = d.key_capacity;
+        if (key_cap == 0) key_cap = (uint32_t)std::min<uint64_t>((uint64_t)tiles * 1024, 1u << 20);
+        uint32_t index_cap = d.index_capacity;
+        if (index_cap == 0) index_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)key_cap * L, 1), 1u << 24);
+        f.tile_base = tile_base;
+        f.leaf_base = leaf_base;
+        f.offs_base = offs_base;
+        f.L = L;
+        f.light_pos = d.light_positions;
+        f.lights = d.point_lights;
+        f.n_dir = d.directional_light_count;
+        f.dir_lights = d.directional_lights;
+        f.depth = d.depth;
+        f.normals = d.normals;
+        f.albedo_rough = d.albedo_roughness;
+        f.metallic = d.metallic;
+        f.out = d.output;
+        f.key_cap = key_cap;
+        f.index_cap = index_cap;
+        f.flags = ctx->d_flags;
+        f.bvh_padded = L ? vren_calc_bvh_padded_leaf_count(L) : 0;
+        f.bvh_root = L ? vren_calc_bvh_root_index(L) : 0;
+        f.bvh_levels = L ? vren_calc_bvh_level_count(L) : 0;
+        f.bvh_length = L ? vren_calc_bvh_buffer_length(L) : 0;
+        const uint32_t slots = L ? (f.bvh_padded + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN : 0;
+        b->leaf_slots[i] = slots;
+        tile_base += tiles;
+        leaf_base += slots;
+        offs_base += (key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
+        max_L = std::max(max_L, L);
+        size_t* o = &off[i * 12];
+        o[0] = arena; arena += a256((size_t)L * 16);                 // view_pos
+        o[1] = arena; arena += a256((size_t)L * 8);                  // codes
+        o[2] = arena; arena += a256(L > 4096 ? (size_t)L * 8 : 0);   // codes_tmp
+        o[3] = arena; arena += a256((size_t)f.bvh_length * 32);      // bvh
+        o[4] = arena; arena += a256((size_t)key_cap * 4);            // keys
+        o[5] = arena; arena += 256;                                  // cluster_count
+        o[6] = arena; arena += a256((size_t)d.width * d.height * 4); // refs
+        o[7] = arena; arena += a256((size_t)key_cap * 4);            // counts
+        o[8] = arena; arena += a256((size_t)key_cap * 4);            // offsets
+        o[9] = arena; arena += a256((size_t)index_cap * 4);          // indices
+        o[10] = arena; arena += 256;                                 // assigned_total
+    }
+    hipError_t e = hipMalloc(&b->arena, std::max<size_t>(arena, 256));
+    if (e == hipSuccess) e = hipMemset(b->arena, 0, std::max<size_t>(arena, 256));
+    const size_t wbytes = chain_work_bytes(tile_base, offs_base);
+    if (e == hipSuccess) e = hipMalloc(&b->work, wbytes);
+    if (e == hipSuccess) e = hipMalloc(&b->d_table, sizeof(FrameDev) * n_frames);
+    if (e == hipSuccess) e = hipMalloc(&b->d_single, sizeof(FrameDev) * n_frames);
+    if (e != hipSuccess) {
+        vren_batch_destroy(b);
+        return vren_hip_fail(ctx, e, "batch allocation");
+    }
+    b->arena_bytes = arena;
+    for (uint32_t i = 0; i < n_frames; i++) {
+        FrameDev& f = b->host[i];
+        size_t* o = &off[i * 12];
+        f.view_pos = reinterpret_cast<float*>(b->arena + o[0]);
+        f.codes = reinterpret_cast<uint32_t*>(b->arena + o[1]);
+        f.codes_tmp = f.L > 4096 ? reinterpret_cast<uint32_t*>(b->arena + o[2]) : nullptr;
+        f.bvh = reinterpret_cast<vren_bvh_node*>(b->arena + o[3]);
+        f.keys = reinterpret_cast<uint32_t*>(b->arena + o[4]);
+        f.cluster_count = reinterpret_cast<uint32_t*>(b->arena + o[5]);
+        f.refs = reinterpret_cast<uint32_t*>(b->arena + o[6]);
+        f.counts = reinterpret_cast<uint32_t*>(b->arena + o[7]);
+        f.offsets = reinterpret_cast<uint32_t*>(b->arena + o[8]);
+        f.indices = reinterpret_cast<uint32_t*>(b->arena + o[9]);
+        f.assigned_total = reinterpret_cast<uint32_t*>(b->arena + o[10]);
+    }
+    chain_work_bind(&b->w, b->work, tile_base, offs_base);
+    b->w.total_leaf_slots = leaf_base;
+    b->w.max_L = max_L;
+    // per-frame tables: same buffers, bases 0, the workspace viewed from its start
+    std::vector<FrameDev> single(b->host);
+    b->w_frame.resize(n_frames);
+    for (uint32_t i = 0; i < n_frames; i++) {
+        single[i].tile_base = 0;
+        single[i].leaf_base = 0;
+        single[i].offs_base = 0;
+        ChainWork& wf = b->w_frame[i];
+        chain_work_bind(&wf, b->work, single[i].Sx * single[i].Sy, (single[i].key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE);
+        wf.total_leaf_slots = b->leaf_slots[i];
+        wf.max_L = single[i].L;
+    }
+    hipStream_t s = nullptr;
+    st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);
+    if (!st) st = vren_upload_frames(ctx, s, b->d_single, single.data(), n_frames);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch: upload failed");
+    if (st) {
+        vren_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return VREN_OK;
+}
+
+VREN_EXPORT int vren_batch_run(vren_batch* b, vren_stream_t stream, int mode)
+{
+    if (!b) return VREN_EINVAL;
+    vren_ctx* ctx = b->ctx;
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == VREN_RUN_PER_FRAME) {
+        for (uint32_t i = 0; i < b->F; i++) {
+            int st = run_chain(b, s, b->d_single + i, 1, b->w_frame[i], b->leaf_slots[i]);
+            if (st) return st;
+            VREN_HIP(ctx, hipStreamSynchronize(s));
+        }
+        return VREN_OK;
+    }
+    if (mode == VREN_RUN_BATCHED) return run_chain(b, s, b->d_table, b->F, b->w, b->w.total_leaf_slots);
+    if (mode == VREN_RUN_GRAPH) {
+        if (!b->exec) {
+            VREN_HIP(ctx, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+            VREN_HIP(ctx, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+            int st = run_chain(b, b->capture, b->d_table, b->F, b->w, b->w.total_leaf_slots);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->capture, &g);
+            if (st) return st;
+            if (e != hipSuccess) return vren_hip_fail(ctx, e, "hipStreamEndCapture");
+            b->graph = g;
+            VREN_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        VREN_HIP(ctx, hipGraphLaunch(b->exec, s));
+        return VREN_OK;
+    }
+    return vren_fail(ctx, VREN_EINVAL, "batch_run: unknown mode %d", mode);
+}
+
+VREN_EXPORT int vren_batch_frame_outputs(vren_batch* b, uint32_t frame, vren_frame_outputs* out)
+{
+    if (!b || !out || frame >= b->F) return VREN_EINVAL;
+    const FrameDev& f = b->host[frame];
+    out->view_positions = f.view_pos;
+    out->light_codes = f.codes;
+    out->bvh = f.bvh;
+    out->bvh_length = f.bvh_length;
+    out->cluster_keys = f.keys;
+    out->cluster_count = f.cluster_count;
+    out->cluster_refs = f.refs;
+    out->key_capacity = f.key_cap;
+    out->light_counts = f.counts;
+    out->light_offsets = f.offsets;
+    out->light_indices = f.indices;
+    out->index_capacity = f.index_cap;
+    out->assigned_total = f.assigned_total;
+    return VREN_OK;
+}
+
+VREN_EXPORT void vren_batch_destroy(vren_batch* b)
+{
+    if (!b) return;
+    hipDeviceSynchronize();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->capture) hipStreamDestroy(b->capture);
+    if (b->arena) hipFree(b->arena);
+    if (b->work) hipFree(b->work);
+    if (b->d_table) hipFree(b->d_table);
+    if (b->d_single) hipFree(b->d_single);
+    delete b;
+}
+
+} // extern "C"

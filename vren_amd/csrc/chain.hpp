@@ -1,0 +1,52 @@
+// chain.hpp -- internal launchers of the clustered-shading chain, shared by the per-stage entry points (one
+// frame) and the batched chain (many frames in one launch per stage).
+#pragma once
+
+#include "clustered.hpp"
+
+int vren_validate_cfg(vren_ctx* ctx, const vren_cluster_cfg* cfg);
+int vren_validate_camera(vren_ctx* ctx, const vren_camera* cam);
+int vren_upload_frames(vren_ctx* ctx, hipStream_t s, FrameDev* dst, const FrameDev* src, uint32_t n);
+FrameDev* vren_next_frame_slot(vren_ctx* ctx);
+
+#define VREN_LEAF_ALIGN 256u
+#define VREN_SCAN_TILE 4096u
+
+// look-back statuses and ticket counters of one chain launch
+struct ChainWork {
+    uint64_t* tile_status; // cluster compaction: one word per 32x32 tile of every frame
+    uint64_t* offs_status; // light-offset scan: one word per 4096 clusters of capacity of every frame
+    uint32_t* tickets;     // [0] cluster tiles, [1] offset tiles
+    uint32_t total_tiles;
+    uint32_t total_offs_tiles;
+    uint32_t total_leaf_slots;
+    uint32_t max_L;
+};
+
+inline size_t chain_work_bytes(uint32_t total_tiles, uint32_t total_offs_tiles)
+{
+    return 256 + ((size_t)total_tiles + total_offs_tiles) * sizeof(uint64_t);
+}
+void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t total_offs_tiles);
+
+// stage launchers; d_frames is a device table of F FrameDev records
+int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w);
+int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t max_L);
+int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_leaf_slots,
+               bool leaves_from_lights);
+int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
+int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
+int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_tiles);
+
+// device: frame of a flattened index, given a per-frame base field (bases ascending)
+template <typename Get>
+__device__ __forceinline__ uint32_t find_frame(const FrameDev* frames, uint32_t F, uint32_t g, Get base)
+{
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (base(frames[mid]) <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}

@@ -12,9 +12,8 @@ struct FrameDev {
     // sizes
     uint32_t W, H, Sx, Sy;
     uint32_t tile_base;     // first flattened 32x32 tile of this frame in the table
-    uint32_t light_base;    // first flattened light / BVH-leaf slot (used by batched light kernels)
-    uint32_t leaf_base;
-    uint32_t pad0;
+    uint32_t leaf_base;     // first flattened leaf slot (256-aligned) of this frame's BVH leaves
+    uint32_t offs_base;     // first flattened tile of this frame's light-offset scan
     // camera constants
     float view[16];         // camera::get_view()
     float inv_proj[16];     // inverse(camera::get_projection()), rounded from double
