@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the clustered-shading compute path on one MI355X.
+
+Workload (the metric's configuration): batches of 64 independent 1920x1080 frames, each with its own procedural
+scene (64 boxes + ground plane, depth/normals ray-cast) and its own 4096 point lights (positions U[-50,50]^3,
+radius U[1,8], colour U[0,1]^3, intensity logU[0.1,100]), per-pixel materials; camera at (0,0,-50), 90 deg fov,
+near 0.1, far 100. All inputs are synthetic and generated on the device before timing; they stay resident in HBM.
+
+One step = the whole clustered-shading chain (light BVH, unique clusters, light assignment, shade) over one
+batch of 64 frames:
+  - batched: the chain launched once for all 64 frames, replayed from a captured hipGraph  -> `value`
+  - plain:   each frame's chain launched separately and followed by hipStreamSynchronize (the reference's
+             record -> vkQueueSubmit -> fence wait per frame)                                 -> `plain_frames_per_s`
+Both produce identical bytes (tests/test_clustered_gpu.py). Also measured: the 2^26-pair onesweep radix sort
+(`sort`), per-kernel durations of the batched chain with HIP events, and the CPU restatement on the host cores
+(`cpu_baseline`).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "clustered-shade frames/s (1080p, 4096 lights) batched vs plain; sort Gkeys/s %HBM"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["chain_init", "lights", "bvh_leaf", "bvh_upper", "clusters", "assign_count", "assign_offsets",
+           "assign_write", "shade"]
+W, H, L_LIGHTS = 1920, 1080, 4096
+SEED_FRAMES = 42 + 1000 * 4
+# algorithmic bytes per unit of the chain's kernels (per pixel / per light of one frame)
+BYTES_PER_PIXEL = {"clusters": 16.0, "shade": 52.0}
+BYTES_PER_LIGHT = {"lights": 40.0}
+SORT_BYTES_PER_PAIR = 68.0
+
+
+def make_camera(vren_amd):
+    import numpy as np
+    c = vren_amd.Camera()
+    c.position[0], c.position[1], c.position[2] = 0.0, 0.0, -50.0
+    c.fov_y = float(np.float32(np.deg2rad(90.0)))
+    c.aspect_ratio = float(np.float32(W) / np.float32(H))
+    c.near_plane, c.far_plane, c.yaw, c.pitch = 0.1, 100.0, 0.0, 0.0
+    return c
+
+
+def build_batch(ctx, vren_amd, n_frames):
+    import torch
+    cam = make_camera(vren_amd)
+    keep, descs = [], []
+    for f in range(n_frames):
+        seed = SEED_FRAMES + f
+        d = torch.empty(W * H, dtype=torch.float32, device="cuda")
+        n = torch.empty(W * H * 4, dtype=torch.int16, device="cuda")
+        a = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+        m = torch.empty(W * H, dtype=torch.float32, device="cuda")
+        p = torch.empty(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+        l = torch.empty(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+        o = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        ctx.synth_frame(seed, W, H, cam, 64, d, n)
+        ctx.synth_materials(seed, W * H, a, m)
+        ctx.synth_lights(seed + 7, L_LIGHTS, p, l)
+        fd = vren_amd.FrameDesc()
+        fd.width, fd.height, fd.camera = W, H, cam
+        fd.depth, fd.normals, fd.albedo_roughness, fd.metallic = d.data_ptr(), n.data_ptr(), a.data_ptr(), m.data_ptr()
+        fd.light_positions, fd.point_lights, fd.light_count = p.data_ptr(), l.data_ptr(), L_LIGHTS
+        fd.directional_lights, fd.directional_light_count, fd.output = None, 0, o.data_ptr()
+        fd.key_capacity, fd.index_capacity = 0, 0
+        descs.append(fd)
+        keep.append((d, n, a, m, p, l, o))
+    torch.cuda.synchronize()
+    return vren_amd.Batch(ctx, descs, vren_amd.default_cfg()), keep
+
+
+def time_events(fn, steps):
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1), wall * 1e3
+
+
+def bench_sort(ctx, log2n, steps, warmup, tune):
+    import torch
+    n = 1 << log2n
+    keys0 = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.synth_keys(42, 54, n, keys0)  # configuration 1's generator: pcg32(initstate 42, initseq 54)
+    vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
+    keys, vals = torch.empty_like(keys0), torch.empty_like(vals0)
+    scratch = torch.empty(ctx.radix_sort_scratch_bytes(n, True), dtype=torch.uint8, device="cuda")
+    cfg_id, tuned_ms = (None, None)
+    if tune:
+        cfg_id, tuned_ms = ctx.autotune_radix_sort(n, True, 32)
+    s = torch.cuda.current_stream()
+    times = []
+    for r in range(warmup + steps):
+        keys.copy_(keys0)
+        vals.copy_(vals0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        ctx.radix_sort(keys, vals, n, scratch)
+        e1.record(s)
+        torch.cuda.synchronize()
+        if r >= warmup:
+            times.append(e0.elapsed_time(e1))
+    # correctness spot check on the device: sorted
+    ku = keys.to(torch.int64) & 0xFFFFFFFF
+    assert bool((ku[1:] >= ku[:-1]).all())
+    ms = sorted(times)[len(times) // 2]
+    achieved = SORT_BYTES_PER_PAIR * n / (ms * 1e-3) / 1e9
+    configs = ctx.radix_configs()
+    return {
+        "n": n, "pairs": True, "ms": round(ms, 4), "gpairs_per_s": round(n / (ms * 1e-3) / 1e9, 3),
+        "config": configs[cfg_id] if cfg_id is not None else "default",
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_pair": SORT_BYTES_PER_PAIR},
+    }
+
+
+def cpu_baseline(frames=2, sort_log2=22):
+    """The CPU restatement (oracle/) on one host thread: the chain on one 1080p frame with the reference's fp32
+    shade formula, and std::stable_sort of configuration-1 pairs. Inputs are generated outside the timing."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    o = ol.Oracle()
+    co = ol.ClusteredOracle(o)
+    cam = ol.camera(W, H)
+    cf = ol.cfg()
+    total = 0.0
+    for f in range(frames):
+        seed = SEED_FRAMES + f
+        depth, normals = co.synth_frame(seed, W, H, cam)
+        ar, me = co.synth_materials(seed, W * H)
+        pos, lig = co.synth_lights(seed + 7, L_LIGHTS)
+        t0 = time.perf_counter()
+        vp, codes, bvh = co.light_bvh(pos, lig, cam, cf)
+        keys, U, refs = co.unique_clusters(W, H, cam, cf, depth, normals)
+        counts, offsets, idx, tot = co.assign(W, H, cam, cf, keys, U, bvh, codes, vp)
+        cc = np.zeros(1 << 20, np.uint32)
+        oo = np.zeros(1 << 20, np.uint32)
+        cc[:counts.size] = counts
+        oo[:offsets.size] = offsets
+        co.shade(W, H, cam, cf, depth, normals, ar, me, refs, 1 << 20, cc, oo, idx, pos, lig, f32=True)
+        total += time.perf_counter() - t0
+    n = 1 << sort_log2
+    keys = o.pcg32(42, 54, n)
+    vals = np.arange(n, dtype=np.uint32)
+    t0 = time.perf_counter()
+    o.stable_sort_kv(keys, vals)
+    st = time.perf_counter() - t0
+    return {
+        "value": round(frames / total, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+        "sample": "%d frames of 1920x1080 / 4096 lights through the oracle chain (BVH, clusters, assignment, "
+                  "reference fp32 shade), one thread, input generation excluded" % frames,
+        "sort": {"value": round(n / st / 1e9, 4), "unit": "Gpairs/s", "cores": 1,
+                 "sample": "std::stable_sort of 2^%d configuration-1 (key, index) pairs" % sort_log2},
+        "host_cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip() + " (%d logical cpus)" % os.cpu_count()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--sort-log2", type=int, default=26)
+    ap.add_argument("--no-sort", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: the clustered-shading batches shard by frame with no exchange between "
+              "GPUs, so there is no multi-GPU run to make here (asked for --gpus %d, WORLD_SIZE %d)"
+              % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import vren_amd
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    ctx = vren_amd.Context(0)
+    print("[bench] building %d frames of %dx%d with %d lights" % (args.frames, W, H, L_LIGHTS), file=sys.stderr,
+          flush=True)
+    batch, keep = build_batch(ctx, vren_amd, args.frames)
+
+    # batched: one graph replay per step
+    for _ in range(max(1, args.warmup)):
+        batch.run(batch.GRAPH)
+    torch.cuda.synchronize()
+    ms_b, wall_b = time_events(lambda: batch.run(batch.GRAPH), args.steps)
+    ms_per_step = ms_b / args.steps
+    value = args.frames * args.steps / (ms_b * 1e-3)
+    print("[bench] batched: %.3f ms/step (wall %.3f)" % (ms_per_step, wall_b / args.steps), file=sys.stderr,
+          flush=True)
+
+    # plain: per frame launches + stream sync
+    for _ in range(max(1, args.warmup)):
+        batch.run(batch.PER_FRAME)
+    plain_steps = max(2, args.steps // 2)
+    ms_p, wall_p = time_events(lambda: batch.run(batch.PER_FRAME), plain_steps)
+    plain_fps = args.frames * plain_steps / (wall_p * 1e-3)
+    print("[bench] plain: %.3f ms/step (wall %.3f)" % (ms_p / plain_steps, wall_p / plain_steps), file=sys.stderr,
+          flush=True)
+    ctx.sync_status()
+
+    # per-kernel durations of the batched chain (HIP events between the kernels, same stream)
+    acc = [0.0] * len(KERNELS)
+    prof_steps = max(3, args.steps // 2)
+    for _ in range(prof_steps):
+        ms = batch.run_profiled()
+        for i, v in enumerate(ms):
+            acc[i] += v
+    kernel_ms = {k: round(a / prof_steps, 5) for k, a in zip(KERNELS, acc)}
+    dom = max(kernel_ms, key=lambda k: kernel_ms[k])
+    pixels = args.frames * W * H
+    if dom in BYTES_PER_PIXEL:
+        alg_bytes = BYTES_PER_PIXEL[dom] * pixels
+    elif dom in BYTES_PER_LIGHT:
+        alg_bytes = BYTES_PER_LIGHT[dom] * args.frames * L_LIGHTS
+    else:
+        alg_bytes = None
+    achieved = alg_bytes / (kernel_ms[dom] * 1e-3) / 1e9 if alg_bytes else None
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": None, "launch_ms": kernel_ms[dom],
+                "algorithmic_bytes_per_launch": alg_bytes}
+
+    sort = None
+    if not args.no_sort:
+        print("[bench] sort 2^%d pairs" % args.sort_log2, file=sys.stderr, flush=True)
+        sort = bench_sort(ctx, args.sort_log2, args.steps, args.warmup, not args.no_tune)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (device-generated procedural scenes and lights)",
+        "config": {"workload": "clustered shading, batches of %d frames of 1920x1080 with 4096 point lights each "
+                               "(light BVH + unique clusters + BVH light assignment + PBR shade), hipGraph replay"
+                               % args.frames,
+                   "frames_per_step": args.frames, "resolution": "1920x1080", "lights_per_frame": L_LIGHTS,
+                   "parallelism": "1 GPU"},
+        "plain_frames_per_s": round(plain_fps, 2),
+        "batched_over_plain": round(value / plain_fps, 3),
+        "kernel_ms": kernel_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "sort": sort,
+    }
+    batch.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -280,6 +280,12 @@ enum {
 VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, uint32_t n_frames,
                                   const vren_cluster_cfg* cfg, vren_batch** out);
 VREN_EXPORT int vren_batch_run(vren_batch* batch, vren_stream_t stream, int mode);
+/* VREN_RUN_BATCHED with a HIP event after every kernel; kernel_ms[VREN_CHAIN_KERNEL_COUNT] receives the
+ * durations of chain init, lights, BVH leaves, BVH upper levels, clusters, light counts, light offsets,
+ * light lists and shade (the reference's GPU timestamp profiler, profiler.cpp:32-104, has no slot wired on
+ * this path). Synchronises the stream. */
+#define VREN_CHAIN_KERNEL_COUNT 9
+VREN_EXPORT int vren_batch_run_profiled(vren_batch* batch, vren_stream_t stream, float* kernel_ms);
 VREN_EXPORT int vren_batch_frame_outputs(vren_batch* batch, uint32_t frame, vren_frame_outputs* out);
 VREN_EXPORT void vren_batch_destroy(vren_batch* batch);
 
@@ -287,6 +293,9 @@ VREN_EXPORT void vren_batch_destroy(vren_batch* batch);
  * Synthetic inputs on the device (include/vren_hip/synth.h; no reference counterpart: the reference draws its
  * inputs from a rasterised glTF scene). initstate = 42 + 1000 * config (+ frame).
  * ------------------------------------------------------------------------------------------------------- */
+/* n uint32 PCG32 draws of stream (initstate, initseq) (configuration 1's keys: initstate 42, initseq 54) */
+VREN_EXPORT int vren_synth_keys(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint64_t initseq, uint64_t n,
+                                uint32_t* out);
 VREN_EXPORT int vren_synth_lights(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t light_count,
                                   float* light_positions, float* point_lights);
 VREN_EXPORT int vren_synth_frame(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t width,

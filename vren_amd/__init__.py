@@ -163,6 +163,9 @@ class Context:
             _ptr(g_albedo_rough)))
 
     # ---- synthetic inputs ----------------------------------------------------------------------------------
+    def synth_keys(self, initstate, initseq, n, out):
+        return self.check(self.lib.vren_synth_keys(self.h, self.stream(), initstate, initseq, n, _ptr(out)))
+
     def synth_lights(self, initstate, L, light_pos, lights):
         return self.check(self.lib.vren_synth_lights(self.h, self.stream(), initstate, L, _ptr(light_pos),
                                                      _ptr(lights)))
@@ -192,6 +195,11 @@ class Batch:
 
     def run(self, mode):
         return self.ctx.check(self.ctx.lib.vren_batch_run(self.h, self.ctx.stream(), mode))
+
+    def run_profiled(self):
+        ms = (ctypes.c_float * 9)()
+        self.ctx.check(self.ctx.lib.vren_batch_run_profiled(self.h, self.ctx.stream(), ms))
+        return list(ms)
 
     def outputs(self, f):
         o = FrameOutputs()

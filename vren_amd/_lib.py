@@ -94,8 +94,10 @@ PROTOTYPES = {
                                     c_void_p]),
     "vren_batch_create": (c_int, [c_void_p, P(FrameDesc), c_uint32, P(ClusterCfg), P(c_void_p)]),
     "vren_batch_run": (c_int, [c_void_p, c_void_p, c_int]),
+    "vren_batch_run_profiled": (c_int, [c_void_p, c_void_p, P(c_float)]),
     "vren_batch_frame_outputs": (c_int, [c_void_p, c_uint32, P(FrameOutputs)]),
     "vren_batch_destroy": (None, [c_void_p]),
+    "vren_synth_keys": (c_int, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint64, c_void_p]),
     "vren_synth_lights": (c_int, [c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_void_p]),
     "vren_synth_frame": (c_int, [c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, P(Camera), c_uint32, c_void_p,
                                  c_void_p]),

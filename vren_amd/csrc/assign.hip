@@ -299,10 +299,13 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
     assign_count_kernel<<<persistent_blocks(ctx, 4), 4 * VREN_WAVE, 4 * kStack * 4, s>>>(d_frames, F);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     assign_offsets_kernel<<<w.total_offs_tiles, 256, 0, s>>>(d_frames, F, w.offs_status, w.tickets + 1);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     assign_write_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     return VREN_OK;
 }
 
@@ -349,6 +352,7 @@ extern "C" VREN_EXPORT int vren_assign_lights(vren_ctx* ctx, vren_stream_t strea
     f.flags = ctx->d_flags;
     f.offs_base = 0;
     ChainWork w;
+    memset(&w, 0, sizeof(w));
     const uint32_t offs_tiles = (key_capacity + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
     st = vren_ensure_scratch(ctx, chain_work_bytes(0, offs_tiles));
     if (st) return st;

@@ -40,11 +40,11 @@ int run_chain(vren_batch* b, hipStream_t s, const FrameDev* table, uint32_t F, c
 {
     vren_ctx* ctx = b->ctx;
     int st = launch_chain_init(ctx, s, w);
-    if (!st) st = launch_lights(ctx, s, table, F, w.max_L);
-    if (!st && leaf_slots) st = launch_bvh(ctx, s, table, F, leaf_slots, true);
+    if (!st) st = launch_lights(ctx, s, table, F, w);
+    if (!st && leaf_slots) st = launch_bvh(ctx, s, table, F, leaf_slots, true, &w);
     if (!st) st = launch_clusters(ctx, s, table, F, w);
     if (!st) st = launch_assign(ctx, s, table, F, w);
-    if (!st) st = launch_shade(ctx, s, table, F, w.total_tiles);
+    if (!st) st = launch_shade(ctx, s, table, F, w);
     return st;
 }
 
@@ -217,6 +217,28 @@ VREN_EXPORT int vren_batch_run(vren_batch* b, vren_stream_t stream, int mode)
         return VREN_OK;
     }
     return vren_fail(ctx, VREN_EINVAL, "batch_run: unknown mode %d", mode);
+}
+
+VREN_EXPORT int vren_batch_run_profiled(vren_batch* b, vren_stream_t stream, float* kernel_ms)
+{
+    if (!b || !kernel_ms) return VREN_EINVAL;
+    vren_ctx* ctx = b->ctx;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[VREN_CHAIN_KERNELS + 1];
+    for (int i = 0; i <= VREN_CHAIN_KERNELS; i++) VREN_HIP(ctx, hipEventCreate(&ev[i]));
+    int n = 0;
+    ChainWork w = b->w;
+    w.events = ev + 1;
+    w.n_events = &n;
+    VREN_HIP(ctx, hipEventRecord(ev[0], s));
+    int st = run_chain(b, s, b->d_table, b->F, w, b->w.total_leaf_slots);
+    if (!st) VREN_HIP(ctx, hipEventSynchronize(ev[n]));
+    for (int i = 0; i < VREN_CHAIN_KERNELS; i++) {
+        kernel_ms[i] = 0.0f;
+        if (!st && i < n) hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]);
+    }
+    for (int i = 0; i <= VREN_CHAIN_KERNELS; i++) hipEventDestroy(ev[i]);
+    return st;
 }
 
 VREN_EXPORT int vren_batch_frame_outputs(vren_batch* b, uint32_t frame, vren_frame_outputs* out)

@@ -21,7 +21,18 @@ struct ChainWork {
     uint32_t total_offs_tiles;
     uint32_t total_leaf_slots;
     uint32_t max_L;
+    // optional per-kernel timing: an event is recorded after every kernel of the chain
+    hipEvent_t* events = nullptr;
+    int* n_events = nullptr;
 };
+
+inline void chain_mark(const ChainWork& w, hipStream_t s)
+{
+    if (w.events) hipEventRecord(w.events[(*w.n_events)++], s);
+}
+
+// kernels of one chain launch, in order (chain_mark follows each)
+#define VREN_CHAIN_KERNELS 9
 
 inline size_t chain_work_bytes(uint32_t total_tiles, uint32_t total_offs_tiles)
 {
@@ -31,12 +42,12 @@ void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t tot
 
 // stage launchers; d_frames is a device table of F FrameDev records
 int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w);
-int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t max_L);
+int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_leaf_slots,
-               bool leaves_from_lights);
+               bool leaves_from_lights, const ChainWork* w);
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
-int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_tiles);
+int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 
 // device: frame of a flattened index, given a per-frame base field (bases ascending)
 template <typename Get>

@@ -179,6 +179,7 @@ int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w)
     uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(256, (n + 255) / 256));
     chain_init_kernel<<<blocks, 256, 0, s>>>(w.tile_status, n, w.tickets);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     return VREN_OK;
 }
 
@@ -186,6 +187,7 @@ int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint
 {
     clusters_kernel<<<w.total_tiles, 1024, 0, s>>>(d_frames, F, w.tile_status, w.tickets + 0);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     return VREN_OK;
 }
 
@@ -217,6 +219,7 @@ extern "C" VREN_EXPORT int vren_find_unique_cluster_list(vren_ctx* ctx, vren_str
     f.refs = cluster_refs;
     f.flags = ctx->d_flags;
     ChainWork w;
+    memset(&w, 0, sizeof(w));
     uint32_t tiles = f.Sx * f.Sy;
     st = vren_ensure_scratch(ctx, chain_work_bytes(tiles, 0));
     if (st) return st;

@@ -300,21 +300,23 @@ __global__ __launch_bounds__(1024) void bvh_upper_kernel(const FrameDev* frames)
 
 } // namespace
 
-int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t max_L)
+int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    (void)max_L;
     lights_kernel<<<F, kLT, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     return VREN_OK;
 }
 
 int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_leaf_slots,
-               bool leaves_from_lights)
+               bool leaves_from_lights, const ChainWork* w)
 {
     bvh_leaf_kernel<<<total_leaf_slots / 256, 256, 0, s>>>(d_frames, F, leaves_from_lights);
     VREN_LAUNCH_CHECK(ctx);
+    if (w) chain_mark(*w, s);
     bvh_upper_kernel<<<F, 1024, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
+    if (w) chain_mark(*w, s);
     return VREN_OK;
 }
 
@@ -345,7 +347,7 @@ VREN_EXPORT int vren_build_bvh(vren_ctx* ctx, vren_stream_t stream, vren_bvh_nod
     int st = vren_upload_frames(ctx, s, slot, &f, 1);
     if (st) return st;
     uint32_t slots = (padded_leaf_count + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN;
-    return launch_bvh(ctx, s, slot, 1, slots, false);
+    return launch_bvh(ctx, s, slot, 1, slots, false, nullptr);
 }
 
 VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stream, const float* light_positions,
@@ -385,10 +387,12 @@ VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stre
     FrameDev* slot = vren_next_frame_slot(ctx);
     st = vren_upload_frames(ctx, s, slot, &f, 1);
     if (st) return st;
-    st = launch_lights(ctx, s, slot, 1, light_count);
+    ChainWork w;
+    memset(&w, 0, sizeof(w));
+    st = launch_lights(ctx, s, slot, 1, w);
     if (st) return st;
     uint32_t slots = (f.bvh_padded + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN;
-    return launch_bvh(ctx, s, slot, 1, slots, true);
+    return launch_bvh(ctx, s, slot, 1, slots, true, nullptr);
 }
 
 } // extern "C"

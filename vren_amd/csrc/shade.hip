@@ -181,10 +181,11 @@ __global__ __launch_bounds__(1024) void shade_kernel(const FrameDev* frames, uin
 
 } // namespace
 
-int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_tiles)
+int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    shade_kernel<<<total_tiles, 1024, 0, s>>>(d_frames, F);
+    shade_kernel<<<w.total_tiles, 1024, 0, s>>>(d_frames, F);
     VREN_LAUNCH_CHECK(ctx);
+    chain_mark(w, s);
     return VREN_OK;
 }
 
@@ -233,6 +234,9 @@ extern "C" VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint3
     f.out = output;
     FrameDev* slot = vren_next_frame_slot(ctx);
     st = vren_upload_frames(ctx, s, slot, &f, 1);
-    if (!st) st = launch_shade(ctx, s, slot, 1, f.Sx * f.Sy);
+    ChainWork w;
+    memset(&w, 0, sizeof(w));
+    w.total_tiles = f.Sx * f.Sy;
+    if (!st) st = launch_shade(ctx, s, slot, 1, w);
     return st;
 }

@@ -63,9 +63,33 @@ __global__ void synth_materials_kernel(uint64_t initstate, uint64_t n, float4* a
     metal[p] = m;
 }
 
+// PCG32 draws [chunk * 64, chunk * 64 + 64) of stream (initstate, initseq): one jump-ahead per 64 draws
+__global__ void synth_keys_kernel(uint64_t initstate, uint64_t initseq, uint64_t n, uint32_t* out)
+{
+    const uint64_t chunk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t start = chunk * 64;
+    if (start >= n) return;
+    vren_pcg32 r = vren_pcg32_at(initstate, initseq, start);
+    const uint64_t end = start + 64 < n ? start + 64 : n;
+    for (uint64_t i = start; i < end; i++) out[i] = vren_pcg32_next(&r);
+}
+
 } // namespace
 
 extern "C" {
+
+VREN_EXPORT int vren_synth_keys(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint64_t initseq,
+                                uint64_t n, uint32_t* out)
+{
+    if (!ctx) return VREN_EINVAL;
+    if (n == 0) return VREN_OK;
+    if (!out) return vren_fail(ctx, VREN_EINVAL, "synth_keys: null buffer");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "synth_keys: context has no device");
+    const uint64_t chunks = (n + 63) / 64;
+    synth_keys_kernel<<<(uint32_t)((chunks + 255) / 256), 256, 0, (hipStream_t)stream>>>(initstate, initseq, n, out);
+    VREN_LAUNCH_CHECK(ctx);
+    return VREN_OK;
+}
 
 VREN_EXPORT int vren_synth_lights(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t light_count,
                                   float* light_positions, float* point_lights)
