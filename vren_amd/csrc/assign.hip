@@ -285,7 +285,8 @@ __global__ void assign_write_kernel(const FrameDev* frames, uint32_t F, uint32_t
     }
 }
 
-uint32_t persistent_blocks(vren_ctx* ctx, uint32_t wpb) { return (uint32_t)ctx->num_cus * (wpb >= 4 ? 4 : 8); }
+// enough resident waves per CU (up to 32) to hide the dependent node loads of the traversals
+uint32_t persistent_blocks(vren_ctx* ctx, uint32_t wpb) { return (uint32_t)ctx->num_cus * (32u / wpb); }
 
 } // namespace
 

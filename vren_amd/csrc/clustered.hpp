@@ -20,6 +20,7 @@ struct FrameDev {
     double inv_proj_d[16];  // shade reconstruction (double)
     double inv_view_d[16];
     double eye[3];
+    double inv_W, inv_H;    // 1 / W, 1 / H (shade: pixel x/y only feed the zero coefficients of the z/w rows)
     float near_plane, far_plane;
     float slice_a;          // 2 tan(fov/2) / Sy + 1
     float log_a;            // vren_logf(slice_a)

@@ -1,10 +1,11 @@
 // clusters.hip -- find_unique_cluster_list (clustered_shading.cpp:363-448, find_unique_clusters.comp:46-121).
 //
-// One 1024-thread workgroup per 32x32 tile (flattened over every frame of a batch). Each thread derives its
-// pixel's 32-bit cluster key (depth slice from the D32 depth, cube-face normal code from the RGBA16F normal),
-// the tile's keys are sorted with a bitonic network (strides < 64 through wave64 shuffles, the ten strides
-// >= 64 through LDS), runs are reduced to unique keys, and the tile's unique count is chained to the previous
-// tiles of the same frame by a decoupled look-back. This replaces the reference's atomicAdd allocation (whose
+// One 256-thread workgroup per 32x32 tile (flattened over every frame of a batch); each thread derives the
+// 32-bit cluster keys of 4 adjacent pixels (depth slice from the D32 depth, cube-face normal code from the
+// RGBA16F normal). The reference sorts the tile's 1024 keys with a 55-step LDS bitonic network and 20 barriers;
+// here each wave extracts its unique keys in ascending order by repeated wave64 minima (one iteration per
+// distinct key -- a tile of a real scene holds a handful), wave 0 merges the four sorted wave lists the same
+// way, and the tile's unique count is chained to the previous tiles of the same frame by a decoupled look-back. This replaces the reference's atomicAdd allocation (whose
 // order across tiles is nondeterministic, find_unique_clusters.comp:100-103) by the canonical row-major tile
 // order, and needs no separate scan dispatch. Each pixel finds its key among the tile's unique keys by binary
 // search and stores the frame-wide cluster index.
@@ -88,16 +89,30 @@ __global__ void chain_init_kernel(uint64_t* status, uint32_t n_status, uint32_t*
     if (blockIdx.x == 0 && threadIdx.x < 4) tickets[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(1024) void clusters_kernel(const FrameDev* frames, uint32_t F, uint64_t* tile_status,
-                                                        uint32_t* ticket)
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
-    __shared__ uint32_t s_x[1024];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint64_t o = shfl_xor(v, m);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+constexpr uint64_t kNone = 1ull << 32; // larger than every 32-bit key
+
+// One 256-thread workgroup per 32x32 tile; thread t owns the 4 horizontally adjacent pixels
+// (4 * (t % 8) + q, t / 8), q = 0..3, so depth and normal loads are 16- and 32-byte vectors.
+__global__ __launch_bounds__(256) void clusters_kernel(const FrameDev* frames, uint32_t F, uint64_t* tile_status,
+                                                       uint32_t* ticket)
+{
+    __shared__ uint32_t s_wlist[4][256];
+    __shared__ uint32_t s_wn[4];
     __shared__ uint32_t s_unique[1024];
-    __shared__ uint32_t s_wave[16];
-    __shared__ uint32_t s_t;
-    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_t, s_count, s_prefix;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
+    const uint32_t wave = tid / VREN_WAVE;
     if (tid == 0) s_t = atomic_add_agent(ticket, 1u);
     __syncthreads();
     const uint32_t g = s_t;
@@ -105,61 +120,100 @@ __global__ __launch_bounds__(1024) void clusters_kernel(const FrameDev* frames, 
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-    const uint32_t gx = ti * 32 + (tid & 31), gy = tj * 32 + (tid >> 5);
-    const uint32_t key = pixel_key(fd, ti, tj, gx, gy);
+    const uint32_t gy = tj * 32 + (tid >> 3);
+    const uint32_t gx0 = ti * 32 + (tid & 7) * 4;
+    uint32_t key[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) key[q] = pixel_key(fd, ti, tj, gx0 + q, gy);
 
-    // bitonic sort of the tile's 1024 keys (ascending)
-    uint32_t v = key;
-    for (uint32_t k = 2; k <= 1024; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            uint32_t o;
-            if (j >= VREN_WAVE) {
-                s_x[tid] = v;
-                __syncthreads();
-                o = s_x[tid ^ j];
-                __syncthreads();
-            } else {
-                o = shfl_xor(v, (int)j);
+    // unique keys of the wave's 256 pixels, extracted in ascending order by repeated wave minima
+    uint32_t rem = 0xFu;
+    uint32_t n = 0;
+    while (true) {
+        uint64_t mine = kNone;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if ((rem >> q) & 1u) mine = (uint64_t)key[q] < mine ? (uint64_t)key[q] : mine;
+        const uint64_t m = wave_min_u64(mine);
+        if (m == kNone) break;
+        if (lane == 0) s_wlist[wave][n] = (uint32_t)m;
+        n++;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (key[q] == (uint32_t)m) rem &= ~(1u << q);
+    }
+    if (lane == 0) s_wn[wave] = n;
+    __syncthreads();
+    // merge the four sorted wave lists into the tile's sorted unique list (wave 0)
+    if (wave == 0) {
+        const uint32_t n0 = s_wn[0], n1 = s_wn[1], n2 = s_wn[2], n3 = s_wn[3];
+        const uint32_t total = n0 + n1 + n2 + n3;
+        uint32_t ent[16];
+        uint32_t emask = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t e = lane + 64u * i;
+            ent[i] = 0u;
+            if (e < total) {
+                uint32_t w = 0, idx = e;
+                if (idx >= n0) { idx -= n0; w = 1; if (idx >= n1) { idx -= n1; w = 2; if (idx >= n2) { idx -= n2; w = 3; } } }
+                ent[i] = s_wlist[w][idx];
+                emask |= 1u << i;
             }
-            const bool up = (tid & k) == 0;
-            const bool lower = (tid & j) == 0;
-            const uint32_t lo = v < o ? v : o, hi = v < o ? o : v;
-            v = (lower == up) ? lo : hi;
+        }
+        uint32_t u = 0;
+        while (true) {
+            uint64_t mine = kNone;
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if ((emask >> i) & 1u) mine = (uint64_t)ent[i] < mine ? (uint64_t)ent[i] : mine;
+            const uint64_t m = wave_min_u64(mine);
+            if (m == kNone) break;
+            if (lane == 0) s_unique[u] = (uint32_t)m;
+            u++;
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (ent[i] == (uint32_t)m) emask &= ~(1u << i);
+        }
+        // decoupled look-back over the frame's tiles (row-major order)
+        const uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, u);
+        if (lane == 0) {
+            s_count = u;
+            s_prefix = prefix;
         }
     }
-    // unique keys: keep the last of each run (find_unique_clusters.comp:87)
-    s_x[tid] = v;
     __syncthreads();
-    const bool keep = tid == 1023 || v != s_x[tid + 1];
-    uint32_t count;
-    const uint32_t rank = block_exclusive_sum<1024>(keep ? 1u : 0u, s_wave, &count);
-    if (keep) s_unique[rank] = v;
-    if (tid < VREN_WAVE) {
-        uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, count);
-        if (tid == 0) s_prefix = prefix;
-    }
-    __syncthreads();
-    const uint32_t prefix = s_prefix;
-    if (keep) {
-        uint32_t idx = prefix + rank;
-        if (idx < fd.key_cap) fd.keys[idx] = v;
+    const uint32_t count = s_count, prefix = s_prefix;
+    for (uint32_t i = tid; i < count; i += 256) {
+        const uint32_t idx = prefix + i;
+        if (idx < fd.key_cap) fd.keys[idx] = s_unique[i];
     }
     if (tid == 0 && t == fd.Sx * fd.Sy - 1) {
-        uint32_t total = prefix + count;
+        const uint32_t total = prefix + count;
         *fd.cluster_count = total;
         if (total > fd.key_cap) atomic_or_agent(fd.flags, VREN_FLAG_KEY_CAPACITY);
     }
-    if (gx < fd.W && gy < fd.H) {
-        // binary search of this pixel's key among the tile's unique keys
-        uint32_t lo = 0, hi = count;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (s_unique[mid] <= key) lo = mid;
-            else hi = mid;
+    if (gy < fd.H) {
+        uint32_t ref[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t lo = 0, hi = count;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_unique[mid] <= key[q]) lo = mid;
+                else hi = mid;
+            }
+            ref[q] = prefix + lo;
         }
-        fd.refs[(uint64_t)gy * fd.W + gx] = prefix + lo;
+        uint32_t* row = fd.refs + (uint64_t)gy * fd.W;
+        if (gx0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + gx0) & 15) == 0)) {
+            *reinterpret_cast<uint4*>(row + gx0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (gx0 + q < fd.W) row[gx0 + q] = ref[q];
+        }
     }
-    (void)lane;
 }
 
 } // namespace
@@ -185,7 +239,7 @@ int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w)
 
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    clusters_kernel<<<w.total_tiles, 1024, 0, s>>>(d_frames, F, w.tile_status, w.tickets + 0);
+    clusters_kernel<<<w.total_tiles, 256, 0, s>>>(d_frames, F, w.tile_status, w.tickets + 0);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

@@ -66,6 +66,8 @@ void vren_frame_setup(FrameDev* f, uint32_t W, uint32_t H, const vren_camera* ca
     mat4_inverse_d(view_d, f->inv_view_d);
     for (int i = 0; i < 16; i++) f->inv_proj[i] = (float)f->inv_proj_d[i];
     for (int i = 0; i < 3; i++) f->eye[i] = (double)cam->position[i];
+    f->inv_W = 1.0 / (double)W;
+    f->inv_H = 1.0 / (double)H;
     f->near_plane = cam->near_plane;
     f->far_plane = cam->far_plane;
     float tan_half = (float)std::tan((double)(cam->fov_y / 2.0f));

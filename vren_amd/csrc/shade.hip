@@ -1,14 +1,14 @@
 // shade.hip -- shade (clustered_shading.cpp:709-811, shade.comp:69-141, pbr.glsl:6-112) with resolved per-pixel
 // materials, for gfx950.
 //
-// One thread per pixel, one 1024-thread workgroup per 32x32 tile, flattened over every frame of a batch. The
+// One thread per pixel, one 256-thread workgroup per 16x16 quarter tile, flattened over every frame of a batch. The
 // reference evaluates everything in fp32, including inverse(P) and inverse(V) per invocation; that formula loses
 // ~1e-4 relative accuracy wherever a light sits close to the surface or a low roughness makes the GGX
 // denominator cancel. This kernel keeps the reference's light model (constants PI = 3.14, EPSILON = 0.001,
 // a = roughness, k = roughness, +0.0001 in the specular denominator, ambient 0.03 * albedo, gamma 1/2.2) and
 // reorganises the arithmetic so the fp32 result stays within ~1e-6 of an FP64 evaluation:
-//   - the pixel's world position is reconstructed in double (per pixel), and d = p - light, |d| and the
-//     attenuation are evaluated in double per light (the attenuation cancels near the light's radius);
+//   - the pixel's world position is reconstructed in double (per pixel), d = p - light and |d| are evaluated in
+//     double per light, and the attenuation through (r - |d|) / r, where it cancels near the light's radius;
 //   - the GGX denominator n.h^2 (a^2 - 1) + 1 is evaluated as (1 - |n|^2) + |n x h|^2 / |h|^2 + a^2 (n.h)^2 / |h|^2
 //     with the unnormalised half vector h = v - l, a sum of non-negative terms (no cancellation near n.h = 1);
 //   - contributions that are exactly zero in the reference formula (attenuation 0 or n.l <= 0) are skipped.
@@ -28,7 +28,9 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
 {
 #pragma clang fp contract(off)
     PixelIn o;
-    double v[4] = {(double)gx / (double)fd.W, (double)gy / (double)fd.H, (double)depth, 1.0};
+    // view depth and the background test exactly as the oracle evaluates them (shade.comp:75-86): the projection's
+    // inverse has zero x/y coefficients in its z and w rows, so r[2] and r[3] do not depend on the pixel's x/y
+    double v[4] = {(double)gx * fd.inv_W, (double)gy * fd.inv_H, (double)depth, 1.0};
     v[1] = 1.0 - v[1];
     v[0] = v[0] * 2.0 - 1.0;
     v[1] = v[1] * 2.0 - 1.0;
@@ -37,11 +39,13 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
         r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) + fd.inv_proj_d[2 * 4 + i] * v[2]) +
                fd.inv_proj_d[3 * 4 + i] * v[3];
     const double w = r[3];
-    for (int i = 0; i < 4; i++) r[i] = r[i] / w;
-    o.shaded = r[2] < (double)fd.bg_threshold;
+    const double z = r[2] / w;
+    o.shaded = z < (double)fd.bg_threshold;
+    const double iw = 1.0 / w;
+    const double q[4] = {r[0] * iw, r[1] * iw, z, 1.0};
     for (int i = 0; i < 3; i++)
-        o.P[i] = ((fd.inv_view_d[0 * 4 + i] * r[0] + fd.inv_view_d[1 * 4 + i] * r[1]) + fd.inv_view_d[2 * 4 + i] * r[2]) +
-                 fd.inv_view_d[3 * 4 + i] * r[3];
+        o.P[i] = ((fd.inv_view_d[0 * 4 + i] * q[0] + fd.inv_view_d[1 * 4 + i] * q[1]) + fd.inv_view_d[2 * 4 + i] * q[2]) +
+                 fd.inv_view_d[3 * 4 + i] * q[3];
     return o;
 }
 
@@ -89,76 +93,147 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
     }
 }
 
-__global__ __launch_bounds__(1024) void shade_kernel(const FrameDev* frames, uint32_t F)
+constexpr uint32_t kStage = 512; // lights staged in LDS per 16x16 block
+
+// One 256-thread workgroup per 16x16 quarter of a 32x32 tile. The clusters of a tile are numbered consecutively
+// and their light lists are consecutive in the index buffer, so the block first reduces the range of clusters
+// its shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS,
+// and then shades each pixel from LDS instead of chasing index -> light through global memory per light.
+__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint32_t F)
 {
-    const uint32_t g = blockIdx.x;
+    __shared__ float4 s_lpos[kStage];
+    __shared__ float4 s_lcol[kStage];
+    __shared__ uint32_t s_min[4], s_max[4];
+    __shared__ uint32_t s_start, s_n, s_staged;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
+    const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
     const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-    const uint32_t gx = ti * 32 + (threadIdx.x & 31), gy = tj * 32 + (threadIdx.x >> 5);
-    if (gx >= fd.W || gy >= fd.H) return;
-    const uint64_t p = (uint64_t)gy * fd.W + gx;
-    const PixelIn px = reconstruct(fd, gx, gy, fd.depth[p]);
-    if (!px.shaded) return;
-    Surface s;
-    double V[3], vl = 0.0;
-    for (int i = 0; i < 3; i++) {
-        V[i] = fd.eye[i] - px.P[i];
-        vl += V[i] * V[i];
+    const uint32_t gx = ti * 32 + (sub & 1) * 16 + (tid & 15), gy = tj * 32 + (sub >> 1) * 16 + (tid >> 4);
+    const bool in = gx < fd.W && gy < fd.H;
+    const uint64_t p = in ? (uint64_t)gy * fd.W + gx : 0;
+    PixelIn px;
+    px.shaded = false;
+    uint2 nb = make_uint2(0u, 0u);
+    float4 ar = make_float4(0.f, 0.f, 0.f, 0.f);
+    float metal = 0.0f;
+    if (in) {
+        const float depth = fd.depth[p];
+        nb = reinterpret_cast<const uint2*>(fd.normals)[p];
+        ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
+        metal = fd.metallic[p];
+        px = reconstruct(fd, gx, gy, depth);
     }
-    vl = sqrt(vl);
-    for (int i = 0; i < 3; i++) s.V[i] = (float)(V[i] / vl);
-    const uint2 nb = reinterpret_cast<const uint2*>(fd.normals)[p];
+    const uint32_t cl = px.shaded ? fd.refs[p] : 0xFFFFFFFFu;
+    const bool has_list = px.shaded && fd.L > 0 && cl < fd.key_cap;
+    // range of referenced clusters
+    uint32_t cmin = has_list ? cl : 0xFFFFFFFFu, cmax = has_list ? cl : 0u;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        cmin = min(cmin, shfl_xor(cmin, m));
+        cmax = max(cmax, shfl_xor(cmax, m));
+    }
+    if (lane == 0) {
+        s_min[wave] = cmin;
+        s_max[wave] = cmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+        const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+        uint32_t start = 0, n = 0;
+        if (a != 0xFFFFFFFFu) {
+            start = min(fd.offsets[a], fd.index_cap);
+            const uint64_t e = (uint64_t)fd.offsets[b] + fd.counts[b];
+            const uint32_t end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+            n = end > start ? end - start : 0;
+        }
+        s_start = start;
+        s_n = n;
+        s_staged = n <= kStage ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t start = s_start, nstage = s_n;
+    const bool staged = s_staged != 0;
+    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
+    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
+    if (staged) {
+        for (uint32_t i = tid; i < nstage; i += 256) {
+            const uint32_t li = fd.indices[start + i];
+            if (li < fd.L) {
+                s_lpos[i] = lpos[li];
+                s_lcol[i] = lcol[li];
+            } else {
+                s_lcol[i] = make_float4(0.f, 0.f, 0.f, 0.f); // radius 0: never contributes
+                s_lpos[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+    __syncthreads();
+    if (!px.shaded) return;
+
+    Surface s;
+    {
+        const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
+        const float il = rsqrtf(dot3(V, V));
+        for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
+    }
     s.N[0] = vren_half_to_float((uint16_t)(nb.x & 0xFFFFu));
     s.N[1] = vren_half_to_float((uint16_t)(nb.x >> 16));
     s.N[2] = vren_half_to_float((uint16_t)(nb.y & 0xFFFFu));
     s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
-    const float4 ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
     s.albedo[0] = ar.x;
     s.albedo[1] = ar.y;
     s.albedo[2] = ar.z;
     s.rough = ar.w;
-    s.metal = fd.metallic[p];
+    s.metal = metal;
     s.a2 = s.rough * s.rough;
     for (int c = 0; c < 3; c++) s.F0[c] = 0.04f * (1.0f - s.metal) + s.albedo[c] * s.metal;
     s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
     s.ggxV = s.NdotV / (s.NdotV * (1.0f - s.rough) + s.rough);
 
     float acc[3] = {0.0f, 0.0f, 0.0f};
-    const uint32_t cl = fd.refs[p];
-    if (fd.L > 0 && cl < fd.key_cap) {
+    if (has_list) {
         const uint32_t cnt = fd.counts[cl];
         const uint32_t off = fd.offsets[cl];
         const uint32_t end = (uint64_t)off + cnt > fd.index_cap ? fd.index_cap : off + cnt;
-        const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
-        const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
         for (uint32_t e = off; e < end; e++) {
-            const uint32_t li = fd.indices[e];
-            if (li >= fd.L) continue;
-            const float4 lp = lpos[li];
-            const float4 lc = lcol[li];
-            // d = p - light (pbr.glsl:87), its length and the attenuation in double: near the edge of the light's
-            // radius 1 - (d / r)^1.5 (or 1 - (d / r)^4) cancels, and a float distance would leave ~1e-7 / (1 - d / r)
-            // relative error on a contribution that can dominate the pixel
+            float4 lp, lc;
+            if (staged) {
+                lp = s_lpos[e - start];
+                lc = s_lcol[e - start];
+            } else {
+                const uint32_t li = fd.indices[e];
+                if (li >= fd.L) continue;
+                lp = lpos[li];
+                lc = lcol[li];
+            }
+            // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
+            // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
             const double d[3] = {px.P[0] - (double)lp.x, px.P[1] - (double)lp.y, px.P[2] - (double)lp.z};
             const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-            if (!(dist2 > 0.0)) continue;
-            const double dist = sqrt(dist2);
-            const double q = dist / (double)lc.w;
-            double attd;
+            const float r = lc.w;
+            if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < (double)r * (double)r)) continue;
+            // sqrt in double from a float estimate and one Newton step (relative error ~1e-14)
+            const float s0 = sqrtf((float)dist2);
+            const double dist = 0.5 * ((double)s0 + dist2 / (double)s0);
+            const float inv_r = 1.0f / r;
+            const float omx = (float)((double)r - dist) * inv_r; // 1 - x, accurate even when tiny
+            const float x = 1.0f - omx;
+            float att;
             if (fd.attenuation == VREN_ATTEN_SMOOTH) {
-                const double q2 = q * q;
-                const double w = 1.0 - q2 * q2;
-                attd = w > 0.0 ? w * w : 0.0;
+                // (1 - x^4)^2 = ((1 - x)(1 + x)(1 + x^2))^2
+                const float w = omx * (1.0f + x) * (1.0f + x * x);
+                att = w * w;
             } else {
-                const double x = q < 1.0 ? q : 1.0;
-                attd = 1.0 - x * sqrt(x);
+                // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5)
+                att = omx * (1.0f + x + x * x) / (1.0f + x * sqrtf(x));
             }
-            if (!(attd > 0.0)) continue;
-            const double inv = 1.0 / dist;
-            const float L[3] = {(float)(d[0] * inv), (float)(d[1] * inv), (float)(d[2] * inv)};
-            const float att = (float)attd;
+            if (!(att > 0.0f)) continue;
+            const float inv = 1.0f / (float)dist;
+            const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
             const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
             apply_light(s, L, rad, acc);
         }
@@ -183,7 +258,7 @@ __global__ __launch_bounds__(1024) void shade_kernel(const FrameDev* frames, uin
 
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    shade_kernel<<<w.total_tiles, 1024, 0, s>>>(d_frames, F);
+    shade_kernel<<<w.total_tiles * 4, 256, 0, s>>>(d_frames, F);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
