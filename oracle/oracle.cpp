@@ -1111,6 +1111,51 @@ void shade_frame(uint32_t W, uint32_t H, const vren_camera* cam, const vren_clus
         }
 }
 
+
+/* one pixel of shade_frame<double>: returns false for background, writes rgb */
+bool shade_pixel_f64(const frame_consts& c, const vren_camera* cam, const vren_cluster_cfg* cfg, uint32_t gx,
+                     uint32_t gy, const float* depth, const uint16_t* normals, const float* albedo_rough,
+                     const float* metallic, const uint32_t* refs, uint32_t key_capacity, const uint32_t* counts,
+                     const uint32_t* offsets, const uint32_t* indices, uint64_t index_capacity, const double* pos4,
+                     const double* light4, double out[3], double* albedo_override, double* rough_override)
+{
+    typedef double T;
+    uint64_t p = (uint64_t)gy * c.W + gx;
+    v3<T> P;
+    if (!reconstruct<T>(c, gx, gy, depth[p], P)) return false;
+    v3<T> N = mk((T)vren_half_to_float(normals[4 * p]), (T)vren_half_to_float(normals[4 * p + 1]),
+                 (T)vren_half_to_float(normals[4 * p + 2]));
+    v3<T> albedo = mk((T)albedo_rough[4 * p], (T)albedo_rough[4 * p + 1], (T)albedo_rough[4 * p + 2]);
+    T rough = (T)albedo_rough[4 * p + 3];
+    if (albedo_override) albedo = mk(albedo_override[0], albedo_override[1], albedo_override[2]);
+    if (rough_override) rough = *rough_override;
+    T metal = (T)metallic[p];
+    v3<T> eye = mk((T)cam->position[0], (T)cam->position[1], (T)cam->position[2]);
+    v3<T> V = normalize(eye - P);
+    v3<T> Lo = mk(T(0), T(0), T(0));
+    uint32_t cl = refs[p];
+    if (cl < key_capacity) {
+        uint32_t cnt = counts[cl], off = offsets[cl];
+        for (uint32_t e = 0; e < cnt; e++) {
+            if ((uint64_t)off + e >= index_capacity) break;
+            uint32_t li = indices[off + e];
+            v3<T> lp = mk(pos4[4 * li], pos4[4 * li + 1], pos4[4 * li + 2]);
+            v3<T> d = P - lp;
+            T dist = std::sqrt(dot(d, d));
+            v3<T> L = mk(d.x / dist, d.y / dist, d.z / dist);
+            T att = attenuation<T>(dist, light4[4 * li + 3], cfg->attenuation);
+            v3<T> rad = mk(light4[4 * li] * att, light4[4 * li + 1] * att, light4[4 * li + 2] * att);
+            Lo = Lo + pbr_apply_light<T>(V, N, L, rad, albedo, metal, rough);
+        }
+    }
+    v3<T> col = albedo * T(0.03f) + Lo;
+    T g = T(1) / T(2.2f);
+    out[0] = std::pow(col.x, g);
+    out[1] = std::pow(col.y, g);
+    out[2] = std::pow(col.z, g);
+    return true;
+}
+
 } // namespace
 
 OEXPORT void o_shade_f64(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
@@ -1131,4 +1176,181 @@ OEXPORT void o_shade_f32(uint32_t W, uint32_t H, const vren_camera* cam, const v
 {
     shade_frame<float>(W, H, cam, cfg, depth, normals, albedo_rough, metallic, refs, key_capacity, counts, offsets,
                        indices, index_capacity, pos4, light4, dir_lights, n_dir, out, shaded_mask);
+}
+
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Differentiable shade (no reference counterpart): L2 loss and its FP64 adjoint with the light lists held fixed.
+ * Parameters are the light positions and light records (rgb = colour * intensity, radius) in double, and the
+ * per-pixel albedo / roughness. Use the smooth attenuation (VREN_ATTEN_SMOOTH).
+ * ---------------------------------------------------------------------------------------------------------- */
+
+/* sum over the listed pixels (background pixels contribute nothing) of |out - target|^2 over rgb */
+OEXPORT double o_shade_loss_f64(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
+                                const float* depth, const uint16_t* normals, const float* albedo_rough,
+                                const float* metallic, const uint32_t* refs, uint32_t key_capacity,
+                                const uint32_t* counts, const uint32_t* offsets, const uint32_t* indices,
+                                uint64_t index_capacity, const double* pos4, const double* light4,
+                                const float* target, const uint32_t* pixels, uint64_t n_pixels,
+                                const double* pixel_albedo_rough)
+{
+    frame_consts c = make_consts(W, H, cam);
+    double loss = 0.0;
+    for (uint64_t i = 0; i < n_pixels; i++) {
+        uint32_t p = pixels[i];
+        double out[3];
+        double alb[3], rgh;
+        double* ao = nullptr;
+        double* ro = nullptr;
+        if (pixel_albedo_rough) {
+            alb[0] = pixel_albedo_rough[4 * i];
+            alb[1] = pixel_albedo_rough[4 * i + 1];
+            alb[2] = pixel_albedo_rough[4 * i + 2];
+            rgh = pixel_albedo_rough[4 * i + 3];
+            ao = alb;
+            ro = &rgh;
+        }
+        if (!shade_pixel_f64(c, cam, cfg, p % W, p / W, depth, normals, albedo_rough, metallic, refs, key_capacity,
+                             counts, offsets, indices, index_capacity, pos4, light4, out, ao, ro))
+            continue;
+        for (int ch = 0; ch < 3; ch++) {
+            double e = out[ch] - (double)target[4 * (uint64_t)p + ch];
+            loss += e * e;
+        }
+    }
+    return loss;
+}
+
+/* FP64 adjoint: given dLoss/d(output rgb) per pixel, ACCUMULATES into g_pos[L*4] (xyz), g_light[L*4]
+ * (rgb, radius) and g_ar[W*H*4] (albedo rgb, roughness). Derived by hand from pbr.glsl:42-93 with the smooth window
+ * att = (1 - (d/r)^4)^2; clamps max(., 0) contribute no gradient where they are active. */
+OEXPORT void o_shade_backward_f64(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
+                                  const float* depth, const uint16_t* normals, const float* albedo_rough,
+                                  const float* metallic, const uint32_t* refs, uint32_t key_capacity,
+                                  const uint32_t* counts, const uint32_t* offsets, const uint32_t* indices,
+                                  uint64_t index_capacity, const double* pos4, const double* light4,
+                                  const float* grad_out, double* g_pos, double* g_light, double* g_ar)
+{
+    typedef double T;
+    frame_consts c = make_consts(W, H, cam);
+    const T PI = T(k_pi_ref);
+    v3<T> eye = mk((T)cam->position[0], (T)cam->position[1], (T)cam->position[2]);
+    for (uint32_t gy = 0; gy < H; gy++)
+        for (uint32_t gx = 0; gx < W; gx++) {
+            uint64_t p = (uint64_t)gy * W + gx;
+            v3<T> P;
+            if (!reconstruct<T>(c, gx, gy, depth[p], P)) continue;
+            uint32_t cl = refs[p];
+            v3<T> N = mk((T)vren_half_to_float(normals[4 * p]), (T)vren_half_to_float(normals[4 * p + 1]),
+                         (T)vren_half_to_float(normals[4 * p + 2]));
+            T alb[3] = {(T)albedo_rough[4 * p], (T)albedo_rough[4 * p + 1], (T)albedo_rough[4 * p + 2]};
+            T rough = (T)albedo_rough[4 * p + 3], metal = (T)metallic[p];
+            v3<T> V = normalize(eye - P);
+            T NdotV = mx0(dot(N, V));
+            T k = rough, a2 = rough * rough;
+            T gvden = NdotV * (T(1) - k) + k;
+            T ggxV = NdotV / gvden;
+            T F0[3];
+            for (int ch = 0; ch < 3; ch++) F0[ch] = T(0.04f) * (T(1) - metal) + alb[ch] * metal;
+            uint32_t cnt = 0, off = 0;
+            if (cl < key_capacity) {
+                cnt = counts[cl];
+                off = offsets[cl];
+            }
+            struct Pair {
+                uint32_t li;
+                v3<T> L;
+                T dist, r, x, w, att;
+            };
+            std::vector<Pair> pairs;
+            T col[3] = {T(0.03f) * alb[0], T(0.03f) * alb[1], T(0.03f) * alb[2]};
+            for (uint32_t e = 0; e < cnt; e++) {
+                if ((uint64_t)off + e >= index_capacity) break;
+                uint32_t li = indices[off + e];
+                v3<T> lp = mk(pos4[4 * li], pos4[4 * li + 1], pos4[4 * li + 2]);
+                v3<T> d = P - lp;
+                T dist = std::sqrt(dot(d, d));
+                Pair pr;
+                pr.li = li;
+                pr.dist = dist;
+                pr.r = light4[4 * li + 3];
+                pr.x = dist / pr.r;
+                pr.w = T(1) - pr.x * pr.x * pr.x * pr.x;
+                pr.att = attenuation<T>(dist, pr.r, cfg->attenuation);
+                pr.L = mk(d.x / dist, d.y / dist, d.z / dist);
+                v3<T> rad = mk(light4[4 * li] * pr.att, light4[4 * li + 1] * pr.att, light4[4 * li + 2] * pr.att);
+                v3<T> ctb = pbr_apply_light<T>(V, N, pr.L, rad, mk(alb[0], alb[1], alb[2]), metal, rough);
+                col[0] += ctb.x;
+                col[1] += ctb.y;
+                col[2] += ctb.z;
+                if (pr.att > T(0) && pr.w > T(0)) pairs.push_back(pr);
+            }
+            T g = T(1) / T(2.2f);
+            T gT[3];
+            for (int ch = 0; ch < 3; ch++) {
+                gT[ch] = (T)grad_out[4 * p + ch] * g * std::pow(col[ch], g - T(1));
+                g_ar[4 * p + ch] += T(0.03f) * gT[ch];
+            }
+            for (const Pair& pr : pairs) {
+                const uint32_t li = pr.li;
+                v3<T> mL = mk(-pr.L.x, -pr.L.y, -pr.L.z);
+                T NdotL = dot(N, mL);
+                if (!(NdotL > T(0))) continue;
+                v3<T> h = V + mL;
+                T hl = std::sqrt(dot(h, h));
+                v3<T> Hn = mk(h.x / hl, h.y / hl, h.z / hl);
+                T NHraw = dot(N, Hn);
+                T NdotH = mx0(NHraw);
+                T inner = NdotH * NdotH * (a2 - T(1)) + T(1);
+                T NDF = a2 / (PI * inner * inner);
+                T glden = NdotL * (T(1) - k) + k;
+                T glv = NdotL / glden;
+                T G = ggxV * glv;
+                T HVraw = dot(Hn, V);
+                T HdotV = mx0(HVraw);
+                T om = T(1) - HdotV;
+                T f5 = std::pow(om, T(5));
+                T D = T(4) * NdotV * NdotL + T(0.0001f);
+                T spec = NDF * G / D;
+                T km = T(1) - metal;
+                T lc[3] = {light4[4 * li], light4[4 * li + 1], light4[4 * li + 2]};
+                T g_att = 0, g_NdotL = 0, g_spec = 0, g_f5 = 0;
+                for (int ch = 0; ch < 3; ch++) {
+                    T F = F0[ch] + (T(1) - F0[ch]) * f5;
+                    T kD = (T(1) - F) * km;
+                    T B = kD * alb[ch] / PI + F * spec;
+                    g_light[4 * li + ch] += gT[ch] * B * pr.att * NdotL;
+                    T gB = gT[ch] * lc[ch] * pr.att * NdotL;
+                    g_att += gT[ch] * B * lc[ch] * NdotL;
+                    g_NdotL += gT[ch] * B * lc[ch] * pr.att;
+                    g_spec += gB * F;
+                    T gF = gB * (spec - alb[ch] * km / PI);
+                    g_ar[4 * p + ch] += gB * kD / PI + gF * (T(1) - f5) * metal;
+                    g_f5 += gF * (T(1) - F0[ch]);
+                }
+                T g_HdotV = HVraw > T(0) ? g_f5 * (T(-5) * std::pow(om, T(4))) : T(0);
+                T g_NDF = g_spec * G / D, g_G = g_spec * NDF / D;
+                g_NdotL += -g_spec * spec / D * T(4) * NdotV;
+                T g_gl = g_G * ggxV, g_ggxV = g_G * glv;
+                g_NdotL += g_gl * k / (glden * glden);
+                T g_k = g_gl * (-NdotL * (T(1) - NdotL) / (glden * glden)) +
+                        g_ggxV * (-NdotV * (T(1) - NdotV) / (gvden * gvden));
+                T g_inner = g_NDF * (T(-2) * NDF / inner);
+                T g_a2 = g_NDF / (PI * inner * inner) + g_inner * NdotH * NdotH;
+                T g_NdotH = NHraw > T(0) ? g_inner * T(2) * NdotH * (a2 - T(1)) : T(0);
+                g_ar[4 * p + 3] += g_a2 * T(2) * rough + g_k;
+                T gh[3], gL[3];
+                T Ncomp[3] = {N.x, N.y, N.z}, Vcomp[3] = {V.x, V.y, V.z}, Hcomp[3] = {Hn.x, Hn.y, Hn.z};
+                T Lcomp[3] = {pr.L.x, pr.L.y, pr.L.z};
+                for (int i = 0; i < 3; i++) {
+                    gh[i] = (g_NdotH * (Ncomp[i] - NdotH * Hcomp[i]) + g_HdotV * (Vcomp[i] - HdotV * Hcomp[i])) / hl;
+                    gL[i] = -(gh[i] + g_NdotL * Ncomp[i]);
+                }
+                T datt_dx = T(-8) * pr.x * pr.x * pr.x * pr.w;
+                T g_dist = g_att * datt_dx / pr.r;
+                T gLL = gL[0] * Lcomp[0] + gL[1] * Lcomp[1] + gL[2] * Lcomp[2];
+                for (int i = 0; i < 3; i++) g_pos[4 * li + i] -= (gL[i] - gLL * Lcomp[i]) / pr.dist + g_dist * Lcomp[i];
+                g_light[4 * li + 3] += g_att * datt_dx * (-pr.x / pr.r);
+            }
+        }
 }

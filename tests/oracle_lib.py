@@ -239,3 +239,26 @@ class ClusteredOracle:
            c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(pos), _p(lig),
            _p(dirl) if dirl is not None else None, c_u32(nd), _p(out), _p(mask))
         return out, mask.astype(bool)
+
+    def shade_loss(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos_d, lig_d,
+                   target, pixels, pixel_ar=None):
+        self.lib.o_shade_loss_f64.restype = ctypes.c_double
+        pixels = np.ascontiguousarray(pixels, np.uint32)
+        return self.lib.o_shade_loss_f64(
+            c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(ar), _p(metal), _p(refs),
+            c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(np.ascontiguousarray(pos_d)),
+            _p(np.ascontiguousarray(lig_d)), _p(target), _p(pixels), c_u64(pixels.size),
+            _p(np.ascontiguousarray(pixel_ar, np.float64)) if pixel_ar is not None else None)
+
+    def shade_backward(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos_d,
+                       lig_d, grad_out):
+        L = pos_d.shape[0]
+        g_pos = np.zeros((L, 4), np.float64)
+        g_lig = np.zeros((L, 4), np.float64)
+        g_ar = np.zeros((W * H, 4), np.float64)
+        self.lib.o_shade_backward_f64(
+            c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(ar), _p(metal), _p(refs),
+            c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(np.ascontiguousarray(pos_d)),
+            _p(np.ascontiguousarray(lig_d)), _p(np.ascontiguousarray(grad_out, np.float32)), _p(g_pos), _p(g_lig),
+            _p(g_ar))
+        return g_pos, g_lig, g_ar

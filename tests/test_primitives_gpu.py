@@ -151,3 +151,39 @@ def test_radix_sort_headline_size_properties(vctx, oracle):
     eq = ku[1:] == ku[:-1]
     assert bool((dv[1:][eq] > dv[:-1][eq]).all())                            # stable: ties keep input order
     del seen, orig
+
+
+@pytest.mark.parametrize("n_buckets,shift", [(65536, 0), (64, 24), (1, 0)])
+def test_bucket_sort(vctx, oracle, n_buckets, shift):
+    # bucket_sort.main (vren_test/vren_test/primitives/bucket_sort.cpp:182-192): N = 1, then x(4..6) while < 10^7,
+    # key = rand() % 65536, value = i -- here seeded PCG32 draws and a fixed growth sequence
+    n = 1
+    sizes = []
+    while n < 3_000_000:
+        sizes.append(n)
+        n *= 5
+    for n in sizes:
+        raw = oracle.pcg32(4242, 54, n)
+        keys = raw % 65536 if n_buckets == 65536 else raw >> 2  # 30-bit codes for the 64 coarse buckets
+        pairs = np.stack([keys.astype(np.uint32), np.arange(n, dtype=np.uint32)], axis=1)
+        out = empty_dev(8 * n)
+        offs = empty_dev(4 * n_buckets)
+        vctx.bucket_sort(to_dev(pairs), n, out, offs, n_buckets, shift)
+        opairs, ooffs = oracle.bucket_sort(pairs, n_buckets, shift)
+        assert np.array_equal(from_dev(out, np.uint32, (n, 2)), opairs), n
+        assert np.array_equal(from_dev(offs, np.uint32, (n_buckets,)), ooffs), n
+
+
+@pytest.mark.parametrize("n", [0, 1, 10, 100, 1000, 10000])
+def test_build_bvh_standalone(vctx, oracle, n):
+    # build_bvh.main (vren_test/vren_test/primitives/build_bvh.cpp:213-221): random AABBs, pos and extent U[0,100]
+    from test_oracle_primitives import BVH_DTYPE, random_leaves
+    rng = np.random.default_rng(100 + n)
+    padded = oracle.calc("padded_leaf_count", n)
+    length = oracle.calc("buffer_length", n)
+    nodes = random_leaves(n, padded, length, rng)
+    d = to_dev(nodes)
+    vctx.build_bvh(d, padded)
+    got = from_dev(d, BVH_DTYPE, (length,))
+    want = oracle.build_bvh(nodes, padded)
+    assert np.array_equal(got.view(np.uint32).reshape(-1, 8)[:, :7], want.view(np.uint32).reshape(-1, 8)[:, :7])

@@ -9,3 +9,8 @@ int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t*
                                 uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes);
 // configuration for this problem: forced, tuned (cached), or the default
 int vren_radix_pick_config(vren_ctx* ctx, hipStream_t s, uint64_t n, bool vals, uint32_t key_bits);
+
+// exclusive scan with a caller-provided look-back workspace of vren_scan_work_bytes(n, blocks)
+size_t vren_scan_work_bytes(uint64_t n, uint32_t blocks);
+int vren_scan_with_work(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t blocks,
+                        void* work);

@@ -91,25 +91,37 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
 
 } // namespace
 
-extern "C" VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in,
-                                                   uint32_t* out, uint64_t n, uint32_t blocks)
+size_t vren_scan_work_bytes(uint64_t n, uint32_t blocks)
 {
-    if (!ctx) return VREN_EINVAL;
-    if (!in || !out || n == 0 || blocks == 0)
-        return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: null buffer, n == 0 or blocks == 0");
+    return 256 + ceil_div_u64(n, kTile) * blocks * sizeof(uint64_t);
+}
+
+// exclusive scan with caller-provided look-back workspace (vren_scan_work_bytes)
+int vren_scan_with_work(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t blocks,
+                        void* work)
+{
     uint64_t tps = ceil_div_u64(n, kTile);
     uint64_t tiles = tps * blocks;
-    if (tiles >= (1ull << 31)) return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: input too large");
-    size_t bytes = 256 + tiles * sizeof(uint64_t);
-    int st = vren_ensure_scratch(ctx, bytes);
-    if (st) return st;
-    uint32_t* counter = static_cast<uint32_t*>(ctx->scratch);
-    uint64_t* status = reinterpret_cast<uint64_t*>(static_cast<char*>(ctx->scratch) + 256);
-    hipStream_t s = (hipStream_t)stream;
+    uint32_t* counter = static_cast<uint32_t*>(work);
+    uint64_t* status = reinterpret_cast<uint64_t*>(static_cast<char*>(work) + 256);
     uint32_t init_blocks = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, ceil_div_u64(tiles, 256)));
     scan_init_kernel<<<init_blocks, 256, 0, s>>>(status, tiles, counter);
     VREN_LAUNCH_CHECK(ctx);
     scan_kernel<<<(uint32_t)tiles, kThreads, 0, s>>>(in, out, n, (uint32_t)tps, status, counter);
     VREN_LAUNCH_CHECK(ctx);
     return VREN_OK;
+}
+
+extern "C" VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in,
+                                                   uint32_t* out, uint64_t n, uint32_t blocks)
+{
+    if (!ctx) return VREN_EINVAL;
+    if (!in || !out || n == 0 || blocks == 0)
+        return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: null buffer, n == 0 or blocks == 0");
+    if (ceil_div_u64(n, kTile) * blocks >= (1ull << 31))
+        return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: input too large");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: context has no device");
+    int st = vren_ensure_scratch(ctx, vren_scan_work_bytes(n, blocks));
+    if (st) return st;
+    return vren_scan_with_work(ctx, (hipStream_t)stream, in, out, n, blocks, ctx->scratch);
 }

@@ -254,6 +254,307 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     reinterpret_cast<float4*>(fd.out)[p] = o;
 }
 
+
+// ---- adjoint ---------------------------------------------------------------------------------------------------
+
+struct LightTerm {
+    float L[3];       // unit direction light -> surface
+    float dist, r, x, w, att;
+    bool live;
+};
+
+// forward quantities of one (pixel, light) pair shared by the two passes of the adjoint
+__device__ __forceinline__ LightTerm light_term(const double P[3], const float4& lp, const float4& lc, uint32_t atten)
+{
+    LightTerm t;
+    t.live = false;
+    const double d[3] = {P[0] - (double)lp.x, P[1] - (double)lp.y, P[2] - (double)lp.z};
+    const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+    t.r = lc.w;
+    if (!(dist2 > 0.0) || !(t.r > 0.0f) || !(dist2 < (double)t.r * (double)t.r)) return t;
+    const float s0 = sqrtf((float)dist2);
+    const double dist = 0.5 * ((double)s0 + dist2 / (double)s0);
+    const float omx = (float)((double)t.r - dist) * (1.0f / t.r);
+    t.x = 1.0f - omx;
+    if (atten == VREN_ATTEN_SMOOTH) {
+        t.w = omx * (1.0f + t.x) * (1.0f + t.x * t.x);
+        t.att = t.w * t.w;
+    } else {
+        t.w = 0.0f;
+        t.att = omx * (1.0f + t.x + t.x * t.x) / (1.0f + t.x * sqrtf(t.x));
+    }
+    t.dist = (float)dist;
+    const float inv = 1.0f / t.dist;
+    t.L[0] = (float)d[0] * inv;
+    t.L[1] = (float)d[1] * inv;
+    t.L[2] = (float)d[2] * inv;
+    t.live = t.att > 0.0f;
+    return t;
+}
+
+// Adjoint of one pair's contribution (pbr.glsl:42-93 with the smooth window). gT = dLoss/d(contribution rgb).
+// Accumulates d/d(light position), d/d(light rgb), d/d(radius) into gl[7] and d/d(albedo, roughness) into gp[4].
+__device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm& t, const float4& lc, const float gT[3],
+                                              float gl[7], float gp[4])
+{
+    const float mL[3] = {-t.L[0], -t.L[1], -t.L[2]};
+    const float NdotL = dot3(s.N, mL);
+    if (!(NdotL > 0.0f)) return;
+    const float h[3] = {s.V[0] + mL[0], s.V[1] + mL[1], s.V[2] + mL[2]};
+    const float hh = dot3(h, h);
+    if (!(hh > 0.0f)) return;
+    const float ih = rsqrtf(hh);
+    const float H[3] = {h[0] * ih, h[1] * ih, h[2] * ih};
+    const float Nh = dot3(s.N, h);
+    float NdotH = 0.0f, inner = 1.0f;
+    if (Nh > 0.0f) {
+        const float c[3] = {s.N[1] * h[2] - s.N[2] * h[1], s.N[2] * h[0] - s.N[0] * h[2], s.N[0] * h[1] - s.N[1] * h[0]};
+        const float inv_hh = ih * ih;
+        NdotH = Nh * ih;
+        inner = s.nn + dot3(c, c) * inv_hh + s.a2 * (Nh * Nh) * inv_hh;
+    }
+    const float NDF = s.a2 / (kPi * inner * inner);
+    const float k = s.rough;
+    const float gl_den = NdotL * (1.0f - k) + k;
+    const float glv = NdotL / gl_den;
+    const float G = s.ggxV * glv;
+    const float HdotVraw = dot3(h, s.V) * ih;
+    const float HdotV = fmaxf(HdotVraw, 0.0f);
+    const float om = 1.0f - HdotV;
+    const float om2 = om * om;
+    const float f5 = om2 * om2 * om;
+    const float D = 4.0f * s.NdotV * NdotL + 0.0001f;
+    const float spec = NDF * G / D;
+    const float km = 1.0f - s.metal;
+    float g_att = 0.0f, g_NdotL = 0.0f, g_spec = 0.0f, g_f5 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float F = s.F0[c] + (1.0f - s.F0[c]) * f5;
+        const float kD = (1.0f - F) * km;
+        const float B = kD * s.albedo[c] / kPi + F * spec;
+        const float lcc = c == 0 ? lc.x : (c == 1 ? lc.y : lc.z);
+        gl[3 + c] += gT[c] * B * t.att * NdotL;
+        const float gB = gT[c] * lcc * t.att * NdotL;
+        g_att += gT[c] * B * lcc * NdotL;
+        g_NdotL += gT[c] * B * lcc * t.att;
+        g_spec += gB * F;
+        const float gF = gB * (spec - s.albedo[c] * km / kPi);
+        gp[c] += gB * kD / kPi + gF * (1.0f - f5) * s.metal;
+        g_f5 += gF * (1.0f - s.F0[c]);
+    }
+    const float g_HdotV = HdotVraw > 0.0f ? g_f5 * (-5.0f * om2 * om2) : 0.0f;
+    const float g_NDF = g_spec * G / D;
+    const float g_G = g_spec * NDF / D;
+    g_NdotL += -g_spec * spec / D * 4.0f * s.NdotV;
+    const float g_gl = g_G * s.ggxV;
+    const float g_ggxV = g_G * glv;
+    g_NdotL += g_gl * k / (gl_den * gl_den);
+    const float gv_den = s.NdotV * (1.0f - k) + k;
+    float g_k = g_gl * (-NdotL * (1.0f - NdotL) / (gl_den * gl_den)) +
+                g_ggxV * (-s.NdotV * (1.0f - s.NdotV) / (gv_den * gv_den));
+    const float g_inner = g_NDF * (-2.0f * NDF / inner);
+    const float g_a2 = g_NDF / (kPi * inner * inner) + g_inner * NdotH * NdotH;
+    const float g_NdotH = Nh > 0.0f ? g_inner * 2.0f * NdotH * (s.a2 - 1.0f) : 0.0f;
+    gp[3] += g_a2 * 2.0f * s.rough + g_k;
+    // geometry: h = V - L, NdotL = -N.L, NdotH = N.h/|h|, HdotV = V.h/|h|
+    float g_L[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float g_h = (g_NdotH * (s.N[i] - NdotH * H[i]) + g_HdotV * (s.V[i] - HdotV * H[i])) * ih;
+        g_L[i] = -(g_h + g_NdotL * s.N[i]);
+    }
+    // att(x), x = |d| / r (smooth window: att = w^2, w = 1 - x^4)
+    const float datt_dx = -8.0f * t.x * t.x * t.x * t.w;
+    const float g_dist = g_att * datt_dx / t.r;
+    const float gLL = dot3(g_L, t.L);
+    const float idist = 1.0f / t.dist;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float g_d = (g_L[i] - gLL * t.L[i]) * idist + g_dist * t.L[i];
+        gl[i] -= g_d; // d = p - light
+    }
+    gl[6] += g_att * datt_dx * (-t.x / t.r);
+}
+
+__global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* frames, uint32_t F, const float* grad_out,
+                                                             float* g_pos, float* g_light, float* g_ar)
+{
+    __shared__ float4 s_lpos[kStage];
+    __shared__ float4 s_lcol[kStage];
+    __shared__ float s_g[kStage * 7];
+    __shared__ uint32_t s_li[kStage];
+    __shared__ uint32_t s_min[4], s_max[4];
+    __shared__ uint32_t s_start, s_n, s_staged;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
+    const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
+    const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const FrameDev& fd = frames[f];
+    const uint32_t t = g - fd.tile_base;
+    const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
+    const uint32_t gx = ti * 32 + (sub & 1) * 16 + (tid & 15), gy = tj * 32 + (sub >> 1) * 16 + (tid >> 4);
+    const bool in = gx < fd.W && gy < fd.H;
+    const uint64_t p = in ? (uint64_t)gy * fd.W + gx : 0;
+    PixelIn px;
+    px.shaded = false;
+    uint2 nb = make_uint2(0u, 0u);
+    float4 ar = make_float4(0.f, 0.f, 0.f, 0.f);
+    float metal = 0.0f;
+    float4 go = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) {
+        const float depth = fd.depth[p];
+        nb = reinterpret_cast<const uint2*>(fd.normals)[p];
+        ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
+        metal = fd.metallic[p];
+        go = reinterpret_cast<const float4*>(grad_out)[p];
+        px = reconstruct(fd, gx, gy, depth);
+    }
+    const uint32_t cl = px.shaded ? fd.refs[p] : 0xFFFFFFFFu;
+    const bool has_list = px.shaded && fd.L > 0 && cl < fd.key_cap;
+    uint32_t cmin = has_list ? cl : 0xFFFFFFFFu, cmax = has_list ? cl : 0u;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        cmin = min(cmin, shfl_xor(cmin, m));
+        cmax = max(cmax, shfl_xor(cmax, m));
+    }
+    if (lane == 0) {
+        s_min[wave] = cmin;
+        s_max[wave] = cmax;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+        const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+        uint32_t start = 0, n = 0;
+        if (a != 0xFFFFFFFFu) {
+            start = min(fd.offsets[a], fd.index_cap);
+            const uint64_t e = (uint64_t)fd.offsets[b] + fd.counts[b];
+            const uint32_t end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+            n = end > start ? end - start : 0;
+        }
+        s_start = start;
+        s_n = n;
+        s_staged = n <= kStage ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t start = s_start, nstage = s_n;
+    const bool staged = s_staged != 0;
+    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
+    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
+    if (staged) {
+        for (uint32_t i = tid; i < nstage; i += 256) {
+            const uint32_t li = fd.indices[start + i];
+            s_li[i] = li;
+            const bool ok = li < fd.L;
+            s_lpos[i] = ok ? lpos[li] : make_float4(0.f, 0.f, 0.f, 0.f);
+            s_lcol[i] = ok ? lcol[li] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (uint32_t i = tid; i < nstage * 7; i += 256) s_g[i] = 0.0f;
+    }
+    __syncthreads();
+    if (px.shaded) {
+        Surface s;
+        {
+            const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
+            const float il = rsqrtf(dot3(V, V));
+            for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
+        }
+        s.N[0] = vren_half_to_float((uint16_t)(nb.x & 0xFFFFu));
+        s.N[1] = vren_half_to_float((uint16_t)(nb.x >> 16));
+        s.N[2] = vren_half_to_float((uint16_t)(nb.y & 0xFFFFu));
+        s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
+        s.albedo[0] = ar.x;
+        s.albedo[1] = ar.y;
+        s.albedo[2] = ar.z;
+        s.rough = ar.w;
+        s.metal = metal;
+        s.a2 = s.rough * s.rough;
+        for (int c = 0; c < 3; c++) s.F0[c] = 0.04f * (1.0f - s.metal) + s.albedo[c] * s.metal;
+        s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
+        s.ggxV = s.NdotV / (s.NdotV * (1.0f - s.rough) + s.rough);
+        uint32_t off = 0, end = 0;
+        if (has_list) {
+            off = fd.offsets[cl];
+            const uint64_t e = (uint64_t)off + fd.counts[cl];
+            end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+        }
+        // pass 1: the pixel's colour (the gamma derivative needs it)
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t e = off; e < end; e++) {
+            uint32_t li;
+            float4 lp, lc;
+            if (staged) {
+                lp = s_lpos[e - start];
+                lc = s_lcol[e - start];
+            } else {
+                li = fd.indices[e];
+                if (li >= fd.L) continue;
+                lp = lpos[li];
+                lc = lcol[li];
+            }
+            const LightTerm lt = light_term(px.P, lp, lc, fd.attenuation);
+            if (!lt.live) continue;
+            const float rad[3] = {lc.x * lt.att, lc.y * lt.att, lc.z * lt.att};
+            apply_light(s, lt.L, rad, acc);
+        }
+        const float gexp = 1.0f / 2.2f;
+        float gT[3];
+        const float goc[3] = {go.x, go.y, go.z};
+        float gp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < 3; c++) {
+            const float col = 0.03f * s.albedo[c] + acc[c];
+            gT[c] = goc[c] * gexp * powf(col, gexp - 1.0f);
+            gp[c] += 0.03f * gT[c];
+        }
+        // pass 2: adjoint of every contribution
+        for (uint32_t e = off; e < end; e++) {
+            uint32_t li = 0xFFFFFFFFu;
+            float4 lp, lc;
+            if (staged) {
+                lp = s_lpos[e - start];
+                lc = s_lcol[e - start];
+            } else {
+                li = fd.indices[e];
+                if (li >= fd.L) continue;
+                lp = lpos[li];
+                lc = lcol[li];
+            }
+            const LightTerm lt = light_term(px.P, lp, lc, fd.attenuation);
+            if (!lt.live) continue;
+            float gl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            light_adjoint(s, lt, lc, gT, gl, gp);
+            if (staged) {
+                float* slot = s_g + (e - start) * 7;
+#pragma unroll
+                for (int i = 0; i < 7; i++) atomicAdd(&slot[i], gl[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 3; i++) atomicAdd(&g_pos[4 * li + i], gl[i]);
+#pragma unroll
+                for (int i = 0; i < 4; i++) atomicAdd(&g_light[4 * li + i], gl[3 + i]);
+            }
+        }
+        float4* pa = reinterpret_cast<float4*>(g_ar) + p;
+        float4 cur = *pa;
+        cur.x += gp[0];
+        cur.y += gp[1];
+        cur.z += gp[2];
+        cur.w += gp[3];
+        *pa = cur;
+    }
+    __syncthreads();
+    if (staged) {
+        for (uint32_t i = tid; i < nstage; i += 256) {
+            const uint32_t li = s_li[i];
+            if (li >= fd.L) continue;
+            const float* slot = s_g + i * 7;
+            for (int c = 0; c < 3; c++)
+                if (slot[c] != 0.0f) atomicAdd(&g_pos[4 * li + c], slot[c]);
+            for (int c = 0; c < 4; c++)
+                if (slot[3 + c] != 0.0f) atomicAdd(&g_light[4 * li + c], slot[3 + c]);
+        }
+    }
+}
+
 } // namespace
 
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
@@ -314,4 +615,54 @@ extern "C" VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint3
     w.total_tiles = f.Sx * f.Sy;
     if (!st) st = launch_shade(ctx, s, slot, 1, w);
     return st;
+}
+
+extern "C" VREN_EXPORT int vren_shade_backward(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                               const vren_camera* camera, const vren_cluster_cfg* cfg,
+                                               const float* depth, const uint16_t* normals,
+                                               const float* albedo_roughness, const float* metallic,
+                                               const uint32_t* cluster_refs, uint32_t key_capacity,
+                                               const uint32_t* light_counts, const uint32_t* light_offsets,
+                                               const uint32_t* light_indices, uint32_t index_capacity,
+                                               const float* light_positions, const float* point_lights,
+                                               uint32_t light_count, const float* grad_output,
+                                               float* grad_light_positions, float* grad_point_lights,
+                                               float* grad_albedo_roughness)
+{
+    if (!ctx) return VREN_EINVAL;
+    int st = vren_validate_camera(ctx, camera);
+    if (!st) st = vren_validate_cfg(ctx, cfg);
+    if (st) return st;
+    if (width == 0 || height == 0 || width > 8192 || height > 8192)
+        return vren_fail(ctx, VREN_EINVAL, "shade_backward: screen must be in [1, 8192]^2");
+    if (!depth || !normals || !albedo_roughness || !metallic || !cluster_refs || !grad_output || !grad_albedo_roughness)
+        return vren_fail(ctx, VREN_EINVAL, "shade_backward: null G-buffer, reference or gradient buffer");
+    if (light_count > 0 && (!light_counts || !light_offsets || !light_indices || !light_positions || !point_lights ||
+                            !grad_light_positions || !grad_point_lights))
+        return vren_fail(ctx, VREN_EINVAL, "shade_backward: null light buffer");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "shade_backward: context has no device");
+    hipStream_t s = (hipStream_t)stream;
+    FrameDev f;
+    memset(&f, 0, sizeof(f));
+    vren_frame_setup(&f, width, height, camera, cfg);
+    f.depth = depth;
+    f.normals = normals;
+    f.albedo_rough = albedo_roughness;
+    f.metallic = metallic;
+    f.refs = const_cast<uint32_t*>(cluster_refs);
+    f.key_cap = key_capacity;
+    f.counts = const_cast<uint32_t*>(light_counts);
+    f.offsets = const_cast<uint32_t*>(light_offsets);
+    f.indices = const_cast<uint32_t*>(light_indices);
+    f.index_cap = index_capacity;
+    f.light_pos = light_positions;
+    f.lights = point_lights;
+    f.L = light_count;
+    FrameDev* slot = vren_next_frame_slot(ctx);
+    st = vren_upload_frames(ctx, s, slot, &f, 1);
+    if (st) return st;
+    shade_backward_kernel<<<f.Sx * f.Sy * 4, 256, 0, s>>>(slot, 1, grad_output, grad_light_positions, grad_point_lights,
+                                                          grad_albedo_roughness);
+    VREN_LAUNCH_CHECK(ctx);
+    return VREN_OK;
 }
