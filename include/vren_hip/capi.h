@@ -92,6 +92,8 @@ typedef struct vren_cluster_cfg {
 
 typedef struct vren_ctx vren_ctx;
 
+/* hip_device = -1 creates a host-only context: argument validation and the host helpers work and every call
+ * that would touch a device returns VREN_EINVAL after validating its arguments (for tests without a GPU) */
 VREN_EXPORT int vren_create(int hip_device, vren_ctx** out);
 VREN_EXPORT void vren_destroy(vren_ctx* ctx);
 VREN_EXPORT const char* vren_status_string(int status);

@@ -83,6 +83,7 @@ VREN_EXPORT int vren_autotune_radix_sort(vren_ctx* ctx, vren_stream_t stream, ui
                                          uint32_t key_bits, int* best_config, float* best_ms)
 {
     if (!ctx || n == 0 || n >= (1ull << 30) || key_bits == 0 || key_bits > 32) return VREN_EINVAL;
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "autotune: context has no device");
     hipStream_t s = (hipStream_t)stream;
     size_t scratch_bytes = vren_radix_sort_scratch_bytes(n, with_values);
     uint32_t *keys = nullptr, *vals = nullptr;

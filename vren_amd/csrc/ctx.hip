@@ -47,6 +47,13 @@ VREN_EXPORT int vren_create(int hip_device, vren_ctx** out)
 {
     if (!out) return VREN_EINVAL;
     *out = nullptr;
+    if (hip_device == -1) {
+        // host-only context: argument validation and the host helpers work, every device call returns EINVAL
+        vren_ctx* ctx = new vren_ctx();
+        ctx->device = -1;
+        *out = ctx;
+        return VREN_OK;
+    }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || hip_device < 0 || hip_device >= count) return VREN_EINVAL;
     vren_ctx* ctx = new vren_ctx();
@@ -70,6 +77,10 @@ VREN_EXPORT int vren_create(int hip_device, vren_ctx** out)
 VREN_EXPORT void vren_destroy(vren_ctx* ctx)
 {
     if (!ctx) return;
+    if (ctx->device < 0) {
+        delete ctx;
+        return;
+    }
     hipSetDevice(ctx->device);
     hipDeviceSynchronize();
     if (ctx->d_flags) hipFree(ctx->d_flags);
@@ -95,6 +106,7 @@ VREN_EXPORT const char* vren_last_error(vren_ctx* ctx) { return ctx ? ctx->last_
 VREN_EXPORT int vren_sync_status(vren_ctx* ctx, vren_stream_t stream)
 {
     if (!ctx) return VREN_EINVAL;
+    if (ctx->device < 0) return VREN_OK;
     hipStream_t s = (hipStream_t)stream;
     uint32_t flags = 0;
     VREN_HIP(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));

@@ -313,6 +313,10 @@ VREN_EXPORT int vren_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint32_t* k
     if (!keys || n == 0 || !scratch) return vren_fail(ctx, VREN_EINVAL, "radix_sort: null buffer or n == 0");
     if (n >= (1ull << 30)) return vren_fail(ctx, VREN_EINVAL, "radix_sort: n must be below 2^30");
     if (begin_bit > end_bit || end_bit > 32) return vren_fail(ctx, VREN_EINVAL, "radix_sort: bad bit range");
+    if (scratch_bytes < vren_radix_sort_scratch_bytes(n, vals != nullptr))
+        return vren_fail(ctx, VREN_EINVAL, "radix_sort: scratch of %zu bytes, %zu required", scratch_bytes,
+                         vren_radix_sort_scratch_bytes(n, vals != nullptr));
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "radix_sort: context has no device");
     int cfg = vren_radix_pick_config(ctx, (hipStream_t)stream, n, vals != nullptr, end_bit - begin_bit);
     return vren_radix_sort_with_config(ctx, (hipStream_t)stream, cfg, keys, vals, n, begin_bit, end_bit, scratch,
                                        scratch_bytes);

@@ -240,6 +240,7 @@ VREN_EXPORT int vren_reduce_u32(vren_ctx* ctx, vren_stream_t stream, const uint3
         return vren_fail(ctx, VREN_EINVAL, "reduce_u32: null buffer, n == 0, n > 2^31 or blocks not in [1, 65535]");
     if (in == out_tree && blocks > 1 && next_pow2_u32(n) != n)
         return vren_fail(ctx, VREN_EINVAL, "reduce_u32: in-place multi-block reduce needs a power-of-2 length");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "reduce_u32: context has no device");
     hipStream_t s = (hipStream_t)stream;
     switch (op) {
     case VREN_REDUCE_ADD: return reduce_tree<uint32_t, OpAddU, 16>(ctx, s, in, n, out_tree, blocks);
@@ -257,6 +258,7 @@ VREN_EXPORT int vren_reduce_vec4(vren_ctx* ctx, vren_stream_t stream, const floa
         return vren_fail(ctx, VREN_EINVAL, "reduce_vec4: null buffer, n == 0, n > 2^31 or blocks not in [1, 65535]");
     if (in == out_tree && blocks > 1 && next_pow2_u32(n) != n)
         return vren_fail(ctx, VREN_EINVAL, "reduce_vec4: in-place multi-block reduce needs a power-of-2 length");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "reduce_vec4: context has no device");
     hipStream_t s = (hipStream_t)stream;
     const float4* vin = reinterpret_cast<const float4*>(in);
     float4* vout = reinterpret_cast<float4*>(out_tree);
@@ -273,6 +275,7 @@ VREN_EXPORT int vren_reduce_total_u32(vren_ctx* ctx, vren_stream_t stream, const
 {
     if (!ctx) return VREN_EINVAL;
     if (!in || !out || n == 0) return vren_fail(ctx, VREN_EINVAL, "reduce_total_u32: null buffer or n == 0");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "reduce_total_u32: context has no device");
     hipStream_t s = (hipStream_t)stream;
     switch (op) {
     case VREN_REDUCE_ADD: return reduce_total<OpAddU>(ctx, s, in, n, out);
