@@ -19,6 +19,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,6 +36,25 @@ SEED_FRAMES = 42 + 1000 * 4
 BYTES_PER_PIXEL = {"clusters": 16.0, "shade": 52.0}
 BYTES_PER_LIGHT = {"lights": 40.0}
 SORT_BYTES_PER_PAIR = 68.0
+
+
+def profiled_traffic():
+    """HBM bytes per launch of each kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/hbm_traffic_pmc.md; counters scaled by a 1 GiB copy-kernel calibration)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_pmc.md")) as f:
+            text = f.read()
+    except OSError:
+        return {}
+    m = re.search(r"traffic-bytes-per-launch: (.*)", text)
+    if not m:
+        return {}
+    out = {}
+    for item in m.group(1).split(";"):
+        if "=" in item:
+            k, v = item.strip().rsplit("=", 1)
+            out[k] = float(v)
+    return out
 
 
 def make_camera(vren_amd):
@@ -118,11 +138,16 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     ms = sorted(times)[len(times) // 2]
     achieved = SORT_BYTES_PER_PAIR * n / (ms * 1e-3) / 1e9
     configs = ctx.radix_configs()
+    cfg = configs[cfg_id] if cfg_id is not None else (8, 15, 8)  # the library's untuned default
+    traffic = profiled_traffic()
+    one = traffic.get("rs_onesweep_kernel<%d, %d, %d, true>" % tuple(cfg))
+    hist = traffic.get("rs_hist_kernel<%d>" % cfg[0])
+    sort_traffic = hist + one * ((32 + cfg[0] - 1) // cfg[0]) if (one and hist and log2n == 26) else None
     return {
         "n": n, "pairs": True, "ms": round(ms, 4), "gpairs_per_s": round(n / (ms * 1e-3) / 1e9, 3),
-        "config": configs[cfg_id] if cfg_id is not None else "default",
+        "config": list(cfg),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": sort_traffic,
                      "algorithmic_bytes_per_pair": SORT_BYTES_PER_PAIR},
     }
 
@@ -190,6 +215,8 @@ def main():
     ap.add_argument("--no-sort", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB with one plain device copy kernel (scales HBM counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -201,6 +228,15 @@ def main():
     import vren_amd
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     ctx = vren_amd.Context(0)
+    if args.copy_calibration_mib:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] copy calibration: %d bytes read, %d bytes written" % (nb, nb), file=sys.stderr, flush=True)
+        del src, dst
     print("[bench] building %d frames of %dx%d with %d lights" % (args.frames, W, H, L_LIGHTS), file=sys.stderr,
           flush=True)
     batch, keep = build_batch(ctx, vren_amd, args.frames)
@@ -242,9 +278,10 @@ def main():
     else:
         alg_bytes = None
     achieved = alg_bytes / (kernel_ms[dom] * 1e-3) / 1e9 if alg_bytes else None
+    traffic = profiled_traffic()
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": None, "launch_ms": kernel_ms[dom],
+                "traffic": traffic.get(dom + "_kernel") if args.frames == 64 else None, "launch_ms": kernel_ms[dom],
                 "algorithmic_bytes_per_launch": alg_bytes}
 
     sort = None

@@ -53,15 +53,30 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict
     const bool aligned = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
     const uint64_t n4 = aligned ? n / 4 : 0;
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    // eight 128-bit loads in flight per thread before the LDS counting
+    constexpr int U = 8;
+    for (; i + (U - 1) * stride < n4; i += U * stride) {
+        uint4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) q[u] = ntload4(k4 + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            count(q[u].x);
+            count(q[u].y);
+            count(q[u].z);
+            count(q[u].w);
+        }
+    }
+    for (; i < n4; i += stride) {
         uint4 q = k4[i];
         count(q.x);
         count(q.y);
         count(q.z);
         count(q.w);
     }
-    for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-        count(keys[i]);
+    for (uint64_t j = n4 * 4 + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += stride) count(keys[j]);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) {
         uint32_t c = s_h[i];
@@ -236,7 +251,7 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
 
     rs_init_kernel<<<1, 256, 0, s>>>(hist, passes * RADIX, counters, passes);
     VREN_LAUNCH_CHECK(ctx);
-    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 4, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
+    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
     rs_hist_kernel<BITS><<<hb, 256, 0, s>>>(keys, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
     VREN_LAUNCH_CHECK(ctx);
     uint32_t *ksrc = keys, *vsrc = vals, *kdst = tk, *vdst = tv;
