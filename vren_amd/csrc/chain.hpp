@@ -61,3 +61,21 @@ __device__ __forceinline__ uint32_t find_frame(const FrameDev* frames, uint32_t 
     }
     return lo;
 }
+
+// device: the same search done by one whole wave at once (every lane calls it with the same g): lane i reads frame
+// c + i's base, and a ballot gives the last frame whose base is <= g. One round of loads instead of log2(F)
+// dependent ones.
+template <typename Get>
+__device__ __forceinline__ uint32_t find_frame_wave(const FrameDev* frames, uint32_t F, uint32_t g, Get base)
+{
+    const uint32_t lane = lane_id();
+    uint32_t f = 0;
+    for (uint32_t c = 0; c < F; c += VREN_WAVE) {
+        const uint32_t i = c + lane;
+        const uint64_t m = ballot(i < F && base(frames[i]) <= g);
+        if (m == 0ull) break;
+        f = c + 63u - (uint32_t)__builtin_clzll(m);
+        if (m != ~0ull) break;
+    }
+    return f;
+}

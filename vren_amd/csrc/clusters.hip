@@ -57,11 +57,9 @@ __device__ __forceinline__ uint32_t face_of_normal(const float n[3])
     return face_idx;
 }
 
-__device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t gx, uint32_t gy)
+__device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t gx, uint32_t gy,
+                                              float depth, uint2 nb)
 {
-    const uint64_t p = (uint64_t)(gy % fd.H) * fd.W + (gx % fd.W);
-    const float depth = fd.depth[p];
-    const uint2 nb = reinterpret_cast<const uint2*>(fd.normals)[p];
     // find_unique_clusters.comp:48-65
     float v[4] = {(float)gx / (float)fd.W, (float)gy / (float)fd.H, depth, 1.0f};
     v[1] = 1.0f - v[1];
@@ -116,15 +114,40 @@ __global__ __launch_bounds__(256) void clusters_kernel(const FrameDev* frames, u
     if (tid == 0) s_t = atomic_add_agent(ticket, 1u);
     __syncthreads();
     const uint32_t g = s_t;
-    const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
     const uint32_t gy = tj * 32 + (tid >> 3);
     const uint32_t gx0 = ti * 32 + (tid & 7) * 4;
+    // the thread's 4 pixels: one 16-byte depth load and two 16-byte normal loads when they lie inside the frame
+    // on one aligned row; otherwise per pixel, wrapping (x mod W, y mod H) like the reference's REPEAT sampler
+    float dep[4];
+    uint2 nrm[4];
+    const uint64_t p0 = (uint64_t)gy * fd.W + gx0;
+    if (gy < fd.H && gx0 + 3 < fd.W && (p0 & 3) == 0 && ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0) {
+        const float4 d4 = *reinterpret_cast<const float4*>(fd.depth + p0);
+        const uint4 n01 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0);
+        const uint4 n23 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0 + 8);
+        dep[0] = d4.x;
+        dep[1] = d4.y;
+        dep[2] = d4.z;
+        dep[3] = d4.w;
+        nrm[0] = make_uint2(n01.x, n01.y);
+        nrm[1] = make_uint2(n01.z, n01.w);
+        nrm[2] = make_uint2(n23.x, n23.y);
+        nrm[3] = make_uint2(n23.z, n23.w);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t p = (uint64_t)(gy % fd.H) * fd.W + ((gx0 + q) % fd.W);
+            dep[q] = fd.depth[p];
+            nrm[q] = reinterpret_cast<const uint2*>(fd.normals)[p];
+        }
+    }
     uint32_t key[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) key[q] = pixel_key(fd, ti, tj, gx0 + q, gy);
+    for (int q = 0; q < 4; q++) key[q] = pixel_key(fd, ti, tj, gx0 + q, gy, dep[q], nrm[q]);
 
     // unique keys of the wave's 256 pixels, extracted in ascending order by repeated wave minima
     uint32_t rem = 0xFu;

@@ -18,6 +18,11 @@
 namespace {
 
 constexpr float kPi = 3.14f;
+constexpr float kInvPi = 1.0f / 3.14f;
+
+// reciprocal on the transcendental unit (v_rcp_f32, 1 ulp) in place of an IEEE division (~10 instructions); every
+// quotient of the forward shade goes through it, which keeps it within ~1e-6 of the FP64 evaluation
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 struct PixelIn {
     double P[3];
@@ -73,23 +78,23 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
     float inner = 1.0f;
     if (Nh > 0.0f && hh > 0.0f) {
         const float c[3] = {s.N[1] * h[2] - s.N[2] * h[1], s.N[2] * h[0] - s.N[0] * h[2], s.N[0] * h[1] - s.N[1] * h[0]};
-        const float inv_hh = 1.0f / hh;
+        const float inv_hh = frcp(hh);
         inner = s.nn + dot3(c, c) * inv_hh + s.a2 * (Nh * Nh) * inv_hh;
     }
-    const float NDF = s.a2 / (kPi * inner * inner);
+    const float NDF = s.a2 * frcp(kPi * inner * inner);
     const float k = s.rough;
-    const float G = s.ggxV * (NdotL / (NdotL * (1.0f - k) + k));
-    const float HdotV = hh > 0.0f ? fmaxf(dot3(h, s.V) * rsqrtf(hh), 0.0f) : 0.0f;
+    const float G = s.ggxV * (NdotL * frcp(NdotL * (1.0f - k) + k));
+    const float HdotV = hh > 0.0f ? fmaxf(dot3(h, s.V) * __builtin_amdgcn_rsqf(hh), 0.0f) : 0.0f;
     float om = 1.0f - HdotV;
     float om2 = om * om;
     const float f5 = om2 * om2 * om;
-    const float spec = (NDF * G) / (4.0f * s.NdotV * NdotL + 0.0001f);
+    const float spec = (NDF * G) * frcp(4.0f * s.NdotV * NdotL + 0.0001f);
     const float km = 1.0f - s.metal;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const float F = s.F0[c] + (1.0f - s.F0[c]) * f5;
         const float kD = (1.0f - F) * km;
-        acc[c] += fmaxf((kD * s.albedo[c] / kPi + F * spec) * rad[c] * NdotL, 0.0f);
+        acc[c] += fmaxf((kD * s.albedo[c] * kInvPi + F * spec) * rad[c] * NdotL, 0.0f);
     }
 }
 
@@ -107,7 +112,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     __shared__ uint32_t s_start, s_n, s_staged;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
-    const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
@@ -119,40 +124,45 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     uint2 nb = make_uint2(0u, 0u);
     float4 ar = make_float4(0.f, 0.f, 0.f, 0.f);
     float metal = 0.0f;
+    uint32_t cl_raw = 0xFFFFFFFFu;
     if (in) {
+        // every per-pixel load is issued before any of them is used
         const float depth = fd.depth[p];
         nb = reinterpret_cast<const uint2*>(fd.normals)[p];
         ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
         metal = fd.metallic[p];
+        cl_raw = fd.refs[p];
         px = reconstruct(fd, gx, gy, depth);
     }
-    const uint32_t cl = px.shaded ? fd.refs[p] : 0xFFFFFFFFu;
-    const bool has_list = px.shaded && fd.L > 0 && cl < fd.key_cap;
-    // range of referenced clusters
-    uint32_t cmin = has_list ? cl : 0xFFFFFFFFu, cmax = has_list ? cl : 0u;
+    const bool has_list = px.shaded && fd.L > 0 && cl_raw < fd.key_cap;
+    // this pixel's light list, then the block's range of list entries (the clusters of a tile are numbered
+    // consecutively and their lists are consecutive in the index buffer)
+    uint32_t my_off = 0, my_end = 0;
+    if (has_list) {
+        my_off = min(fd.offsets[cl_raw], fd.index_cap);
+        const uint64_t e = (uint64_t)fd.offsets[cl_raw] + fd.counts[cl_raw];
+        my_end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+    }
+    uint32_t rmin = has_list ? my_off : 0xFFFFFFFFu, rmax = has_list ? my_end : 0u;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
-        cmin = min(cmin, shfl_xor(cmin, m));
-        cmax = max(cmax, shfl_xor(cmax, m));
+        rmin = min(rmin, shfl_xor(rmin, m));
+        rmax = max(rmax, shfl_xor(rmax, m));
     }
     if (lane == 0) {
-        s_min[wave] = cmin;
-        s_max[wave] = cmax;
+        s_min[wave] = rmin;
+        s_max[wave] = rmax;
     }
     __syncthreads();
-    if (tid == 0) {
+    {
         const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
         const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
-        uint32_t start = 0, n = 0;
-        if (a != 0xFFFFFFFFu) {
-            start = min(fd.offsets[a], fd.index_cap);
-            const uint64_t e = (uint64_t)fd.offsets[b] + fd.counts[b];
-            const uint32_t end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
-            n = end > start ? end - start : 0;
+        const uint32_t n = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
+        if (tid == 0) {
+            s_start = a == 0xFFFFFFFFu ? 0u : a;
+            s_n = n;
+            s_staged = n <= kStage ? 1u : 0u;
         }
-        s_start = start;
-        s_n = n;
-        s_staged = n <= kStage ? 1u : 0u;
     }
     __syncthreads();
     const uint32_t start = s_start, nstage = s_n;
@@ -192,14 +202,11 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     s.a2 = s.rough * s.rough;
     for (int c = 0; c < 3; c++) s.F0[c] = 0.04f * (1.0f - s.metal) + s.albedo[c] * s.metal;
     s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
-    s.ggxV = s.NdotV / (s.NdotV * (1.0f - s.rough) + s.rough);
+    s.ggxV = s.NdotV * frcp(s.NdotV * (1.0f - s.rough) + s.rough);
 
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (has_list) {
-        const uint32_t cnt = fd.counts[cl];
-        const uint32_t off = fd.offsets[cl];
-        const uint32_t end = (uint64_t)off + cnt > fd.index_cap ? fd.index_cap : off + cnt;
-        for (uint32_t e = off; e < end; e++) {
+        for (uint32_t e = my_off; e < my_end; e++) {
             float4 lp, lc;
             if (staged) {
                 lp = s_lpos[e - start];
@@ -215,12 +222,12 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
             const double d[3] = {px.P[0] - (double)lp.x, px.P[1] - (double)lp.y, px.P[2] - (double)lp.z};
             const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
             const float r = lc.w;
-            if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < (double)r * (double)r)) continue;
-            // sqrt in double from a float estimate and one Newton step (relative error ~1e-14)
-            const float s0 = sqrtf((float)dist2);
-            const double dist = 0.5 * ((double)s0 + dist2 / (double)s0);
-            const float inv_r = 1.0f / r;
-            const float omx = (float)((double)r - dist) * inv_r; // 1 - x, accurate even when tiny
+            const double rr = (double)r * (double)r;
+            if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < rr)) continue;
+            const float s0 = __builtin_amdgcn_sqrtf((float)dist2);
+            // 1 - x = (r - |d|) / r = (r^2 - |d|^2) / ((r + |d|) r): the difference that cancels near the radius is
+            // taken in double, the rest needs only float accuracy
+            const float omx = (float)(rr - dist2) * frcp((r + s0) * r);
             const float x = 1.0f - omx;
             float att;
             if (fd.attenuation == VREN_ATTEN_SMOOTH) {
@@ -229,10 +236,10 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
                 att = w * w;
             } else {
                 // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5)
-                att = omx * (1.0f + x + x * x) / (1.0f + x * sqrtf(x));
+                att = omx * (1.0f + x + x * x) * frcp(1.0f + x * __builtin_amdgcn_sqrtf(x));
             }
             if (!(att > 0.0f)) continue;
-            const float inv = 1.0f / (float)dist;
+            const float inv = frcp(s0);
             const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
             const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
             apply_light(s, L, rad, acc);
@@ -245,11 +252,13 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
         const float rad[3] = {col.x, col.y, col.z};
         apply_light(s, L, rad, acc);
     }
+    // gamma (shade.comp:135-137): c^(1/2.2) = 2^(log2(c) / 2.2) on the transcendental unit (v_log_f32 /
+    // v_exp_f32, ~1 ulp each: ~1e-7 relative on the result); c >= 0, and c = 0 gives 2^-inf = 0
     const float gexp = 1.0f / 2.2f;
     float4 o;
-    o.x = powf(0.03f * s.albedo[0] + acc[0], gexp);
-    o.y = powf(0.03f * s.albedo[1] + acc[1], gexp);
-    o.z = powf(0.03f * s.albedo[2] + acc[2], gexp);
+    o.x = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[0] + acc[0]) * gexp);
+    o.y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[1] + acc[1]) * gexp);
+    o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[2] + acc[2]) * gexp);
     o.w = 1.0f;
     reinterpret_cast<float4*>(fd.out)[p] = o;
 }
@@ -387,7 +396,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* fra
     __shared__ uint32_t s_start, s_n, s_staged;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
-    const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
