@@ -17,6 +17,7 @@ struct FrameDev {
     // camera constants
     float view[16];         // camera::get_view()
     float inv_proj[16];     // inverse(camera::get_projection()), rounded from double
+    uint32_t proj_zw_xy_zero; // inv_proj's z and w rows have zero x and y coefficients (every perspective camera)
     double inv_proj_d[16];  // shade reconstruction (double)
     double inv_view_d[16];
     double eye[3];

@@ -1,14 +1,13 @@
 // clusters.hip -- find_unique_cluster_list (clustered_shading.cpp:363-448, find_unique_clusters.comp:46-121).
 //
-// One 256-thread workgroup per 32x32 tile (flattened over every frame of a batch); each thread derives the
-// 32-bit cluster keys of 4 adjacent pixels (depth slice from the D32 depth, cube-face normal code from the
-// RGBA16F normal). The reference sorts the tile's 1024 keys with a 55-step LDS bitonic network and 20 barriers;
-// here each wave extracts its unique keys in ascending order by repeated wave64 minima (one iteration per
-// distinct key -- a tile of a real scene holds a handful), wave 0 merges the four sorted wave lists the same
-// way, and the tile's unique count is chained to the previous tiles of the same frame by a decoupled look-back. This replaces the reference's atomicAdd allocation (whose
-// order across tiles is nondeterministic, find_unique_clusters.comp:100-103) by the canonical row-major tile
-// order, and needs no separate scan dispatch. Each pixel finds its key among the tile's unique keys by binary
-// search and stores the frame-wide cluster index.
+// One wave per 32x32 tile (tiles flattened over every frame of a batch); each lane derives the 32-bit cluster keys
+// of 16 pixels (depth slice from the D32 depth, cube-face normal code from the RGBA16F normal). The reference
+// sorts the tile's 1024 keys with a 55-step LDS bitonic network and 20 barriers; here the wave extracts the tile's
+// unique keys in ascending order by repeated wave64 minima (one round per distinct key -- a tile of a real scene
+// holds a handful), and the tile's unique count is chained to the previous tiles of the same frame by a decoupled
+// look-back. This replaces the reference's atomicAdd allocation (whose order across tiles is nondeterministic,
+// find_unique_clusters.comp:100-103) by the canonical row-major tile order, and needs no separate scan dispatch.
+// Each pixel finds its key among the tile's unique keys by binary search and stores the frame-wide cluster index.
 //
 // Defined behaviour where the reference has none: pixels of partial tiles sample (x mod W, y mod H) like its
 // REPEAT sampler (gbuffer.cpp:28-34) and do not store a reference; a view depth below near gives slice 0; keys
@@ -18,20 +17,37 @@
 
 namespace {
 
+// The dominant axis of a normal: the reference picks the smallest t_i = sign(n_i) / n_i (first index on ties;
+// a zero component gives NaN and is never picked), clustered_shading.glsl:7-43. The components are f16 values
+// converted exactly, so distinct magnitudes differ by at least 2^-11 relative and their float reciprocals keep
+// the reverse order: the smallest t_i is the largest |n_i| (first index on ties), found without dividing.
+// Returns the axis, or 3 when no component qualifies (the reference then keeps t = 1e35 and axis 0).
+__device__ __forceinline__ uint32_t dominant_axis(const float n[3])
+{
+    uint32_t axis = 3;
+    float best = 0.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < 3; i++) {
+        const float a = fabsf(n[i]);
+        if (a > best) {
+            best = a;
+            axis = i;
+        }
+    }
+    return axis;
+}
+
 __device__ __forceinline__ uint32_t discretize_normal(const float n[3])
 {
     // clustered_shading.glsl:7-43
     if (n[0] == 0.0f && n[1] == 0.0f && n[2] == 0.0f) return 0xFFFFFFFFu;
+    const uint32_t ax = dominant_axis(n);
     float min_t = 1e35f;
     uint32_t axis = 0, face_idx = 0;
-    for (uint32_t i = 0; i < 3; i++) {
-        float sgn = n[i] > 0.0f ? 1.0f : (n[i] < 0.0f ? -1.0f : 0.0f);
-        float t = sgn / n[i];
-        if (t < min_t) {
-            min_t = t;
-            axis = i;
-            face_idx = (n[i] > 0.0f ? 1u : 0u) * 3u + i;
-        }
+    if (ax < 3) {
+        axis = ax;
+        min_t = (n[ax] > 0.0f ? 1.0f : -1.0f) / n[ax];
+        face_idx = (n[ax] > 0.0f ? 1u : 0u) * 3u + ax;
     }
     float p[3] = {n[0] * min_t, n[1] * min_t, n[2] * min_t};
     float u = p[(axis + 1) % 3];
@@ -44,32 +60,42 @@ __device__ __forceinline__ uint32_t discretize_normal(const float n[3])
 __device__ __forceinline__ uint32_t face_of_normal(const float n[3])
 {
     if (n[0] == 0.0f && n[1] == 0.0f && n[2] == 0.0f) return 7u;
-    float min_t = 1e35f;
-    uint32_t face_idx = 0;
-    for (uint32_t i = 0; i < 3; i++) {
-        float sgn = n[i] > 0.0f ? 1.0f : (n[i] < 0.0f ? -1.0f : 0.0f);
-        float t = sgn / n[i];
-        if (t < min_t) {
-            min_t = t;
-            face_idx = (n[i] > 0.0f ? 1u : 0u) * 3u + i;
-        }
-    }
-    return face_idx;
+    const uint32_t ax = dominant_axis(n);
+    return ax < 3 ? (n[ax] > 0.0f ? 1u : 0u) * 3u + ax : 0u;
+}
+
+// f16 -> f32 on the conversion unit (exact); Inf/NaN halves keep the shared software path, whose NaN payload
+// handling is the oracle's
+__device__ __forceinline__ float h2f(uint16_t h)
+{
+    if ((h & 0x7C00u) == 0x7C00u) return vren_half_to_float(h);
+    _Float16 x;
+    __builtin_memcpy(&x, &h, 2);
+    return (float)x;
 }
 
 __device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t gx, uint32_t gy,
                                               float depth, uint2 nb)
 {
     // find_unique_clusters.comp:48-65
-    float v[4] = {(float)gx / (float)fd.W, (float)gy / (float)fd.H, depth, 1.0f};
-    v[1] = 1.0f - v[1];
-    v[0] = v[0] * 2.0f - 1.0f;
-    v[1] = v[1] * 2.0f - 1.0f;
-    float r[4];
-    vren_mat4_mul_vec4(fd.inv_proj, v, r);
-    const float z = r[2] / r[3];
-    const float n[3] = {vren_half_to_float((uint16_t)(nb.x & 0xFFFFu)), vren_half_to_float((uint16_t)(nb.x >> 16)),
-                        vren_half_to_float((uint16_t)(nb.y & 0xFFFFu))};
+    float r2, r3;
+    if (fd.proj_zw_xy_zero) {
+        // the z and w rows of inverse(P) have zero x and y coefficients: ((0 x + 0 y) + m2 depth) + m3 reduces to
+        // m2 depth + m3 exactly (a zero result may only change sign, and z = 0 gives the same slice either way)
+        r2 = fd.inv_proj[2 * 4 + 2] * depth + fd.inv_proj[3 * 4 + 2];
+        r3 = fd.inv_proj[2 * 4 + 3] * depth + fd.inv_proj[3 * 4 + 3];
+    } else {
+        float v[4] = {(float)gx / (float)fd.W, (float)gy / (float)fd.H, depth, 1.0f};
+        v[1] = 1.0f - v[1];
+        v[0] = v[0] * 2.0f - 1.0f;
+        v[1] = v[1] * 2.0f - 1.0f;
+        float r[4];
+        vren_mat4_mul_vec4(fd.inv_proj, v, r);
+        r2 = r[2];
+        r3 = r[3];
+    }
+    const float z = r2 / r3;
+    const float n[3] = {h2f((uint16_t)(nb.x & 0xFFFFu)), h2f((uint16_t)(nb.x >> 16)), h2f((uint16_t)(nb.y & 0xFFFFu))};
     if (fd.key_mode == VREN_KEYS_SLICED) {
         float s = vren_floorf(vren_logf(z / fd.near_plane) / fd.log_fn * (float)fd.depth_slices);
         uint32_t k = vren_f2u_sat(s);
@@ -98,34 +124,16 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 }
 
 constexpr uint64_t kNone = 1ull << 32; // larger than every 32-bit key
+constexpr uint32_t kClusterWaves = 8;   // waves per workgroup, one 32x32 tile per wave
 
-// One 256-thread workgroup per 32x32 tile; thread t owns the 4 horizontally adjacent pixels
-// (4 * (t % 8) + q, t / 8), q = 0..3, so depth and normal loads are 16- and 32-byte vectors.
-__global__ __launch_bounds__(256) void clusters_kernel(const FrameDev* frames, uint32_t F, uint64_t* tile_status,
-                                                       uint32_t* ticket)
+// the 4 horizontally adjacent pixels (x0 .. x0+3, y) of a tile row: one 16-byte depth load and two 16-byte normal
+// loads when they lie inside the frame on an aligned row, otherwise per pixel, wrapping (x mod W, y mod H) like
+// the reference's REPEAT sampler
+__device__ __forceinline__ void load_quad(const FrameDev& fd, uint32_t x0, uint32_t y, float dep[4], uint2 nrm[4])
 {
-    __shared__ uint32_t s_wlist[4][256];
-    __shared__ uint32_t s_wn[4];
-    __shared__ uint32_t s_unique[1024];
-    __shared__ uint32_t s_t, s_count, s_prefix;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = lane_id();
-    const uint32_t wave = tid / VREN_WAVE;
-    if (tid == 0) s_t = atomic_add_agent(ticket, 1u);
-    __syncthreads();
-    const uint32_t g = s_t;
-    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
-    const FrameDev& fd = frames[f];
-    const uint32_t t = g - fd.tile_base;
-    const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-    const uint32_t gy = tj * 32 + (tid >> 3);
-    const uint32_t gx0 = ti * 32 + (tid & 7) * 4;
-    // the thread's 4 pixels: one 16-byte depth load and two 16-byte normal loads when they lie inside the frame
-    // on one aligned row; otherwise per pixel, wrapping (x mod W, y mod H) like the reference's REPEAT sampler
-    float dep[4];
-    uint2 nrm[4];
-    const uint64_t p0 = (uint64_t)gy * fd.W + gx0;
-    if (gy < fd.H && gx0 + 3 < fd.W && (p0 & 3) == 0 && ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0) {
+    const uint64_t p0 = (uint64_t)y * fd.W + x0;
+    if (y < fd.H && x0 + 3 < fd.W && (p0 & 3) == 0 &&
+        ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0) {
         const float4 d4 = *reinterpret_cast<const float4*>(fd.depth + p0);
         const uint4 n01 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0);
         const uint4 n23 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0 + 8);
@@ -140,101 +148,100 @@ __global__ __launch_bounds__(256) void clusters_kernel(const FrameDev* frames, u
     } else {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const uint64_t p = (uint64_t)(gy % fd.H) * fd.W + ((gx0 + q) % fd.W);
+            const uint64_t p = (uint64_t)(y % fd.H) * fd.W + ((x0 + q) % fd.W);
             dep[q] = fd.depth[p];
             nrm[q] = reinterpret_cast<const uint2*>(fd.normals)[p];
         }
     }
-    uint32_t key[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) key[q] = pixel_key(fd, ti, tj, gx0 + q, gy, dep[q], nrm[q]);
+}
 
-    // unique keys of the wave's 256 pixels, extracted in ascending order by repeated wave minima
-    uint32_t rem = 0xFu;
-    uint32_t n = 0;
-    while (true) {
-        uint64_t mine = kNone;
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            if ((rem >> q) & 1u) mine = (uint64_t)key[q] < mine ? (uint64_t)key[q] : mine;
-        const uint64_t m = wave_min_u64(mine);
-        if (m == kNone) break;
-        if (lane == 0) s_wlist[wave][n] = (uint32_t)m;
-        n++;
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (key[q] == (uint32_t)m) rem &= ~(1u << q);
-    }
-    if (lane == 0) s_wn[wave] = n;
+// Tiles are flattened over every frame of the table (tile_base). A workgroup takes ONE ticket for kClusterWaves
+// consecutive tiles, one per wave -- a ticket on a single counter costs ~11 ns of serialised atomic throughput, so
+// one per tile would cap the kernel at ~90 M tiles/s. A wave waits (look-back) only on lower tiles: lower waves of
+// its workgroup, which run beside it, and tiles of lower tickets, whose workgroups are already running, so every
+// wait ends. (Giving a wave several tiles one after another would make the next workgroup's first tile wait for
+// this workgroup's last step, and serialise the grid.) Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8),
+// q, j = 0..3, of its tile.
+__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* frames, uint32_t F,
+                                                                       uint32_t total_tiles, uint64_t* tile_status,
+                                                                       uint32_t* ticket)
+{
+    __shared__ uint32_t s_unique[kClusterWaves][1024];
+    __shared__ uint32_t s_t;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / VREN_WAVE;
+    if (threadIdx.x == 0) s_t = atomic_add_agent(ticket, 1u);
     __syncthreads();
-    // merge the four sorted wave lists into the tile's sorted unique list (wave 0)
-    if (wave == 0) {
-        const uint32_t n0 = s_wn[0], n1 = s_wn[1], n2 = s_wn[2], n3 = s_wn[3];
-        const uint32_t total = n0 + n1 + n2 + n3;
-        uint32_t ent[16];
-        uint32_t emask = 0;
+    uint32_t* uq = s_unique[wave];
+    {
+        const uint32_t g = s_t * kClusterWaves + wave;
+        if (g >= total_tiles) return;
+        const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+        const FrameDev& fd = frames[f];
+        const uint32_t t = g - fd.tile_base;
+        const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
+        const uint32_t x0 = ti * 32 + (lane & 7) * 4;
+        uint32_t key[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t e = lane + 64u * i;
-            ent[i] = 0u;
-            if (e < total) {
-                uint32_t w = 0, idx = e;
-                if (idx >= n0) { idx -= n0; w = 1; if (idx >= n1) { idx -= n1; w = 2; if (idx >= n2) { idx -= n2; w = 3; } } }
-                ent[i] = s_wlist[w][idx];
-                emask |= 1u << i;
-            }
+        for (int j = 0; j < 4; j++) {
+            const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
+            float dep[4];
+            uint2 nrm[4];
+            load_quad(fd, x0, y, dep, nrm);
+#pragma unroll
+            for (int q = 0; q < 4; q++) key[4 * j + q] = pixel_key(fd, ti, tj, x0 + q, y, dep[q], nrm[q]);
         }
+        // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key)
+        uint32_t rem = 0xFFFFu;
         uint32_t u = 0;
         while (true) {
             uint64_t mine = kNone;
 #pragma unroll
-            for (int i = 0; i < 16; i++)
-                if ((emask >> i) & 1u) mine = (uint64_t)ent[i] < mine ? (uint64_t)ent[i] : mine;
+            for (int q = 0; q < 16; q++)
+                if ((rem >> q) & 1u) mine = (uint64_t)key[q] < mine ? (uint64_t)key[q] : mine;
             const uint64_t m = wave_min_u64(mine);
             if (m == kNone) break;
-            if (lane == 0) s_unique[u] = (uint32_t)m;
+            if (lane == 0) uq[u] = (uint32_t)m;
             u++;
 #pragma unroll
-            for (int i = 0; i < 16; i++)
-                if (ent[i] == (uint32_t)m) emask &= ~(1u << i);
+            for (int q = 0; q < 16; q++)
+                if (key[q] == (uint32_t)m) rem &= ~(1u << q);
         }
+        __builtin_amdgcn_wave_barrier();
         // decoupled look-back over the frame's tiles (row-major order)
         const uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, u);
-        if (lane == 0) {
-            s_count = u;
-            s_prefix = prefix;
+        for (uint32_t i = lane; i < u; i += VREN_WAVE) {
+            const uint32_t idx = prefix + i;
+            if (idx < fd.key_cap) fd.keys[idx] = uq[i];
         }
-    }
-    __syncthreads();
-    const uint32_t count = s_count, prefix = s_prefix;
-    for (uint32_t i = tid; i < count; i += 256) {
-        const uint32_t idx = prefix + i;
-        if (idx < fd.key_cap) fd.keys[idx] = s_unique[i];
-    }
-    if (tid == 0 && t == fd.Sx * fd.Sy - 1) {
-        const uint32_t total = prefix + count;
-        *fd.cluster_count = total;
-        if (total > fd.key_cap) atomic_or_agent(fd.flags, VREN_FLAG_KEY_CAPACITY);
-    }
-    if (gy < fd.H) {
-        uint32_t ref[4];
+        if (lane == 0 && t == fd.Sx * fd.Sy - 1) {
+            const uint32_t total = prefix + u;
+            *fd.cluster_count = total;
+            if (total > fd.key_cap) atomic_or_agent(fd.flags, VREN_FLAG_KEY_CAPACITY);
+        }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            uint32_t lo = 0, hi = count;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_unique[mid] <= key[q]) lo = mid;
-                else hi = mid;
+        for (int j = 0; j < 4; j++) {
+            const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
+            if (y >= fd.H) continue;
+            uint32_t ref[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uint32_t lo = 0, hi = u;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (uq[mid] <= key[4 * j + q]) lo = mid;
+                    else hi = mid;
+                }
+                ref[q] = prefix + lo;
             }
-            ref[q] = prefix + lo;
-        }
-        uint32_t* row = fd.refs + (uint64_t)gy * fd.W;
-        if (gx0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + gx0) & 15) == 0)) {
-            *reinterpret_cast<uint4*>(row + gx0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-        } else {
+            uint32_t* row = fd.refs + (uint64_t)y * fd.W;
+            if (x0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + x0) & 15) == 0)) {
+                *reinterpret_cast<uint4*>(row + x0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+            } else {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (gx0 + q < fd.W) row[gx0 + q] = ref[q];
+                for (int q = 0; q < 4; q++)
+                    if (x0 + q < fd.W) row[x0 + q] = ref[q];
+            }
         }
     }
 }
@@ -262,7 +269,8 @@ int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w)
 
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    clusters_kernel<<<w.total_tiles, 256, 0, s>>>(d_frames, F, w.tile_status, w.tickets + 0);
+    const uint32_t blocks = (w.total_tiles + kClusterWaves - 1) / kClusterWaves;
+    clusters_kernel<<<blocks, kClusterWaves * 64, 0, s>>>(d_frames, F, w.total_tiles, w.tile_status, w.tickets + 0);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

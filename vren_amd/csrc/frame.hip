@@ -65,6 +65,8 @@ void vren_frame_setup(FrameDev* f, uint32_t W, uint32_t H, const vren_camera* ca
     mat4_inverse_d(proj_d, f->inv_proj_d);
     mat4_inverse_d(view_d, f->inv_view_d);
     for (int i = 0; i < 16; i++) f->inv_proj[i] = (float)f->inv_proj_d[i];
+    f->proj_zw_xy_zero = (f->inv_proj[0 * 4 + 2] == 0.0f && f->inv_proj[1 * 4 + 2] == 0.0f &&
+                          f->inv_proj[0 * 4 + 3] == 0.0f && f->inv_proj[1 * 4 + 3] == 0.0f) ? 1u : 0u;
     for (int i = 0; i < 3; i++) f->eye[i] = (double)cam->position[i];
     f->inv_W = 1.0 / (double)W;
     f->inv_H = 1.0 / (double)H;
