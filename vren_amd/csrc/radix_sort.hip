@@ -84,22 +84,45 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict
     }
 }
 
+// exclusive scan of each pass's digit histogram in place: the digit's first output position. One workgroup.
+template <int BITS>
+__global__ __launch_bounds__(256) void rs_hexscan_kernel(uint32_t* hist, uint32_t passes)
+{
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_wave[4];
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t v = threadIdx.x < (uint32_t)RADIX ? hist[p * RADIX + threadIdx.x] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_sum<256>(v, s_wave, &total);
+        if (threadIdx.x < (uint32_t)RADIX) hist[p * RADIX + threadIdx.x] = ex;
+    }
+}
+
+// keys and values are staged through LDS one after the other when staging both at once would leave room for
+// fewer than two workgroups per CU
+template <int KPT, int WAVES, bool VALS>
+constexpr bool rs_sequential_staging()
+{
+    return VALS && 2 * WAVES * 64 * KPT * 4 > 80 * 1024;
+}
+
 template <int BITS, int KPT, int WAVES, bool VALS>
 __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                                  const uint32_t* __restrict__ vals_in,
                                                                  uint32_t* __restrict__ keys_out,
                                                                  uint32_t* __restrict__ vals_out, uint64_t n,
                                                                  uint32_t shift, uint32_t width,
-                                                                 const uint32_t* __restrict__ hist,
+                                                                 const uint32_t* __restrict__ digit_base,
                                                                  uint32_t* status, uint32_t* counter,
                                                                  uint32_t* __restrict__ clear_next)
 {
     constexpr int THREADS = WAVES * 64;
     constexpr int RADIX = 1 << BITS;
     constexpr int TILE = THREADS * KPT;
+    constexpr bool SEQ = rs_sequential_staging<KPT, WAVES, VALS>();
     static_assert(THREADS >= RADIX, "one thread per digit");
     __shared__ uint32_t s_keys[TILE];
-    __shared__ uint32_t s_vals[VALS ? TILE : 1];
+    __shared__ uint32_t s_vals[(VALS && !SEQ) ? TILE : 1];
     __shared__ uint32_t s_whist[WAVES * RADIX];
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_goff[RADIX];
@@ -175,11 +198,8 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             st_agent_u32(my, RS_INCL | (prefix + dcount));
         }
     }
-    // global digit base = exclusive scan of this pass's histogram (every tile recomputes it from L2)
-    uint32_t h = tid < (uint32_t)RADIX ? hist[tid] : 0u;
-    uint32_t htot;
-    uint32_t hex = block_exclusive_sum<THREADS>(h, s_scan, &htot);
-    if (tid < (uint32_t)RADIX) s_goff[tid] = hex + prefix - dstart;
+    // global digit base (exclusive scan of this pass's histogram, rs_hexscan_kernel)
+    if (tid < (uint32_t)RADIX) s_goff[tid] = digit_base[tid] + prefix - dstart;
     __syncthreads();
     // scatter into LDS in tile-local sorted order
 #pragma unroll
@@ -189,17 +209,47 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             uint32_t d = digit[j];
             uint32_t dst = s_start[d] + s_whist[wave * RADIX + d] + local[j];
             s_keys[dst] = key[j];
-            if (VALS) s_vals[dst] = val[j];
+            if (VALS && !SEQ) s_vals[dst] = val[j];
         }
     }
     __syncthreads();
-    for (uint32_t i = tid; i < tile_count; i += THREADS) {
-        uint32_t k = s_keys[i];
-        uint32_t d = (k >> shift) & mask;
-        uint32_t o = s_goff[d] + i;
-        if (o < n) {
-            keys_out[o] = k;
-            if (VALS) vals_out[o] = s_vals[i];
+    if (!SEQ) {
+        for (uint32_t i = tid; i < tile_count; i += THREADS) {
+            uint32_t k = s_keys[i];
+            uint32_t d = (k >> shift) & mask;
+            uint32_t o = s_goff[d] + i;
+            if (o < n) {
+                keys_out[o] = k;
+                if (VALS) vals_out[o] = s_vals[i];
+            }
+        }
+    } else {
+        // keys first, keeping each staged position's destination, then the values through the same buffer
+        uint32_t o[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            const uint32_t i = tid + j * THREADS;
+            o[j] = 0xFFFFFFFFu;
+            if (i < tile_count) {
+                const uint32_t k = s_keys[i];
+                o[j] = s_goff[(k >> shift) & mask] + i;
+                if (o[j] < n) keys_out[o[j]] = k;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            uint32_t pos = wbase + j * VREN_WAVE + lane;
+            if (pos < tile_count) {
+                uint32_t d = digit[j];
+                s_keys[s_start[d] + s_whist[wave * RADIX + d] + local[j]] = val[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            const uint32_t i = tid + j * THREADS;
+            if (i < tile_count && o[j] < n) vals_out[o[j]] = s_keys[i];
         }
     }
 }
@@ -254,6 +304,8 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
     uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
     rs_hist_kernel<BITS><<<hb, 256, 0, s>>>(keys, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
     VREN_LAUNCH_CHECK(ctx);
+    rs_hexscan_kernel<BITS><<<1, 256, 0, s>>>(hist, passes);
+    VREN_LAUNCH_CHECK(ctx);
     uint32_t *ksrc = keys, *vsrc = vals, *kdst = tk, *vdst = tv;
     for (uint32_t p = 0; p < passes; p++) {
         uint32_t shift = begin_bit + p * BITS;
@@ -287,7 +339,7 @@ struct SortConfig {
 #define RS_CFG(B, K, W) {B, K, W, &run_sort<B, K, W>}
 const SortConfig kConfigs[] = {
     RS_CFG(8, 8, 4),  RS_CFG(8, 15, 4), RS_CFG(8, 23, 4), RS_CFG(8, 8, 8),  RS_CFG(8, 15, 8),
-    RS_CFG(8, 23, 8), RS_CFG(8, 8, 16), RS_CFG(6, 8, 4),  RS_CFG(6, 15, 4), RS_CFG(6, 23, 4),
+    RS_CFG(8, 23, 8), RS_CFG(8, 8, 16), RS_CFG(8, 15, 16), RS_CFG(8, 23, 16), RS_CFG(6, 8, 4),  RS_CFG(6, 15, 4), RS_CFG(6, 23, 4),
     RS_CFG(6, 8, 8),  RS_CFG(6, 15, 8), RS_CFG(6, 23, 8), RS_CFG(6, 8, 16), RS_CFG(4, 8, 4),
     RS_CFG(4, 15, 4), RS_CFG(4, 23, 4), RS_CFG(4, 8, 8),  RS_CFG(4, 15, 8), RS_CFG(4, 23, 8),
     RS_CFG(4, 8, 16),
