@@ -150,13 +150,22 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
         if (active) {
             level = e >> kLevelShift;
             const uint32_t addr = (e & kAddrMask) + sub;
+            const bool at_leaf = level == leaf_level && addr < fd.L;
+            // a leaf's light index and view position are addressed by the leaf slot, so they load together with
+            // the node (leaf_pos holds the positions in leaf order; without it the position needs the index)
+            uint32_t li = 0xFFFFFFFFu;
+            float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (at_leaf) {
+                li = codes[addr].y;
+                if (fd.leaf_pos) lp = reinterpret_cast<const float4*>(fd.leaf_pos)[addr];
+            }
             const float4* q = reinterpret_cast<const float4*>(&bvh[addr]);
             const float4 lo = q[0], hi = q[1];
             const uint32_t next = __float_as_uint(lo.w);
             if (level == leaf_level) {
-                if (next == VREN_BVH_LEAF_NODE && addr < fd.L) {
-                    const uint32_t li = codes[addr].y;
-                    leaf_hit = li < fd.L && test_leaf(box, vpos[li], lo.x, hi.x);
+                if (next == VREN_BVH_LEAF_NODE && at_leaf && li < fd.L) {
+                    if (!fd.leaf_pos) lp = vpos[li];
+                    leaf_hit = test_leaf(box, lp, lo.x, hi.x);
                     if (MODE == 1 && leaf_hit) atomicOr(&bitmask[li >> 5], 1u << (li & 31u));
                 }
             } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {
@@ -177,21 +186,59 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
     return hits;
 }
 
-__global__ void assign_count_kernel(const FrameDev* frames, uint32_t F)
+// Emits the lights set in a wave's bitmask in ascending index to dst[0 ..) (nothing when dst is null), clearing
+// the mask. Returns the number emitted.
+__device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t words, uint32_t* dst, uint32_t cap)
+{
+    const uint32_t lane = lane_id();
+    uint32_t out = 0;
+    for (uint32_t w0 = 0; w0 < words; w0 += VREN_WAVE) {
+        const uint32_t wi = w0 + lane;
+        uint32_t bits = wi < words ? bitmask[wi] : 0u;
+        if (wi < words && bits) bitmask[wi] = 0u;
+        const uint32_t cnt = (uint32_t)__popc(bits);
+        const uint32_t incl = wave_inclusive_sum(cnt);
+        uint32_t pos = out + incl - cnt;
+        if (dst) {
+            while (bits) {
+                const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (pos < cap) dst[pos] = wi * 32u + b;
+                pos++;
+            }
+        }
+        out += shfl(incl, VREN_WAVE - 1);
+    }
+    return out;
+}
+
+// COUNT (assign_lights.comp, count pass): the traversal marks the hit lights in the wave's LDS bitmask; the count
+// is stored, and a list of at most VREN_LIST_SLOT lights is kept in the cluster's slot (ascending), so the writing
+// pass copies it instead of traversing again.
+__global__ void assign_count_kernel(const FrameDev* frames, uint32_t F, uint32_t mask_words)
 {
     extern __shared__ uint32_t s_mem[];
+    const uint32_t wpb = blockDim.x / VREN_WAVE;
     const uint32_t wave = threadIdx.x / VREN_WAVE;
     uint32_t* stack = s_mem + wave * kStack;
-    const uint32_t waves_total = gridDim.x * (blockDim.x / VREN_WAVE);
-    const uint32_t gw = blockIdx.x * (blockDim.x / VREN_WAVE) + wave;
+    uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
+    for (uint32_t i = lane_id(); i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t waves_total = gridDim.x * wpb;
+    const uint32_t gw = blockIdx.x * wpb + wave;
     for (uint32_t f = 0; f < F; f++) {
         const FrameDev& fd = frames[f];
         if (fd.L == 0 || fd.cluster_count == nullptr) continue;
         const uint32_t n = min(*fd.cluster_count, fd.key_cap);
+        const uint32_t words = (fd.L + 31) / 32;
         for (uint32_t c = gw; c < n; c += waves_total) {
             const Box box = cluster_aabb(fd, fd.keys[c]);
-            const uint32_t h = traverse<0>(fd, box, stack, nullptr);
+            const uint32_t h = traverse<1>(fd, box, stack, bitmask);
+            __builtin_amdgcn_wave_barrier();
+            const bool keep = fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT;
+            emit_bitmask(bitmask, words, keep ? fd.list_slots + (size_t)c * VREN_LIST_SLOT : nullptr, VREN_LIST_SLOT);
             if (lane_id() == 0) fd.counts[c] = h;
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -260,26 +307,20 @@ __global__ void assign_write_kernel(const FrameDev* frames, uint32_t F, uint32_t
         const uint32_t n = min(*fd.cluster_count, fd.key_cap);
         const uint32_t words = (fd.L + 31) / 32;
         for (uint32_t c = gw; c < n; c += waves_total) {
+            const uint32_t cnt = fd.counts[c];
+            const uint32_t out = fd.offsets[c];
+            if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
+                // the counting pass kept this list
+                if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
+                    fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
+                continue;
+            }
             const Box box = cluster_aabb(fd, fd.keys[c]);
             traverse<1>(fd, box, stack, bitmask);
             __builtin_amdgcn_wave_barrier();
             // emit set bits in ascending light index, clearing the mask as we go
-            uint32_t out = fd.offsets[c];
-            for (uint32_t w0 = 0; w0 < words; w0 += VREN_WAVE) {
-                const uint32_t wi = w0 + lane;
-                uint32_t bits = wi < words ? bitmask[wi] : 0u;
-                if (wi < words && bits) bitmask[wi] = 0u;
-                const uint32_t cnt = (uint32_t)__popc(bits);
-                const uint32_t incl = wave_inclusive_sum(cnt);
-                uint32_t pos = out + incl - cnt;
-                while (bits) {
-                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                    bits &= bits - 1u;
-                    if (pos < fd.index_cap) fd.indices[pos] = wi * 32u + b;
-                    pos++;
-                }
-                out += shfl(incl, VREN_WAVE - 1);
-            }
+            const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
+            emit_bitmask(bitmask, words, fd.indices + out, room);
             __builtin_amdgcn_wave_barrier();
         }
     }
@@ -298,7 +339,7 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     uint32_t wpb = 4;
     while (wpb > 1 && per_wave * wpb > 150 * 1024) wpb--;
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
-    assign_count_kernel<<<persistent_blocks(ctx, 4), 4 * VREN_WAVE, 4 * kStack * 4, s>>>(d_frames, F);
+    assign_count_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     assign_offsets_kernel<<<w.total_offs_tiles, 256, 0, s>>>(d_frames, F, w.offs_status, w.tickets + 1);

@@ -82,7 +82,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->leaf_slots.resize(n_frames);
     // sizes and arena layout
     uint32_t tile_base = 0, leaf_base = 0, offs_base = 0, max_L = 0;
-    std::vector<size_t> off(n_frames * 12);
+    std::vector<size_t> off(n_frames * 14);
     size_t arena = 0;
     for (uint32_t i = 0; i < n_frames; i++) {
         const vren_frame_desc& d = frames[i];
@@ -121,7 +121,9 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         leaf_base += slots;
         offs_base += (key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
         max_L = std::max(max_L, L);
-        size_t* o = &off[i * 12];
+        const uint32_t slot_cap = std::min<uint32_t>(key_cap, 1u << 16);
+        f.slot_cap = L ? slot_cap : 0u;
+        size_t* o = &off[i * 14];
         o[0] = arena; arena += a256((size_t)L * 16);                 // view_pos
         o[1] = arena; arena += a256((size_t)L * 8);                  // codes
         o[2] = arena; arena += a256(L > 4096 ? (size_t)L * 8 : 0);   // codes_tmp
@@ -133,6 +135,8 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         o[8] = arena; arena += a256((size_t)key_cap * 4);            // offsets
         o[9] = arena; arena += a256((size_t)index_cap * 4);          // indices
         o[10] = arena; arena += 256;                                 // assigned_total
+        o[11] = arena; arena += a256((size_t)L * 16);                // leaf_pos
+        o[12] = arena; arena += a256((size_t)f.slot_cap * VREN_LIST_SLOT * 4); // list slots
     }
     hipError_t e = hipMalloc(&b->arena, std::max<size_t>(arena, 256));
     if (e == hipSuccess) e = hipMemset(b->arena, 0, std::max<size_t>(arena, 256));
@@ -147,7 +151,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->arena_bytes = arena;
     for (uint32_t i = 0; i < n_frames; i++) {
         FrameDev& f = b->host[i];
-        size_t* o = &off[i * 12];
+        size_t* o = &off[i * 14];
         f.view_pos = reinterpret_cast<float*>(b->arena + o[0]);
         f.codes = reinterpret_cast<uint32_t*>(b->arena + o[1]);
         f.codes_tmp = f.L > 4096 ? reinterpret_cast<uint32_t*>(b->arena + o[2]) : nullptr;
@@ -159,6 +163,8 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.offsets = reinterpret_cast<uint32_t*>(b->arena + o[8]);
         f.indices = reinterpret_cast<uint32_t*>(b->arena + o[9]);
         f.assigned_total = reinterpret_cast<uint32_t*>(b->arena + o[10]);
+        f.leaf_pos = f.L ? reinterpret_cast<float*>(b->arena + o[11]) : nullptr;
+        f.list_slots = f.slot_cap ? reinterpret_cast<uint32_t*>(b->arena + o[12]) : nullptr;
     }
     chain_work_bind(&b->w, b->work, tile_base, offs_base);
     b->w.total_leaf_slots = leaf_base;

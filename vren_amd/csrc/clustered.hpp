@@ -8,6 +8,9 @@
 
 #include "common.hpp"
 
+// light lists of at most this many entries are kept by the counting traversal and copied by the writing pass
+#define VREN_LIST_SLOT 16u
+
 struct FrameDev {
     // sizes
     uint32_t W, H, Sx, Sy;
@@ -41,6 +44,7 @@ struct FrameDev {
     uint32_t* codes_tmp;      // uint2[L] scratch for the sort
     vren_bvh_node* bvh;
     uint32_t bvh_padded, bvh_root, bvh_levels, bvh_length;
+    float* leaf_pos;          // optional float4[L]: view position of sorted leaf i (= view_pos[codes[i].y])
     // G-buffer
     const float* depth;
     const uint16_t* normals;
@@ -57,6 +61,8 @@ struct FrameDev {
     uint32_t* offsets;
     uint32_t* indices;
     uint32_t* assigned_total; // device u32
+    uint32_t* list_slots;     // optional: VREN_LIST_SLOT entries per cluster for clusters below slot_cap
+    uint32_t slot_cap;
     uint64_t* offset_status;  // look-back statuses of the offset scan (ceil(key_cap / 4096) words)
     // output
     float* out;

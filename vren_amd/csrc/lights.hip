@@ -228,6 +228,7 @@ __global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* frames, u
             uint32_t li = reinterpret_cast<const uint2*>(fd.codes)[i].y;
             float4 vp = reinterpret_cast<const float4*>(fd.view_pos)[li];
             float r = reinterpret_cast<const float4*>(fd.lights)[li].w;
+            if (fd.leaf_pos) reinterpret_cast<float4*>(fd.leaf_pos)[i] = vp;
             mn[0] = vp.x - r;
             mn[1] = vp.y - r;
             mn[2] = vp.z - r;
