@@ -6,8 +6,8 @@
 // out in traversal order with reversed lanes inside each leaf block.
 // Here: persistent kernels, one wave64 per cluster, the wave tests the 32 children of TWO pending BVH blocks
 // at once (lanes 0-31 and 32-63) from a per-wave LDS stack, so no lane idles at the 32-wide fan-out; the
-// number of clusters is read on the device (it never returns to the host). COUNT counts leaf hits; a
-// segmented single-pass scan over the device-side cluster count gives the offsets; WRITE sets one bit per hit
+// number of clusters is read on the device (it never returns to the host). COUNT counts leaf hits; one
+// workgroup per frame scans the counts into offsets; WRITE sets one bit per hit
 // light in a per-wave LDS bitmask and emits the list in ascending light index (the canonical order) by a
 // popcount prefix over the bitmask words.
 // The cluster AABB and both tests are restated operation for operation (including the inverted y range of
@@ -186,6 +186,30 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
     return hits;
 }
 
+// Clusters of every frame of the table are processed as one flattened range: each wave loads the device-side
+// cluster counts of 64 frames at once (one per lane) and scans them, instead of walking the frames one dependent
+// load after another. Calls fn(fd, cluster) for the wave's share (clusters gw, gw + stride, ...).
+template <typename Fn>
+__device__ __forceinline__ void for_each_cluster(const FrameDev* frames, uint32_t F, uint32_t gw, uint32_t stride, Fn fn)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t f0 = 0; f0 < F; f0 += VREN_WAVE) {
+        const uint32_t fi = f0 + lane;
+        uint32_t nf = 0;
+        if (fi < F) {
+            const FrameDev& d = frames[fi];
+            if (d.L != 0 && d.cluster_count != nullptr) nf = min(*d.cluster_count, d.key_cap);
+        }
+        const uint32_t incl = wave_inclusive_sum(nf);
+        const uint32_t total = shfl(incl, VREN_WAVE - 1);
+        for (uint32_t c = gw; c < total; c += stride) {
+            const uint32_t fl = (uint32_t)__builtin_ctzll(ballot(incl > c));
+            const uint32_t base = shfl(incl - nf, (int)fl);
+            fn(frames[f0 + fl], c - base);
+        }
+    }
+}
+
 // Emits the lights set in a wave's bitmask in ascending index to dst[0 ..) (nothing when dst is null), clearing
 // the mask. Returns the number emitted.
 __device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t words, uint32_t* dst, uint32_t cap)
@@ -224,68 +248,50 @@ __global__ void assign_count_kernel(const FrameDev* frames, uint32_t F, uint32_t
     uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
     for (uint32_t i = lane_id(); i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
     __builtin_amdgcn_wave_barrier();
-    const uint32_t waves_total = gridDim.x * wpb;
     const uint32_t gw = blockIdx.x * wpb + wave;
-    for (uint32_t f = 0; f < F; f++) {
-        const FrameDev& fd = frames[f];
-        if (fd.L == 0 || fd.cluster_count == nullptr) continue;
-        const uint32_t n = min(*fd.cluster_count, fd.key_cap);
-        const uint32_t words = (fd.L + 31) / 32;
-        for (uint32_t c = gw; c < n; c += waves_total) {
-            const Box box = cluster_aabb(fd, fd.keys[c]);
-            const uint32_t h = traverse<1>(fd, box, stack, bitmask);
-            __builtin_amdgcn_wave_barrier();
-            const bool keep = fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT;
-            emit_bitmask(bitmask, words, keep ? fd.list_slots + (size_t)c * VREN_LIST_SLOT : nullptr, VREN_LIST_SLOT);
-            if (lane_id() == 0) fd.counts[c] = h;
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
+    for_each_cluster(frames, F, gw, gridDim.x * wpb, [&](const FrameDev& fd, uint32_t c) {
+        const Box box = cluster_aabb(fd, fd.keys[c]);
+        const uint32_t h = traverse<1>(fd, box, stack, bitmask);
+        __builtin_amdgcn_wave_barrier();
+        const bool keep = fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT;
+        emit_bitmask(bitmask, (fd.L + 31) / 32, keep ? fd.list_slots + (size_t)c * VREN_LIST_SLOT : nullptr,
+                     VREN_LIST_SLOT);
+        if (lane_id() == 0) fd.counts[c] = h;
+        __builtin_amdgcn_wave_barrier();
+    });
 }
 
-// segmented exclusive scan of counts[0 .. min(cluster_count, key_cap)) per frame -> offsets, assigned_total
-__global__ __launch_bounds__(256) void assign_offsets_kernel(const FrameDev* frames, uint32_t F, uint64_t* status,
-                                                             uint32_t* ticket)
+// exclusive scan of counts[0 .. min(cluster_count, key_cap)) -> offsets and assigned_total: one workgroup per
+// frame walks its counts 4096 at a time with a running carry (no cross-workgroup chaining, no tickets; a frame
+// holds a few thousand clusters, at most key_cap)
+__global__ __launch_bounds__(1024) void assign_offsets_kernel(const FrameDev* frames)
 {
-    constexpr uint32_t ITEMS = VREN_SCAN_TILE / 256;
-    __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_t, s_prefix;
+    __shared__ uint32_t s_wave[16];
+    const FrameDev& fd = frames[blockIdx.x];
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) s_t = atomic_add_agent(ticket, 1u);
-    __syncthreads();
-    const uint32_t g = s_t;
-    const uint32_t f = find_frame(frames, F, g, [](const FrameDev& d) { return d.offs_base; });
-    const FrameDev& fd = frames[f];
-    const uint32_t t = g - fd.offs_base;
-    const uint32_t n = (fd.L == 0) ? 0u : min(*fd.cluster_count, fd.key_cap);
-    const uint32_t start = t * VREN_SCAN_TILE;
-    if (start >= n) {
-        if (t == 0 && tid == 0) *fd.assigned_total = 0u;
-        return;
+    const uint32_t n = (fd.L == 0 || fd.cluster_count == nullptr) ? 0u : min(*fd.cluster_count, fd.key_cap);
+    uint32_t carry = 0;
+    for (uint32_t start = 0; start < n; start += 4096) {
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = start + tid * 4 + k;
+            v[k] = i < n ? fd.counts[i] : 0u;
+            sum += v[k];
+        }
+        uint32_t agg;
+        uint32_t run = carry + block_exclusive_sum<1024>(sum, s_wave, &agg);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = start + tid * 4 + k;
+            if (i < n) fd.offsets[i] = run;
+            run += v[k];
+        }
+        carry += agg;
     }
-    uint32_t v[ITEMS], sum = 0;
-    for (uint32_t k = 0; k < ITEMS; k++) {
-        uint32_t i = start + tid * ITEMS + k;
-        v[k] = i < n ? fd.counts[i] : 0u;
-        sum += v[k];
-    }
-    uint32_t agg;
-    uint32_t excl = block_exclusive_sum<256>(sum, s_wave, &agg);
-    if (tid < VREN_WAVE) {
-        uint32_t prefix = wave_lookback(status + fd.offs_base, t, agg);
-        if (tid == 0) s_prefix = prefix;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + excl;
-    for (uint32_t k = 0; k < ITEMS; k++) {
-        uint32_t i = start + tid * ITEMS + k;
-        if (i < n) fd.offsets[i] = run;
-        run += v[k];
-    }
-    if (tid == 0 && start + VREN_SCAN_TILE >= n) {
-        uint32_t total = s_prefix + agg;
-        *fd.assigned_total = total;
-        if (total > fd.index_cap) atomic_or_agent(fd.flags, VREN_FLAG_INDEX_CAPACITY);
+    if (tid == 0) {
+        *fd.assigned_total = carry;
+        if (carry > fd.index_cap) atomic_or_agent(fd.flags, VREN_FLAG_INDEX_CAPACITY);
     }
 }
 
@@ -299,31 +305,24 @@ __global__ void assign_write_kernel(const FrameDev* frames, uint32_t F, uint32_t
     uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
     for (uint32_t i = lane; i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
     __builtin_amdgcn_wave_barrier();
-    const uint32_t waves_total = gridDim.x * wpb;
     const uint32_t gw = blockIdx.x * wpb + wave;
-    for (uint32_t f = 0; f < F; f++) {
-        const FrameDev& fd = frames[f];
-        if (fd.L == 0 || fd.cluster_count == nullptr) continue;
-        const uint32_t n = min(*fd.cluster_count, fd.key_cap);
-        const uint32_t words = (fd.L + 31) / 32;
-        for (uint32_t c = gw; c < n; c += waves_total) {
-            const uint32_t cnt = fd.counts[c];
-            const uint32_t out = fd.offsets[c];
-            if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
-                // the counting pass kept this list
-                if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
-                    fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
-                continue;
-            }
-            const Box box = cluster_aabb(fd, fd.keys[c]);
-            traverse<1>(fd, box, stack, bitmask);
-            __builtin_amdgcn_wave_barrier();
-            // emit set bits in ascending light index, clearing the mask as we go
-            const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
-            emit_bitmask(bitmask, words, fd.indices + out, room);
-            __builtin_amdgcn_wave_barrier();
+    for_each_cluster(frames, F, gw, gridDim.x * wpb, [&](const FrameDev& fd, uint32_t c) {
+        const uint32_t cnt = fd.counts[c];
+        const uint32_t out = fd.offsets[c];
+        if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
+            // the counting pass kept this list
+            if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
+                fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
+            return;
         }
-    }
+        const Box box = cluster_aabb(fd, fd.keys[c]);
+        traverse<1>(fd, box, stack, bitmask);
+        __builtin_amdgcn_wave_barrier();
+        // emit set bits in ascending light index, clearing the mask as we go
+        const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
+        emit_bitmask(bitmask, (fd.L + 31) / 32, fd.indices + out, room);
+        __builtin_amdgcn_wave_barrier();
+    });
 }
 
 // enough resident waves per CU (up to 32) to hide the dependent node loads of the traversals
@@ -342,7 +341,7 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     assign_count_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
-    assign_offsets_kernel<<<w.total_offs_tiles, 256, 0, s>>>(d_frames, F, w.offs_status, w.tickets + 1);
+    assign_offsets_kernel<<<F, 1024, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     assign_write_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
