@@ -64,16 +64,6 @@ __device__ __forceinline__ uint32_t face_of_normal(const float n[3])
     return ax < 3 ? (n[ax] > 0.0f ? 1u : 0u) * 3u + ax : 0u;
 }
 
-// f16 -> f32 on the conversion unit (exact); Inf/NaN halves keep the shared software path, whose NaN payload
-// handling is the oracle's
-__device__ __forceinline__ float h2f(uint16_t h)
-{
-    if ((h & 0x7C00u) == 0x7C00u) return vren_half_to_float(h);
-    _Float16 x;
-    __builtin_memcpy(&x, &h, 2);
-    return (float)x;
-}
-
 __device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t gx, uint32_t gy,
                                               float depth, uint2 nb)
 {
@@ -95,7 +85,8 @@ __device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, u
         r3 = r[3];
     }
     const float z = r2 / r3;
-    const float n[3] = {h2f((uint16_t)(nb.x & 0xFFFFu)), h2f((uint16_t)(nb.x >> 16)), h2f((uint16_t)(nb.y & 0xFFFFu))};
+    const float n[3] = {vren_h2f((uint16_t)(nb.x & 0xFFFFu)), vren_h2f((uint16_t)(nb.x >> 16)),
+                        vren_h2f((uint16_t)(nb.y & 0xFFFFu))};
     if (fd.key_mode == VREN_KEYS_SLICED) {
         float s = vren_floorf(vren_logf(z / fd.near_plane) / fd.log_fn * (float)fd.depth_slices);
         uint32_t k = vren_f2u_sat(s);
@@ -113,17 +104,13 @@ __global__ void chain_init_kernel(uint64_t* status, uint32_t n_status, uint32_t*
     if (blockIdx.x == 0 && threadIdx.x < 4) tickets[threadIdx.x] = 0u;
 }
 
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        uint64_t o = shfl_xor(v, m);
-        v = o < v ? o : v;
-    }
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, shfl_xor(v, m));
     return v;
 }
 
-constexpr uint64_t kNone = 1ull << 32; // larger than every 32-bit key
 constexpr uint32_t kClusterWaves = 8;   // waves per workgroup, one 32x32 tile per wave
 
 // the 4 horizontally adjacent pixels (x0 .. x0+3, y) of a tile row: one 16-byte depth load and two 16-byte normal
@@ -191,21 +178,21 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
 #pragma unroll
             for (int q = 0; q < 4; q++) key[4 * j + q] = pixel_key(fd, ti, tj, x0 + q, y, dep[q], nrm[q]);
         }
-        // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key)
+        // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key); lanes
+        // with no key left offer 0xFFFFFFFF, which only wins when some lane still holds that very key
         uint32_t rem = 0xFFFFu;
         uint32_t u = 0;
-        while (true) {
-            uint64_t mine = kNone;
+        while (ballot(rem != 0u)) {
+            uint32_t mine = 0xFFFFFFFFu;
 #pragma unroll
             for (int q = 0; q < 16; q++)
-                if ((rem >> q) & 1u) mine = (uint64_t)key[q] < mine ? (uint64_t)key[q] : mine;
-            const uint64_t m = wave_min_u64(mine);
-            if (m == kNone) break;
-            if (lane == 0) uq[u] = (uint32_t)m;
+                if ((rem >> q) & 1u) mine = min(mine, key[q]);
+            const uint32_t m = wave_min_u32(mine);
+            if (lane == 0) uq[u] = m;
             u++;
 #pragma unroll
             for (int q = 0; q < 16; q++)
-                if (key[q] == (uint32_t)m) rem &= ~(1u << q);
+                if (key[q] == m) rem &= ~(1u << q);
         }
         __builtin_amdgcn_wave_barrier();
         // decoupled look-back over the frame's tiles (row-major order)

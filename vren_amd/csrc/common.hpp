@@ -101,6 +101,16 @@ __device__ __forceinline__ void atomic_or_agent(uint32_t* p, uint32_t v)
     __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// f16 -> f32 on the conversion unit (exact); Inf/NaN halves take the shared software path, whose NaN payload
+// handling is the oracle's
+__device__ __forceinline__ float vren_h2f(uint16_t h)
+{
+    if ((h & 0x7C00u) == 0x7C00u) return vren_half_to_float(h);
+    _Float16 x;
+    __builtin_memcpy(&x, &h, 2);
+    return (float)x;
+}
+
 // min / max in the total order -0 < +0 (order-independent, so any reduction tree gives the same bits)
 __device__ __host__ __forceinline__ float tmin(float a, float b)
 {

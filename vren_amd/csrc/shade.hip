@@ -190,9 +190,9 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
         const float il = rsqrtf(dot3(V, V));
         for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
     }
-    s.N[0] = vren_half_to_float((uint16_t)(nb.x & 0xFFFFu));
-    s.N[1] = vren_half_to_float((uint16_t)(nb.x >> 16));
-    s.N[2] = vren_half_to_float((uint16_t)(nb.y & 0xFFFFu));
+    s.N[0] = vren_h2f((uint16_t)(nb.x & 0xFFFFu));
+    s.N[1] = vren_h2f((uint16_t)(nb.x >> 16));
+    s.N[2] = vren_h2f((uint16_t)(nb.y & 0xFFFFu));
     s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
     s.albedo[0] = ar.x;
     s.albedo[1] = ar.y;
@@ -206,6 +206,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
 
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (has_list) {
+        const float Pf[3] = {(float)px.P[0], (float)px.P[1], (float)px.P[2]};
         for (uint32_t e = my_off; e < my_end; e++) {
             float4 lp, lc;
             if (staged) {
@@ -216,6 +217,12 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
                 if (li >= fd.L) continue;
                 lp = lpos[li];
                 lc = lcol[li];
+            }
+            // lights clearly beyond their radius contribute exactly 0: a float distance (error < 1e-3 at scene
+            // scale) rejects them before the double evaluation
+            {
+                const float fx = Pf[0] - lp.x, fy = Pf[1] - lp.y, fz = Pf[2] - lp.z;
+                if (fx * fx + fy * fy + fz * fz > lc.w * lc.w * 1.0001f + 1e-3f) continue;
             }
             // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
             // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
@@ -467,9 +474,9 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* fra
             const float il = rsqrtf(dot3(V, V));
             for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
         }
-        s.N[0] = vren_half_to_float((uint16_t)(nb.x & 0xFFFFu));
-        s.N[1] = vren_half_to_float((uint16_t)(nb.x >> 16));
-        s.N[2] = vren_half_to_float((uint16_t)(nb.y & 0xFFFFu));
+        s.N[0] = vren_h2f((uint16_t)(nb.x & 0xFFFFu));
+        s.N[1] = vren_h2f((uint16_t)(nb.x >> 16));
+        s.N[2] = vren_h2f((uint16_t)(nb.y & 0xFFFFu));
         s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
         s.albedo[0] = ar.x;
         s.albedo[1] = ar.y;
