@@ -47,4 +47,39 @@ namespace vren
     private:
         vren::context const* m_context;
     };
+    // Karras LBVH over sorted (Morton code, light index) pairs (no reference counterpart; configuration 2's binary
+    // BVH). Node layout: vren_lbvh_node; internal nodes 0 .. n-2, leaf i at n - 1 + i (capi.h, vren_build_lbvh).
+    class build_lbvh
+    {
+    public:
+        explicit build_lbvh(vren::context const& context) : m_context(&context) {}
+
+        static size_t get_required_node_buffer_size(uint32_t leaf_count)
+        {
+            return (size_t) (2 * (size_t) leaf_count - 1) * sizeof(vren_lbvh_node);
+        }
+        static size_t get_required_parent_buffer_size(uint32_t leaf_count) { return (size_t) (2 * (size_t) leaf_count - 1) * 4; }
+
+        void operator()(vren::stream stream, vren::resource_container& resource_container,
+                        vren::device_buffer const& point_light_index_buffer, uint32_t leaf_count,
+                        vren::device_buffer const& view_space_point_light_position_buffer,
+                        vren::device_buffer const& point_light_buffer, vren::device_buffer const& node_buffer,
+                        vren::device_buffer const& parent_buffer) const
+        {
+            (void) resource_container;
+            if (leaf_count == 0 || node_buffer.m_size < get_required_node_buffer_size(leaf_count) ||
+                parent_buffer.m_size < get_required_parent_buffer_size(leaf_count))
+                throw std::invalid_argument("build_lbvh: no leaves or output buffer too small");
+            vren_ctx* ctx = m_context->handle();
+            check_status(ctx,
+                         vren_build_lbvh(ctx, stream.get(), point_light_index_buffer.at<uint32_t>(), leaf_count,
+                                         view_space_point_light_position_buffer.at<float>(),
+                                         point_light_buffer.at<float>(), node_buffer.at<vren_lbvh_node>(),
+                                         parent_buffer.at<uint32_t>()),
+                         "vren::build_lbvh");
+        }
+
+    private:
+        vren::context const* m_context;
+    };
 } // namespace vren

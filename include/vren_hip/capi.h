@@ -169,6 +169,25 @@ VREN_EXPORT uint32_t vren_calc_bvh_level_count(uint32_t leaf_count);
 VREN_EXPORT int vren_build_bvh(vren_ctx* ctx, vren_stream_t stream, vren_bvh_node* nodes,
                                uint32_t padded_leaf_count);
 
+/* Karras binary radix-tree BVH (LBVH; no reference counterpart: configuration 2 asks for it beside the reference's
+ * 32-ary tree). Leaves are the n >= 1 sorted (Morton code, light index) pairs light_codes[n] (uint2; the 64-bit keys
+ * code << 32 | index must be strictly increasing, as vren_construct_point_light_bvh emits them); leaf i's AABB is
+ * view_positions[light index] -+ radius (point_lights[.].w), as in the 32-ary leaves.
+ * nodes[2n - 1]: internal nodes 0 .. n-2 (root 0; for n == 1 the root is the only leaf), then leaf i at n - 1 + i.
+ * An internal node holds its children's node indices; a leaf has left = VREN_BVH_LEAF_NODE and right = its light
+ * index. Every internal AABB is the exact min/max of its children (refitted bottom-up). parents[2n - 1] receives
+ * each node's parent (0xFFFFFFFF for the root). Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
+ * and k-d Trees", HPG 2012. */
+typedef struct vren_lbvh_node {
+    float min[3];
+    uint32_t left;
+    float max[3];
+    uint32_t right;
+} vren_lbvh_node;
+VREN_EXPORT int vren_build_lbvh(vren_ctx* ctx, vren_stream_t stream, const uint32_t* light_codes, uint32_t n,
+                                const float* view_positions, const float* point_lights, vren_lbvh_node* nodes,
+                                uint32_t* parents);
+
 /* ---------------------------------------------------------------------------------------------------------
  * Clustered shading (vren/vren/pipeline/clustered_shading.{hpp,cpp})
  * ------------------------------------------------------------------------------------------------------- */

@@ -16,6 +16,8 @@
  *   o_bucket_sort        numpy argsort(kind='stable') on the bucket id
  *   o_calc_bvh_*         build_bvh.utils known answers (vren_test/vren_test/primitives/build_bvh.cpp:223-254)
  *   o_build_bvh          numpy min/max over each group of 32 valid children; point-query property test
+ *   o_build_lbvh         radix-tree property of every internal node (its leaves are a contiguous key range split
+ *                        at the first differing bit), exact child unions, sphere queries vs brute force
  *   o_camera_matrices    closed forms of camera.cpp:32-50 for the fixture cameras
  *   o_find_unique_clusters   numpy sort/unique per tile, refs invariant keys[ref[p]] == key(p), key(p) of
  *                        sampled pixels recomputed from the formula
@@ -575,6 +577,79 @@ OEXPORT void o_construct_point_light_bvh(const float* pos4, const float* light4,
     }
     /* 8. build_bvh */
     o_build_bvh(bvh, padded);
+}
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Karras LBVH over the sorted (code, index) pairs (Karras, HPG 2012, "Maximizing Parallelism in the Construction
+ * of BVHs, Octrees, and k-d Trees", section 4 and figure 4), with a sequential bottom-up refit.
+ * ---------------------------------------------------------------------------------------------------------- */
+
+static int lbvh_delta(const uint64_t* k, int64_t n, int64_t i, int64_t j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint64_t x = k[i] ^ k[j];
+    return x ? __builtin_clzll(x) : 64;
+}
+
+static void lbvh_refit(vren_lbvh_node* nodes, uint32_t node, uint32_t n)
+{
+    if (node >= n - 1) return; /* leaf */
+    vren_lbvh_node& p = nodes[node];
+    lbvh_refit(nodes, p.left, n);
+    lbvh_refit(nodes, p.right, n);
+    const vren_lbvh_node& a = nodes[p.left];
+    const vren_lbvh_node& b = nodes[p.right];
+    for (int ax = 0; ax < 3; ax++) {
+        p.min[ax] = tmin(a.min[ax], b.min[ax]);
+        p.max[ax] = tmax(a.max[ax], b.max[ax]);
+    }
+}
+
+OEXPORT void o_build_lbvh(const uint32_t* codes, uint32_t n, const float* view_pos4, const float* light4,
+                          vren_lbvh_node* nodes, uint32_t* parents)
+{
+    std::vector<uint64_t> k(n);
+    for (uint32_t i = 0; i < n; i++) k[i] = ((uint64_t)codes[2 * i] << 32) | codes[2 * i + 1];
+    for (uint32_t i = 0; i < 2 * n - 1; i++) parents[i] = 0xFFFFFFFFu;
+    /* leaves */
+    for (uint32_t i = 0; i < n; i++) {
+        vren_lbvh_node& l = nodes[n - 1 + i];
+        uint32_t li = codes[2 * i + 1];
+        float r = light4[4 * li + 3];
+        for (int ax = 0; ax < 3; ax++) {
+            l.min[ax] = view_pos4[4 * li + ax] - r;
+            l.max[ax] = view_pos4[4 * li + ax] + r;
+        }
+        l.left = VREN_BVH_LEAF_NODE;
+        l.right = li;
+    }
+    /* internal nodes: direction, range end, split (figure 4) */
+    const int64_t N = n;
+    for (int64_t i = 0; i + 1 < N; i++) {
+        int d = lbvh_delta(k.data(), N, i, i + 1) - lbvh_delta(k.data(), N, i, i - 1) > 0 ? 1 : -1;
+        int dmin = lbvh_delta(k.data(), N, i, i - d);
+        int64_t lmax = 2;
+        while (lbvh_delta(k.data(), N, i, i + lmax * d) > dmin) lmax *= 2;
+        int64_t l = 0;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (lbvh_delta(k.data(), N, i, i + (l + t) * d) > dmin) l += t;
+        int64_t j = i + l * d;
+        int dnode = lbvh_delta(k.data(), N, i, j);
+        int64_t s = 0;
+        int64_t t = l;
+        do {
+            t = (t + 1) / 2;
+            if (lbvh_delta(k.data(), N, i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+        uint32_t left = (uint32_t)(std::min(i, j) == gamma ? (N - 1 + gamma) : gamma);
+        uint32_t right = (uint32_t)(std::max(i, j) == gamma + 1 ? (N - 1 + gamma + 1) : gamma + 1);
+        nodes[i].left = left;
+        nodes[i].right = right;
+        parents[left] = (uint32_t)i;
+        parents[right] = (uint32_t)i;
+    }
+    if (n > 1) lbvh_refit(nodes, 0, n);
 }
 
 /* ------------------------------------------------------------------------------------------------------------

@@ -111,6 +111,7 @@ class Oracle:
 from vren_amd._lib import Camera, ClusterCfg  # noqa: E402  (struct layouts only; nothing is loaded)
 
 BVH_DTYPE = np.dtype([("min", "<f4", 3), ("next", "<u4"), ("max", "<f4", 3), ("pad", "<u4")])
+LBVH_DTYPE = np.dtype([("min", "<f4", 3), ("left", "<u4"), ("max", "<f4", 3), ("right", "<u4")])
 
 
 def camera(W, H, fov_deg=90.0, pos=(0.0, 0.0, -50.0), near=0.1, far=100.0, yaw=0.0, pitch=0.0):
@@ -188,6 +189,14 @@ class ClusteredOracle:
         self.lib.o_construct_point_light_bvh(_p(pos), _p(lig), c_u32(L), _bref(cam), _bref(cf), _p(vp), _p(codes),
                                              _p(bvh))
         return vp, codes, bvh
+
+    def lbvh(self, codes, vp, lig):
+        n = codes.shape[0]
+        nodes = np.zeros(2 * n - 1, LBVH_DTYPE)
+        parents = np.zeros(2 * n - 1, np.uint32)
+        self.lib.o_build_lbvh(_p(np.ascontiguousarray(codes, np.uint32)), c_u32(n), _p(vp), _p(lig), _p(nodes),
+                              _p(parents))
+        return nodes, parents
 
     def tile_keys(self, W, H, cam, cf, depth, normals):
         Sx, Sy = (W + 31) // 32, (H + 31) // 32

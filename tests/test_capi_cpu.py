@@ -131,6 +131,12 @@ def test_primitive_argument_validation(lib, hctx):
     for nb in (0, 3, 65537, 1 << 17):
         assert lib.vren_bucket_sort(hctx, None, buf, 10, P(8192), buf, nb, 0) == EINVAL
     assert lib.vren_bucket_sort(hctx, None, buf, 10, buf, buf, 64, 0) == EINVAL  # in place
+    # build_lbvh: at least one leaf, at most 2^20, non-null buffers
+    assert lib.vren_build_lbvh(hctx, None, buf, 0, buf, buf, buf, buf) == EINVAL
+    assert lib.vren_build_lbvh(hctx, None, buf, (1 << 20) + 1, buf, buf, buf, buf) == EINVAL
+    assert lib.vren_build_lbvh(hctx, None, None, 10, buf, buf, buf, buf) == EINVAL
+    assert lib.vren_build_lbvh(hctx, None, buf, 10, buf, buf, buf, buf) == EINVAL
+    assert b"no device" in lib.vren_last_error(hctx)
     # build_bvh: leaf count must be a power of 32 and >= 32 (build_bvh.cpp:45)
     for n in (0, 1, 31, 33, 64, 1000):
         assert lib.vren_build_bvh(hctx, None, buf, n) == EINVAL

@@ -91,6 +91,22 @@ def test_light_bvh_sizes(vctx, co, L):
     assert np.array_equal(bvh.view(np.uint32).reshape(-1, 8)[:, :7], obvh.view(np.uint32).reshape(-1, 8)[:, :7]), L
 
 
+@pytest.mark.parametrize("L,bits", [(1, 5), (2, 5), (3, 5), (33, 5), (1000, 5), (4096, 5), (4096, 10),
+                                    (40000, 10)])
+def test_lbvh_matches_oracle(vctx, co, L, bits):
+    """Karras LBVH (configuration 2): topology, parents and refitted AABBs bit-exact against the oracle."""
+    cam = ol.camera(64, 64)
+    pos, lig = co.synth_lights(5000 + L, L)
+    cf = ol.cfg(morton_bits=bits)
+    vp, codes, _ = co.light_bvh(pos, lig, cam, cf)
+    m = 2 * L - 1
+    nodes, parents = empty_dev(32 * m, fill=0xCD), empty_dev(4 * m, fill=0xCD)
+    vctx.build_lbvh(to_dev(codes), L, to_dev(vp), to_dev(lig), nodes, parents)
+    onodes, oparents = co.lbvh(codes, vp, lig)
+    assert np.array_equal(from_dev(parents, np.uint32, (m,)), oparents)
+    assert np.array_equal(from_dev(nodes, np.uint32, (m, 8)), onodes.view(np.uint32).reshape(m, 8))
+
+
 def gpu_clusters(vctx, W, H, cam, cf, depth, normals, key_cap=1 << 20):
     keys = empty_dev(4 * key_cap)
     cnt = empty_dev(4)

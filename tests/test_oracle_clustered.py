@@ -328,3 +328,90 @@ def test_shade_backward_vs_central_differences(co):
             assert abs(gar[p, comp] - fd) <= 1e-4 * max(abs(fd), 1e-6), (p, comp, gar[p, comp], fd)
             checked += 1
     assert checked >= 100
+
+
+# ---- Karras LBVH (Karras 2012): radix-tree property, exact unions, queries vs brute force ------------------------
+
+def _lbvh_leaf_range(nodes, n, node, memo):
+    if node >= n - 1:
+        return node - (n - 1), node - (n - 1)
+    if node in memo:
+        return memo[node]
+    a = _lbvh_leaf_range(nodes, n, int(nodes["left"][node]), memo)
+    b = _lbvh_leaf_range(nodes, n, int(nodes["right"][node]), memo)
+    assert a[1] + 1 == b[0], "children must cover adjacent key ranges"
+    memo[node] = (a[0], b[1])
+    return memo[node]
+
+
+@pytest.mark.parametrize("L,bits", [(1, 5), (2, 5), (3, 5), (33, 5), (700, 5), (4096, 5), (4096, 10)])
+def test_lbvh_radix_tree_property(co, L, bits):
+    cam = ol.camera(64, 64)
+    pos, lig = co.synth_lights(9000 + L, L)
+    cf = ol.cfg(morton_bits=bits, morton_aabb_mode=1 if bits == 10 else 0,
+                aabb=((-50.0, -50.0, 0.0), (50.0, 50.0, 100.0)) if bits == 10 else None)
+    vp, codes, _ = co.light_bvh(pos, lig, cam, cf)
+    nodes, parents = co.lbvh(codes, vp, lig)
+    n = L
+    keys = (codes[:, 0].astype(np.uint64) << np.uint64(32)) | codes[:, 1].astype(np.uint64)
+    assert np.all(keys[1:] > keys[:-1])
+    # leaves
+    li = codes[:, 1]
+    assert np.all(nodes["left"][n - 1:] == 0xFFFFFFFF) and np.array_equal(nodes["right"][n - 1:], li)
+    assert np.array_equal(nodes["min"][n - 1:], vp[li, :3] - lig[li, 3:4])
+    assert np.array_equal(nodes["max"][n - 1:], vp[li, :3] + lig[li, 3:4])
+    if n == 1:
+        assert parents[0] == 0xFFFFFFFF
+        return
+    memo = {}
+    assert _lbvh_leaf_range(nodes, n, 0, memo) == (0, n - 1)   # the root covers every leaf
+    assert len(memo) == n - 1                                   # every internal node is reached once
+    reached = np.zeros(2 * n - 1, np.int64)
+    for i in range(n - 1):
+        l, r = int(nodes["left"][i]), int(nodes["right"][i])
+        reached[l] += 1
+        reached[r] += 1
+        assert parents[l] == i and parents[r] == i
+        a, b = memo[i]
+        split = _lbvh_leaf_range(nodes, n, l, memo)[1]
+        # the split is the first bit where the range's first and last keys differ: below it on the left,
+        # set on the right (the binary radix tree over the keys is unique)
+        x = int(keys[a]) ^ int(keys[b])
+        top = 63 - (x.bit_length() - 1)
+        bit = np.uint64(1) << np.uint64(63 - top)
+        assert np.all((keys[a:split + 1] & bit) == 0) and np.all((keys[split + 1:b + 1] & bit) != 0)
+        # exact union of the children
+        assert np.array_equal(nodes["min"][i], np.minimum(nodes["min"][l], nodes["min"][r]))
+        assert np.array_equal(nodes["max"][i], np.maximum(nodes["max"][l], nodes["max"][r]))
+    assert reached[0] == 0 and np.all(reached[1:] == 1) and parents[0] == 0xFFFFFFFF
+
+
+def test_lbvh_sphere_queries_match_brute_force(co):
+    L = 3000
+    cam = ol.camera(64, 64)
+    pos, lig = co.synth_lights(4242, L)
+    vp, codes, _ = co.light_bvh(pos, lig, cam, ol.cfg(morton_bits=10))
+    nodes, _ = co.lbvh(codes, vp, lig)
+    n = L
+    lo, hi = nodes["min"].astype(np.float64), nodes["max"].astype(np.float64)
+    rng = np.random.default_rng(8)
+    for _ in range(60):
+        c = rng.uniform(-50, 50, 3) + np.array([0.0, 0.0, 50.0])
+        rad = rng.uniform(0.5, 10.0)
+
+        def hit(node):
+            d = np.maximum(lo[node] - c, 0.0) + np.maximum(c - hi[node], 0.0)
+            return float(d @ d) <= rad * rad
+        found, stack = [], [0]
+        while stack:
+            node = stack.pop()
+            if not hit(node):
+                continue
+            if node >= n - 1:
+                found.append(int(nodes["right"][node]))
+            else:
+                stack += [int(nodes["left"][node]), int(nodes["right"][node])]
+        leaf_lo, leaf_hi = lo[n - 1:], hi[n - 1:]
+        d = np.maximum(leaf_lo - c, 0.0) + np.maximum(c - leaf_hi, 0.0)
+        brute = sorted(int(x) for x in nodes["right"][n - 1:][np.einsum("ij,ij->i", d, d) <= rad * rad])
+        assert sorted(found) == brute

@@ -122,6 +122,11 @@ class Context:
     def build_bvh(self, nodes, padded_leaf_count):
         return self.check(self.lib.vren_build_bvh(self.h, self.stream(), _ptr(nodes), padded_leaf_count))
 
+    def build_lbvh(self, codes, n, view_positions, point_lights, nodes, parents):
+        """Karras LBVH over n sorted (code, light index) pairs: nodes[2n-1] (32 B each), parents[2n-1]."""
+        return self.check(self.lib.vren_build_lbvh(self.h, self.stream(), _ptr(codes), n, _ptr(view_positions),
+                                                   _ptr(point_lights), _ptr(nodes), _ptr(parents)))
+
     # ---- clustered shading stages (one frame) --------------------------------------------------------------
     def camera_matrices(self, cam):
         import numpy as np

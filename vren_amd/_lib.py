@@ -77,6 +77,7 @@ PROTOTYPES = {
     "vren_calc_bvh_root_index": (c_uint32, [c_uint32]),
     "vren_calc_bvh_level_count": (c_uint32, [c_uint32]),
     "vren_build_bvh": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    "vren_build_lbvh": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "vren_camera_matrices": (None, [P(Camera), P(c_float), P(c_float)]),
     "vren_construct_point_light_bvh": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, P(Camera),
                                                P(ClusterCfg), c_void_p, c_void_p, c_void_p]),
