@@ -40,13 +40,25 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
     v[0] = v[0] * 2.0 - 1.0;
     v[1] = v[1] * 2.0 - 1.0;
     double r[4];
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
         r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) + fd.inv_proj_d[2 * 4 + i] * v[2]) +
                fd.inv_proj_d[3 * 4 + i] * v[3];
+    if (fd.proj_zw_xy_zero) {
+        // ((0 x + 0 y) + m2 depth) + m3 is m2 depth + m3 exactly (up to the sign of a zero)
+        r[2] = fd.inv_proj_d[2 * 4 + 2] * v[2] + fd.inv_proj_d[3 * 4 + 2];
+        r[3] = fd.inv_proj_d[2 * 4 + 3] * v[2] + fd.inv_proj_d[3 * 4 + 3];
+    } else {
+        for (int i = 2; i < 4; i++)
+            r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
+                    fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
+    }
     const double w = r[3];
     const double z = r[2] / w;
     o.shaded = z < (double)fd.bg_threshold;
-    const double iw = 1.0 / w;
+    // 1 / w for the x and y coordinates: hardware reciprocal and two Newton steps (~1 ulp, double)
+    double iw = __builtin_amdgcn_rcp(w);
+    iw = iw + iw * (1.0 - w * iw);
+    iw = iw + iw * (1.0 - w * iw);
     const double q[4] = {r[0] * iw, r[1] * iw, z, 1.0};
     for (int i = 0; i < 3; i++)
         o.P[i] = ((fd.inv_view_d[0 * 4 + i] * q[0] + fd.inv_view_d[1 * 4 + i] * q[1]) + fd.inv_view_d[2 * 4 + i] * q[2]) +
@@ -75,20 +87,22 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
     const float h[3] = {s.V[0] + mL[0], s.V[1] + mL[1], s.V[2] + mL[2]};
     const float hh = dot3(h, h);
     const float Nh = dot3(s.N, h);
+    const float irh = hh > 0.0f ? __builtin_amdgcn_rsqf(hh) : 0.0f; // 1 / |h|
     float inner = 1.0f;
     if (Nh > 0.0f && hh > 0.0f) {
         const float c[3] = {s.N[1] * h[2] - s.N[2] * h[1], s.N[2] * h[0] - s.N[0] * h[2], s.N[0] * h[1] - s.N[1] * h[0]};
-        const float inv_hh = frcp(hh);
+        const float inv_hh = irh * irh;
         inner = s.nn + dot3(c, c) * inv_hh + s.a2 * (Nh * Nh) * inv_hh;
     }
-    const float NDF = s.a2 * frcp(kPi * inner * inner);
     const float k = s.rough;
-    const float G = s.ggxV * (NdotL * frcp(NdotL * (1.0f - k) + k));
-    const float HdotV = hh > 0.0f ? fmaxf(dot3(h, s.V) * __builtin_amdgcn_rsqf(hh), 0.0f) : 0.0f;
+    const float HdotV = fmaxf(dot3(h, s.V) * irh, 0.0f);
     float om = 1.0f - HdotV;
     float om2 = om * om;
     const float f5 = om2 * om2 * om;
-    const float spec = (NDF * G) * frcp(4.0f * s.NdotV * NdotL + 0.0001f);
+    // spec = NDF G / (4 n.v n.l + 0.0001) with NDF = a^2 / (PI inner^2) and G = ggxV n.l / (n.l (1 - k) + k):
+    // the three quotients share one reciprocal
+    const float spec = (s.a2 * s.ggxV * NdotL) *
+                       frcp(kPi * inner * inner * (NdotL * (1.0f - k) + k) * (4.0f * s.NdotV * NdotL + 0.0001f));
     const float km = 1.0f - s.metal;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -173,8 +187,11 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
         for (uint32_t i = tid; i < nstage; i += 256) {
             const uint32_t li = fd.indices[start + i];
             if (li < fd.L) {
-                s_lpos[i] = lpos[li];
-                s_lcol[i] = lcol[li];
+                const float4 c = lcol[li];
+                float4 q = lpos[li];
+                q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
+                s_lpos[i] = q;
+                s_lcol[i] = c;
             } else {
                 s_lcol[i] = make_float4(0.f, 0.f, 0.f, 0.f); // radius 0: never contributes
                 s_lpos[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -217,6 +234,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
                 if (li >= fd.L) continue;
                 lp = lpos[li];
                 lc = lcol[li];
+                lp.w = frcp(lc.w);
             }
             // lights clearly beyond their radius contribute exactly 0: a float distance (error < 1e-3 at scene
             // scale) rejects them before the double evaluation
@@ -231,22 +249,25 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
             const float r = lc.w;
             const double rr = (double)r * (double)r;
             if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < rr)) continue;
-            const float s0 = __builtin_amdgcn_sqrtf((float)dist2);
+            const float d2f = (float)dist2;
+            const float inv = __builtin_amdgcn_rsqf(d2f); // 1 / |d|
+            const float s0 = d2f * inv;                     // |d|
             // 1 - x = (r - |d|) / r = (r^2 - |d|^2) / ((r + |d|) r): the difference that cancels near the radius is
             // taken in double, the rest needs only float accuracy
-            const float omx = (float)(rr - dist2) * frcp((r + s0) * r);
-            const float x = 1.0f - omx;
+            const float num = (float)(rr - dist2);
             float att;
             if (fd.attenuation == VREN_ATTEN_SMOOTH) {
                 // (1 - x^4)^2 = ((1 - x)(1 + x)(1 + x^2))^2
+                const float omx = num * frcp((r + s0) * r);
+                const float x = 1.0f - omx;
                 const float w = omx * (1.0f + x) * (1.0f + x * x);
                 att = w * w;
             } else {
-                // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5)
-                att = omx * (1.0f + x + x * x) * frcp(1.0f + x * __builtin_amdgcn_sqrtf(x));
+                // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5), with x = |d| / r (lp.w holds 1 / r)
+                const float x = s0 * lp.w;
+                att = num * (1.0f + x + x * x) * frcp((r + s0) * r * (1.0f + x * __builtin_amdgcn_sqrtf(x)));
             }
             if (!(att > 0.0f)) continue;
-            const float inv = frcp(s0);
             const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
             const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
             apply_light(s, L, rad, acc);
