@@ -138,7 +138,7 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     ms = sorted(times)[len(times) // 2]
     achieved = SORT_BYTES_PER_PAIR * n / (ms * 1e-3) / 1e9
     configs = ctx.radix_configs()
-    cfg = configs[cfg_id] if cfg_id is not None else (8, 15, 8)  # the library's untuned default
+    cfg = configs[cfg_id] if cfg_id is not None else (8, 23, 8)  # the library's untuned default
     traffic = profiled_traffic()
     one = traffic.get("rs_onesweep_kernel<%d, %d, %d, true>" % tuple(cfg))
     hist = traffic.get("rs_hist_kernel<%d>" % cfg[0])

@@ -112,95 +112,17 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
     }
 }
 
-constexpr uint32_t kStage = 512; // lights staged in LDS per 16x16 block
+constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup
+constexpr int kShadePPT = 1;     // pixels per thread of the forward shade (2 measured slower)
 
-// One 256-thread workgroup per 16x16 quarter of a 32x32 tile. The clusters of a tile are numbered consecutively
-// and their light lists are consecutive in the index buffer, so the block first reduces the range of clusters
-// its shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS,
-// and then shades each pixel from LDS instead of chasing index -> light through global memory per light.
-__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint32_t F)
+// Shades one pixel whose inputs are loaded: its list entries [off, end) come from the LDS stage (when `staged`,
+// entry e at e - start) or from global memory.
+__device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar, float metal,
+                                            bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
+                                            const float4* s_lpos, const float4* s_lcol, uint64_t p)
 {
-    __shared__ float4 s_lpos[kStage];
-    __shared__ float4 s_lcol[kStage];
-    __shared__ uint32_t s_min[4], s_max[4];
-    __shared__ uint32_t s_start, s_n, s_staged;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
-    const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
-    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
-    const FrameDev& fd = frames[f];
-    const uint32_t t = g - fd.tile_base;
-    const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-    const uint32_t gx = ti * 32 + (sub & 1) * 16 + (tid & 15), gy = tj * 32 + (sub >> 1) * 16 + (tid >> 4);
-    const bool in = gx < fd.W && gy < fd.H;
-    const uint64_t p = in ? (uint64_t)gy * fd.W + gx : 0;
-    PixelIn px;
-    px.shaded = false;
-    uint2 nb = make_uint2(0u, 0u);
-    float4 ar = make_float4(0.f, 0.f, 0.f, 0.f);
-    float metal = 0.0f;
-    uint32_t cl_raw = 0xFFFFFFFFu;
-    if (in) {
-        // every per-pixel load is issued before any of them is used
-        const float depth = fd.depth[p];
-        nb = reinterpret_cast<const uint2*>(fd.normals)[p];
-        ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
-        metal = fd.metallic[p];
-        cl_raw = fd.refs[p];
-        px = reconstruct(fd, gx, gy, depth);
-    }
-    const bool has_list = px.shaded && fd.L > 0 && cl_raw < fd.key_cap;
-    // this pixel's light list, then the block's range of list entries (the clusters of a tile are numbered
-    // consecutively and their lists are consecutive in the index buffer)
-    uint32_t my_off = 0, my_end = 0;
-    if (has_list) {
-        my_off = min(fd.offsets[cl_raw], fd.index_cap);
-        const uint64_t e = (uint64_t)fd.offsets[cl_raw] + fd.counts[cl_raw];
-        my_end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
-    }
-    uint32_t rmin = has_list ? my_off : 0xFFFFFFFFu, rmax = has_list ? my_end : 0u;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        rmin = min(rmin, shfl_xor(rmin, m));
-        rmax = max(rmax, shfl_xor(rmax, m));
-    }
-    if (lane == 0) {
-        s_min[wave] = rmin;
-        s_max[wave] = rmax;
-    }
-    __syncthreads();
-    {
-        const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
-        const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
-        const uint32_t n = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
-        if (tid == 0) {
-            s_start = a == 0xFFFFFFFFu ? 0u : a;
-            s_n = n;
-            s_staged = n <= kStage ? 1u : 0u;
-        }
-    }
-    __syncthreads();
-    const uint32_t start = s_start, nstage = s_n;
-    const bool staged = s_staged != 0;
     const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
     const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
-    if (staged) {
-        for (uint32_t i = tid; i < nstage; i += 256) {
-            const uint32_t li = fd.indices[start + i];
-            if (li < fd.L) {
-                const float4 c = lcol[li];
-                float4 q = lpos[li];
-                q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
-                s_lpos[i] = q;
-                s_lcol[i] = c;
-            } else {
-                s_lcol[i] = make_float4(0.f, 0.f, 0.f, 0.f); // radius 0: never contributes
-                s_lpos[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-    }
-    __syncthreads();
-    if (!px.shaded) return;
-
     Surface s;
     {
         const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
@@ -224,7 +146,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (has_list) {
         const float Pf[3] = {(float)px.P[0], (float)px.P[1], (float)px.P[2]};
-        for (uint32_t e = my_off; e < my_end; e++) {
+        for (uint32_t e = off; e < end; e++) {
             float4 lp, lc;
             if (staged) {
                 lp = s_lpos[e - start];
@@ -289,6 +211,118 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[2] + acc[2]) * gexp);
     o.w = 1.0f;
     reinterpret_cast<float4*>(fd.out)[p] = o;
+}
+
+// A 256-thread workgroup shades PPT 16x16 quadrants of a 32x32 tile (PPT pixels per thread, their loads issued
+// together), flattened over every frame of the table. The clusters of a tile are numbered consecutively and their
+// light lists are consecutive in the index buffer, so the workgroup first reduces the range of list entries its
+// shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS, and then
+// shades each pixel from LDS instead of chasing index -> light through global memory per light.
+template <int PPT>
+__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint32_t F)
+{
+    constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
+    __shared__ float4 s_lpos[kStage];
+    __shared__ float4 s_lcol[kStage];
+    __shared__ uint32_t s_min[4], s_max[4];
+    __shared__ uint32_t s_start, s_n, s_staged;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
+    const uint32_t g = blockIdx.x / BPT, sub = blockIdx.x % BPT;
+    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const FrameDev& fd = frames[f];
+    const uint32_t t = g - fd.tile_base;
+    const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
+    PixelIn px[PPT];
+    uint2 nb[PPT];
+    float4 ar[PPT];
+    float metal[PPT];
+    uint32_t cl_raw[PPT];
+    uint64_t p[PPT];
+    bool in[PPT];
+    // every per-pixel load is issued before any of them is used
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        const uint32_t quad = sub * PPT + q;
+        const uint32_t gx = ti * 32 + (quad & 1) * 16 + (tid & 15), gy = tj * 32 + (quad >> 1) * 16 + (tid >> 4);
+        in[q] = gx < fd.W && gy < fd.H;
+        p[q] = in[q] ? (uint64_t)gy * fd.W + gx : 0;
+        px[q].shaded = false;
+        nb[q] = make_uint2(0u, 0u);
+        ar[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        metal[q] = 0.0f;
+        cl_raw[q] = 0xFFFFFFFFu;
+        if (in[q]) {
+            const float depth = fd.depth[p[q]];
+            nb[q] = reinterpret_cast<const uint2*>(fd.normals)[p[q]];
+            ar[q] = reinterpret_cast<const float4*>(fd.albedo_rough)[p[q]];
+            metal[q] = fd.metallic[p[q]];
+            cl_raw[q] = fd.refs[p[q]];
+            px[q] = reconstruct(fd, gx, gy, depth);
+        }
+    }
+    // each pixel's light list, then the workgroup's range of list entries
+    bool has_list[PPT];
+    uint32_t my_off[PPT], my_end[PPT];
+    uint32_t rmin = 0xFFFFFFFFu, rmax = 0u;
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        has_list[q] = px[q].shaded && fd.L > 0 && cl_raw[q] < fd.key_cap;
+        my_off[q] = 0;
+        my_end[q] = 0;
+        if (has_list[q]) {
+            my_off[q] = min(fd.offsets[cl_raw[q]], fd.index_cap);
+            const uint64_t e = (uint64_t)fd.offsets[cl_raw[q]] + fd.counts[cl_raw[q]];
+            my_end[q] = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+            rmin = min(rmin, my_off[q]);
+            rmax = max(rmax, my_end[q]);
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        rmin = min(rmin, shfl_xor(rmin, m));
+        rmax = max(rmax, shfl_xor(rmax, m));
+    }
+    if (lane == 0) {
+        s_min[wave] = rmin;
+        s_max[wave] = rmax;
+    }
+    __syncthreads();
+    {
+        const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+        const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+        const uint32_t n = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
+        if (tid == 0) {
+            s_start = a == 0xFFFFFFFFu ? 0u : a;
+            s_n = n;
+            s_staged = n <= kStage ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t start = s_start, nstage = s_n;
+    const bool staged = s_staged != 0;
+    if (staged) {
+        const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
+        const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
+        for (uint32_t i = tid; i < nstage; i += 256) {
+            const uint32_t li = fd.indices[start + i];
+            if (li < fd.L) {
+                const float4 c = lcol[li];
+                float4 q = lpos[li];
+                q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
+                s_lpos[i] = q;
+                s_lcol[i] = c;
+            } else {
+                s_lcol[i] = make_float4(0.f, 0.f, 0.f, 0.f); // radius 0: never contributes
+                s_lpos[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PPT; q++)
+        if (px[q].shaded)
+            shade_pixel(fd, px[q], nb[q], ar[q], metal[q], has_list[q], my_off[q], my_end[q], staged, start, s_lpos,
+                        s_lcol, p[q]);
 }
 
 
@@ -596,7 +630,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* fra
 
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    shade_kernel<<<w.total_tiles * 4, 256, 0, s>>>(d_frames, F);
+    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
