@@ -26,6 +26,7 @@ struct vren_batch {
     ChainWork w;                    // batch-wide
     std::vector<ChainWork> w_frame; // per-frame views of the same memory
     std::vector<uint32_t> leaf_slots;
+    uint32_t* d_tile_frame = nullptr; // frame of every flattened tile
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;
@@ -144,6 +145,13 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     if (e == hipSuccess) e = hipMalloc(&b->work, wbytes);
     if (e == hipSuccess) e = hipMalloc(&b->d_table, sizeof(FrameDev) * n_frames);
     if (e == hipSuccess) e = hipMalloc(&b->d_single, sizeof(FrameDev) * n_frames);
+    if (e == hipSuccess) e = hipMalloc(&b->d_tile_frame, sizeof(uint32_t) * std::max<uint32_t>(tile_base, 1));
+    if (e == hipSuccess) {
+        std::vector<uint32_t> tf(tile_base);
+        for (uint32_t i = 0; i < n_frames; i++)
+            for (uint32_t t = 0; t < b->host[i].Sx * b->host[i].Sy; t++) tf[b->host[i].tile_base + t] = i;
+        e = hipMemcpy(b->d_tile_frame, tf.data(), sizeof(uint32_t) * tile_base, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         vren_batch_destroy(b);
         return vren_hip_fail(ctx, e, "batch allocation");
@@ -167,6 +175,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.list_slots = f.slot_cap ? reinterpret_cast<uint32_t*>(b->arena + o[12]) : nullptr;
     }
     chain_work_bind(&b->w, b->work, tile_base, offs_base);
+    b->w.tile_frame = b->d_tile_frame;
     b->w.total_leaf_slots = leaf_base;
     b->w.max_L = max_L;
     // per-frame tables: same buffers, bases 0, the workspace viewed from its start
@@ -278,6 +287,7 @@ VREN_EXPORT void vren_batch_destroy(vren_batch* b)
     if (b->work) hipFree(b->work);
     if (b->d_table) hipFree(b->d_table);
     if (b->d_single) hipFree(b->d_single);
+    if (b->d_tile_frame) hipFree(b->d_tile_frame);
     delete b;
 }
 

@@ -17,6 +17,7 @@ struct ChainWork {
     uint64_t* tile_status; // cluster compaction: one word per 32x32 tile of every frame
     uint64_t* offs_status; // light-offset scan: one word per 4096 clusters of capacity of every frame
     uint32_t* tickets;     // [0] cluster tiles, [1] offset tiles
+    const uint32_t* tile_frame; // frame of every flattened 32x32 tile (null for a one-frame table)
     uint32_t total_tiles;
     uint32_t total_offs_tiles;
     uint32_t total_leaf_slots;
@@ -48,6 +49,12 @@ int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t 
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
+
+// device: frame of a flattened 32x32 tile: one load from the tile -> frame table (uniform across the workgroup)
+__device__ __forceinline__ uint32_t frame_of_tile(const uint32_t* tile_frame, uint32_t g)
+{
+    return tile_frame ? tile_frame[g] : 0u;
+}
 
 // device: frame of a flattened index, given a per-frame base field (bases ascending)
 template <typename Get>

@@ -149,7 +149,8 @@ __device__ __forceinline__ void load_quad(const FrameDev& fd, uint32_t x0, uint3
 // wait ends. (Giving a wave several tiles one after another would make the next workgroup's first tile wait for
 // this workgroup's last step, and serialise the grid.) Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8),
 // q, j = 0..3, of its tile.
-__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* frames, uint32_t F,
+__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* frames,
+                                                                       const uint32_t* tile_frame,
                                                                        uint32_t total_tiles, uint64_t* tile_status,
                                                                        uint32_t* ticket)
 {
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
     {
         const uint32_t g = s_t * kClusterWaves + wave;
         if (g >= total_tiles) return;
-        const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+        const uint32_t f = frame_of_tile(tile_frame, g);
         const FrameDev& fd = frames[f];
         const uint32_t t = g - fd.tile_base;
         const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
@@ -257,7 +258,8 @@ int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w)
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
     const uint32_t blocks = (w.total_tiles + kClusterWaves - 1) / kClusterWaves;
-    clusters_kernel<<<blocks, kClusterWaves * 64, 0, s>>>(d_frames, F, w.total_tiles, w.tile_status, w.tickets + 0);
+    clusters_kernel<<<blocks, kClusterWaves * 64, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.total_tiles,
+                                                          w.tile_status, w.tickets + 0);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

@@ -219,7 +219,7 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
 // shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS, and then
 // shades each pixel from LDS instead of chasing index -> light through global memory per light.
 template <int PPT>
-__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint32_t F)
+__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, const uint32_t* tile_frame)
 {
     constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
     __shared__ float4 s_lpos[kStage];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, uint
     __shared__ uint32_t s_start, s_n, s_staged;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / BPT, sub = blockIdx.x % BPT;
-    const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
+    const uint32_t f = frame_of_tile(tile_frame, g);
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* fra
 
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F);
+    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
