@@ -138,11 +138,20 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     ms = sorted(times)[len(times) // 2]
     achieved = SORT_BYTES_PER_PAIR * n / (ms * 1e-3) / 1e9
     configs = ctx.radix_configs()
-    cfg = configs[cfg_id] if cfg_id is not None else (8, 23, 8)  # the library's untuned default
+    cfg = configs[cfg_id] if cfg_id is not None else (8, 23, 8, 0)  # the library's untuned default
+    bits, kpt, waves, algo = cfg
+    passes = (32 + bits - 1) // bits
     traffic = profiled_traffic()
-    one = traffic.get("rs_onesweep_kernel<%d, %d, %d, true>" % tuple(cfg))
-    hist = traffic.get("rs_hist_kernel<%d>" % cfg[0])
-    sort_traffic = hist + one * ((32 + cfg[0] - 1) // cfg[0]) if (one and hist and log2n == 26) else None
+    sort_traffic = None
+    if log2n == 26:
+        if algo == 0:  # histogram of every pass + onesweep passes
+            parts = [traffic.get("rs_hist_kernel<%d>" % bits)] + \
+                    [traffic.get("rs_onesweep_kernel<%d, %d, %d, true, false>" % (bits, kpt, waves))] * passes
+        else:          # per pass: tile counts, scan, scatter
+            parts = [traffic.get("rs_upsweep_kernel<%d, %d, %d>" % (bits, kpt, waves)), traffic.get("scan_kernel"),
+                     traffic.get("rs_onesweep_kernel<%d, %d, %d, true, true>" % (bits, kpt, waves))] * passes
+        if all(x is not None for x in parts):
+            sort_traffic = sum(parts)
     return {
         "n": n, "pairs": True, "ms": round(ms, 4), "gpairs_per_s": round(n / (ms * 1e-3) / 1e9, 3),
         "config": list(cfg),

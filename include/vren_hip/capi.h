@@ -135,12 +135,13 @@ VREN_EXPORT int vren_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint32_t* k
                                 uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes);
 
 /* Radix-sort autotuner. No reference counterpart (the reference's digit width is fixed at 4 bits,
- * radix_sort.hpp:15-16). Configurations are (digit bits, keys per thread, waves per workgroup); every
- * configuration gives the same (stable) output. vren_autotune_radix_sort times them all on pseudo-random keys of
- * n elements and remembers the fastest for that size class in the context. */
+ * radix_sort.hpp:15-16). Configurations are (digit bits, keys per thread, waves per workgroup, algorithm: 0 onesweep
+ * with a per-digit decoupled look-back, 1 reduce-then-scan: tile counts, one scan, scatter); every configuration
+ * gives the same (stable) output. vren_autotune_radix_sort times them all on pseudo-random keys of n elements and
+ * remembers the fastest for that size class in the context. Out-pointers may be NULL. */
 VREN_EXPORT int vren_radix_sort_config_count(void);
 VREN_EXPORT int vren_radix_sort_config_info(int config, uint32_t* radix_bits, uint32_t* keys_per_thread,
-                                            uint32_t* waves_per_block);
+                                            uint32_t* waves_per_block, uint32_t* algorithm);
 VREN_EXPORT int vren_radix_sort_force_config(vren_ctx* ctx, int config); /* -1 = automatic */
 VREN_EXPORT int vren_autotune_radix_sort(vren_ctx* ctx, vren_stream_t stream, uint64_t n, int with_values,
                                          uint32_t key_bits, int* best_config, float* best_ms);

@@ -110,9 +110,9 @@ class Context:
         L = lib()
         out = []
         for i in range(L.vren_radix_sort_config_count()):
-            b, k, w = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-            L.vren_radix_sort_config_info(i, ctypes.byref(b), ctypes.byref(k), ctypes.byref(w))
-            out.append((b.value, k.value, w.value))
+            b, k, w, a = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            L.vren_radix_sort_config_info(i, ctypes.byref(b), ctypes.byref(k), ctypes.byref(w), ctypes.byref(a))
+            out.append((b.value, k.value, w.value, a.value))
         return out
 
     def bucket_sort(self, in_pairs, n, out_pairs, offsets, n_buckets=65536, key_shift=0):

@@ -65,7 +65,7 @@ PROTOTYPES = {
     "vren_radix_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p,
                                 c_size_t]),
     "vren_radix_sort_config_count": (c_int, []),
-    "vren_radix_sort_config_info": (c_int, [c_int, P(c_uint32), P(c_uint32), P(c_uint32)]),
+    "vren_radix_sort_config_info": (c_int, [c_int, P(c_uint32), P(c_uint32), P(c_uint32), P(c_uint32)]),
     "vren_radix_sort_force_config": (c_int, [c_void_p, c_int]),
     "vren_autotune_radix_sort": (c_int, [c_void_p, c_void_p, c_uint64, c_int, c_uint32, P(c_int), P(c_float)]),
     "vren_autotune_save": (c_int, [c_void_p, c_char_p]),

@@ -15,7 +15,7 @@
 namespace {
 
 // defaults when nothing is tuned or forced: 8-bit digits, 23 keys per thread, 8 waves (the fastest on MI355X at 2^26)
-constexpr int kDefaultBits = 8, kDefaultKpt = 23, kDefaultWaves = 8;
+constexpr int kDefaultBits = 8, kDefaultKpt = 23, kDefaultWaves = 8, kDefaultAlgorithm = 0;
 
 uint64_t bucket_key(uint64_t n, bool vals, uint32_t key_bits)
 {
@@ -25,9 +25,9 @@ uint64_t bucket_key(uint64_t n, bool vals, uint32_t key_bits)
 int default_config()
 {
     for (int i = 0; i < vren_radix_config_count(); i++) {
-        int b, k, w;
-        vren_radix_config_info(i, &b, &k, &w);
-        if (b == kDefaultBits && k == kDefaultKpt && w == kDefaultWaves) return i;
+        int b, k, w, a;
+        vren_radix_config_info(i, &b, &k, &w, &a);
+        if (b == kDefaultBits && k == kDefaultKpt && w == kDefaultWaves && a == kDefaultAlgorithm) return i;
     }
     return 0;
 }
@@ -61,14 +61,15 @@ extern "C" {
 VREN_EXPORT int vren_radix_sort_config_count(void) { return vren_radix_config_count(); }
 
 VREN_EXPORT int vren_radix_sort_config_info(int config, uint32_t* radix_bits, uint32_t* keys_per_thread,
-                                            uint32_t* waves_per_block)
+                                            uint32_t* waves_per_block, uint32_t* algorithm)
 {
     if (config < 0 || config >= vren_radix_config_count()) return VREN_EINVAL;
-    int b, k, w;
-    vren_radix_config_info(config, &b, &k, &w);
+    int b, k, w, a;
+    vren_radix_config_info(config, &b, &k, &w, &a);
     if (radix_bits) *radix_bits = (uint32_t)b;
     if (keys_per_thread) *keys_per_thread = (uint32_t)k;
     if (waves_per_block) *waves_per_block = (uint32_t)w;
+    if (algorithm) *algorithm = (uint32_t)a;
     return VREN_OK;
 }
 
@@ -139,9 +140,9 @@ VREN_EXPORT int vren_autotune_save(vren_ctx* ctx, const char* path)
     FILE* f = fopen(path, "w");
     if (!f) return vren_fail(ctx, VREN_EINVAL, "autotune_save: cannot open %s", path);
     for (auto& kv : ctx->radix_tuned) {
-        int b, k, w;
-        vren_radix_config_info(kv.second, &b, &k, &w);
-        fprintf(f, "radix %llu %d %d %d\n", (unsigned long long)kv.first, b, k, w);
+        int b, k, w, a;
+        vren_radix_config_info(kv.second, &b, &k, &w, &a);
+        fprintf(f, "radix %llu %d %d %d %d\n", (unsigned long long)kv.first, b, k, w, a);
     }
     fclose(f);
     return VREN_OK;
@@ -153,12 +154,12 @@ VREN_EXPORT int vren_autotune_load(vren_ctx* ctx, const char* path)
     FILE* f = fopen(path, "r");
     if (!f) return vren_fail(ctx, VREN_EINVAL, "autotune_load: cannot open %s", path);
     unsigned long long key;
-    int b, k, w;
-    while (fscanf(f, " radix %llu %d %d %d", &key, &b, &k, &w) == 4) {
+    int b, k, w, a;
+    while (fscanf(f, " radix %llu %d %d %d %d", &key, &b, &k, &w, &a) == 5) {
         for (int i = 0; i < vren_radix_config_count(); i++) {
-            int b2, k2, w2;
-            vren_radix_config_info(i, &b2, &k2, &w2);
-            if (b == b2 && k == k2 && w == w2) ctx->radix_tuned[(uint64_t)key] = i;
+            int b2, k2, w2, a2;
+            vren_radix_config_info(i, &b2, &k2, &w2, &a2);
+            if (b == b2 && k == k2 && w == w2 && a == a2) ctx->radix_tuned[(uint64_t)key] = i;
         }
     }
     fclose(f);

@@ -98,6 +98,49 @@ __global__ __launch_bounds__(256) void rs_hexscan_kernel(uint32_t* hist, uint32_
     }
 }
 
+// Reduce-then-scan form, pass part 1: the digit counts of every tile (tiles as in the scatter pass), written
+// digit-major (count of digit d in tile t at d * tiles + t) so that one exclusive scan of the whole array gives
+// every (digit, tile) its first output position. Per-wave LDS counters keep the atomics uncontended.
+template <int BITS, int KPT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void rs_upsweep_kernel(const uint32_t* __restrict__ keys, uint64_t n,
+                                                                uint32_t shift, uint32_t width,
+                                                                uint32_t* __restrict__ tile_hist, uint32_t tiles)
+{
+    constexpr int THREADS = WAVES * 64;
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = THREADS * KPT;
+    __shared__ uint32_t s_h[WAVES * RADIX];
+    const uint32_t tid = threadIdx.x, wave = tid / VREN_WAVE;
+    for (uint32_t i = tid; i < (uint32_t)(WAVES * RADIX); i += THREADS) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    const uint64_t tile_start = (uint64_t)tile * TILE;
+    const uint32_t tile_count = (uint32_t)min<uint64_t>((uint64_t)TILE, n - tile_start);
+    const uint32_t mask = (1u << width) - 1u;
+    uint32_t* h = s_h + wave * RADIX;
+    const uint32_t* src = keys + tile_start;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const uint32_t n4 = tile_count / 4;
+        for (uint32_t i = tid; i < n4; i += THREADS) {
+            const uint4 q = ntload4(reinterpret_cast<const uint4*>(src) + i);
+            atomicAdd(&h[(q.x >> shift) & mask], 1u);
+            atomicAdd(&h[(q.y >> shift) & mask], 1u);
+            atomicAdd(&h[(q.z >> shift) & mask], 1u);
+            atomicAdd(&h[(q.w >> shift) & mask], 1u);
+        }
+        for (uint32_t i = n4 * 4 + tid; i < tile_count; i += THREADS) atomicAdd(&h[(src[i] >> shift) & mask], 1u);
+    } else {
+        for (uint32_t i = tid; i < tile_count; i += THREADS) atomicAdd(&h[(src[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = tid; d < (uint32_t)RADIX; d += THREADS) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) c += s_h[w * RADIX + d];
+        tile_hist[(uint64_t)d * tiles + tile] = c;
+    }
+}
+
 // keys and values are staged through LDS one after the other when staging both at once would leave room for
 // fewer than two workgroups per CU
 template <int KPT, int WAVES, bool VALS>
@@ -106,7 +149,10 @@ constexpr bool rs_sequential_staging()
     return VALS && 2 * WAVES * 64 * KPT * 4 > 80 * 1024;
 }
 
-template <int BITS, int KPT, int WAVES, bool VALS>
+// RTS = false: onesweep (tiles from a ticket counter, per-digit decoupled look-back, digit_base = the digit's
+// first output position). RTS = true: reduce-then-scan scatter pass (tile = workgroup id; digit_base = the scanned
+// digit-major tile counts of rs_upsweep_kernel), no tickets and no waiting.
+template <int BITS, int KPT, int WAVES, bool VALS, bool RTS = false>
 __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                                  const uint32_t* __restrict__ vals_in,
                                                                  uint32_t* __restrict__ keys_out,
@@ -132,13 +178,13 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id();
     const uint32_t wave = tid / VREN_WAVE;
-    if (tid == 0) s_ticket = atomic_add_agent(counter, 1u);
+    if (!RTS && tid == 0) s_ticket = atomic_add_agent(counter, 1u);
     for (uint32_t i = tid; i < (uint32_t)(WAVES * RADIX); i += THREADS) s_whist[i] = 0u;
     __syncthreads();
-    const uint32_t tile = s_ticket;
+    const uint32_t tile = RTS ? blockIdx.x : s_ticket;
     const uint64_t tile_start = (uint64_t)tile * TILE;
     const uint32_t tile_count = (uint32_t)min<uint64_t>((uint64_t)TILE, n - tile_start);
-    if (clear_next) {
+    if (!RTS && clear_next) {
         for (uint32_t i = tid; i < (uint32_t)RADIX; i += THREADS) clear_next[(uint64_t)tile * RADIX + i] = 0u;
     }
     const uint32_t mask = (1u << width) - 1u;
@@ -175,7 +221,13 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     uint32_t total;
     uint32_t dstart = block_exclusive_sum<THREADS>(dcount, s_scan, &total);
     uint32_t prefix = 0;
-    if (tid < (uint32_t)RADIX) {
+    if (RTS) {
+        if (tid < (uint32_t)RADIX) {
+            const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
+            s_start[tid] = dstart;
+            s_goff[tid] = digit_base[(uint64_t)tid * tiles + tile] - dstart;
+        }
+    } else if (tid < (uint32_t)RADIX) {
         s_start[tid] = dstart;
         // decoupled look-back on this digit's count
         uint32_t* my = status + (uint64_t)tile * RADIX + tid;
@@ -199,7 +251,7 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
         }
     }
     // global digit base (exclusive scan of this pass's histogram, rs_hexscan_kernel)
-    if (tid < (uint32_t)RADIX) s_goff[tid] = digit_base[tid] + prefix - dstart;
+    if (!RTS && tid < (uint32_t)RADIX) s_goff[tid] = digit_base[tid] + prefix - dstart;
     __syncthreads();
     // scatter into LDS in tile-local sorted order
 #pragma unroll
@@ -284,6 +336,46 @@ Layout layout_for(uint64_t n, bool vals, uint32_t min_tile, uint32_t max_radix)
 // smallest tile among the instantiated configurations (4 waves x 8 keys per thread)
 constexpr uint32_t kMinTile = 4 * 64 * 8;
 
+// reduce-then-scan LSD sort: per pass, tile digit counts -> one exclusive scan -> scatter (the tile-count array
+// and the scan workspace reuse the onesweep status regions of the scratch)
+template <int BITS, int KPT, int WAVES>
+int run_sort_rts(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t begin_bit,
+                 uint32_t end_bit, char* scratch, const Layout& L)
+{
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = WAVES * 64 * KPT;
+    const uint32_t passes = (end_bit - begin_bit + BITS - 1) / BITS;
+    const uint32_t tiles = (uint32_t)ceil_div_u64(n, TILE);
+    uint32_t* tile_hist = reinterpret_cast<uint32_t*>(scratch + L.status_a);
+    void* scan_work = scratch + L.status_b;
+    uint32_t* tk = reinterpret_cast<uint32_t*>(scratch + L.tmp_keys);
+    uint32_t* tv = vals ? reinterpret_cast<uint32_t*>(scratch + L.tmp_vals) : nullptr;
+    uint32_t *ksrc = keys, *vsrc = vals, *kdst = tk, *vdst = tv;
+    for (uint32_t p = 0; p < passes; p++) {
+        uint32_t shift = begin_bit + p * BITS;
+        uint32_t width = std::min<uint32_t>(BITS, end_bit - shift);
+        rs_upsweep_kernel<BITS, KPT, WAVES><<<tiles, WAVES * 64, 0, s>>>(ksrc, n, shift, width, tile_hist, tiles);
+        VREN_LAUNCH_CHECK(ctx);
+        int st = vren_scan_with_work(ctx, s, tile_hist, tile_hist, (uint64_t)RADIX * tiles, 1, scan_work);
+        if (st) return st;
+        if (vals) {
+            rs_onesweep_kernel<BITS, KPT, WAVES, true, true><<<tiles, WAVES * 64, 0, s>>>(
+                ksrc, vsrc, kdst, vdst, n, shift, width, tile_hist, nullptr, nullptr, nullptr);
+        } else {
+            rs_onesweep_kernel<BITS, KPT, WAVES, false, true><<<tiles, WAVES * 64, 0, s>>>(
+                ksrc, nullptr, kdst, nullptr, n, shift, width, tile_hist, nullptr, nullptr, nullptr);
+        }
+        VREN_LAUNCH_CHECK(ctx);
+        std::swap(ksrc, kdst);
+        std::swap(vsrc, vdst);
+    }
+    if (ksrc != keys) {
+        VREN_HIP(ctx, hipMemcpyAsync(keys, ksrc, n * 4, hipMemcpyDeviceToDevice, s));
+        if (vals) VREN_HIP(ctx, hipMemcpyAsync(vals, vsrc, n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    return VREN_OK;
+}
+
 template <int BITS, int KPT, int WAVES>
 int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t begin_bit,
              uint32_t end_bit, char* scratch, const Layout& L)
@@ -334,26 +426,30 @@ typedef int (*sort_fn)(vren_ctx*, hipStream_t, uint32_t*, uint32_t*, uint64_t, u
 struct SortConfig {
     int bits, kpt, waves;
     sort_fn fn;
+    int algorithm; // 0 onesweep with look-back, 1 reduce-then-scan
 };
 
-#define RS_CFG(B, K, W) {B, K, W, &run_sort<B, K, W>}
+#define RS_CFG(B, K, W) {B, K, W, &run_sort<B, K, W>, 0}
+#define RS_CFG_RTS(B, K, W) {B, K, W, &run_sort_rts<B, K, W>, 1}
 const SortConfig kConfigs[] = {
     RS_CFG(8, 8, 4),  RS_CFG(8, 15, 4), RS_CFG(8, 23, 4), RS_CFG(8, 8, 8),  RS_CFG(8, 15, 8),
     RS_CFG(8, 23, 8), RS_CFG(8, 8, 16), RS_CFG(8, 15, 16), RS_CFG(8, 23, 16), RS_CFG(6, 8, 4),  RS_CFG(6, 15, 4), RS_CFG(6, 23, 4),
     RS_CFG(6, 8, 8),  RS_CFG(6, 15, 8), RS_CFG(6, 23, 8), RS_CFG(6, 8, 16), RS_CFG(4, 8, 4),
     RS_CFG(4, 15, 4), RS_CFG(4, 23, 4), RS_CFG(4, 8, 8),  RS_CFG(4, 15, 8), RS_CFG(4, 23, 8),
     RS_CFG(4, 8, 16),
+    RS_CFG_RTS(8, 15, 8), RS_CFG_RTS(8, 23, 8), RS_CFG_RTS(8, 8, 16), RS_CFG_RTS(8, 15, 16), RS_CFG_RTS(8, 23, 4),
 };
 
 } // namespace
 
 int vren_radix_config_count() { return (int)(sizeof(kConfigs) / sizeof(kConfigs[0])); }
 
-void vren_radix_config_info(int i, int* bits, int* kpt, int* waves)
+void vren_radix_config_info(int i, int* bits, int* kpt, int* waves, int* algorithm)
 {
     *bits = kConfigs[i].bits;
     *kpt = kConfigs[i].kpt;
     *waves = kConfigs[i].waves;
+    if (algorithm) *algorithm = kConfigs[i].algorithm;
 }
 
 int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t* keys, uint32_t* vals, uint64_t n,

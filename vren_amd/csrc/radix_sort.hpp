@@ -4,7 +4,7 @@
 #include "common.hpp"
 
 int vren_radix_config_count();
-void vren_radix_config_info(int i, int* bits, int* kpt, int* waves);
+void vren_radix_config_info(int i, int* bits, int* kpt, int* waves, int* algorithm);
 int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t* keys, uint32_t* vals, uint64_t n,
                                 uint32_t begin_bit, uint32_t end_bit, void* scratch, size_t scratch_bytes);
 // configuration for this problem: forced, tuned (cached), or the default
