@@ -10,35 +10,66 @@
 
 #include "common.hpp"
 
-template <int BITS>
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid)
+// One digit bit of the wave match: bc = the lane's bit b broadcast to a word (v_bfe_i32), VCC = its ballot, and
+// per 32-lane half peers &= ~(ballot ^ bc) in one v_bitop3_b32 (truth table over S0 = 0xF0, S1 = 0xCC,
+// S2 = 0xAA: 0xF0 & ~(0xCC ^ 0xAA) = 0x90). A VALU write of VCC needs two wait states before a VALU reads it as
+// an operand (s_nop 1).
+template <int B>
+__device__ __forceinline__ void match_bit(uint32_t d, uint32_t& lo, uint32_t& hi)
 {
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < BITS; b++) {
-        bool bit = (d >> b) & 1u;
-        uint64_t m = ballot(bit);
-        peers &= bit ? m : ~m;
+    uint32_t bc;
+    asm("v_bfe_i32 %2, %3, %4, 1\n\t"
+        "v_cmp_ne_u32_e32 vcc, 0, %2\n\t"
+        "s_nop 1\n\t"
+        "v_bitop3_b32 %0, %0, vcc_lo, %2 bitop3:0x90\n\t"
+        "v_bitop3_b32 %1, %1, vcc_hi, %2 bitop3:0x90"
+        : "+v"(lo), "+v"(hi), "=&v"(bc)
+        : "v"(d), "i"(B)
+        : "vcc");
+}
+
+template <int B, int BITS>
+__device__ __forceinline__ void match_bits(uint32_t d, uint32_t& lo, uint32_t& hi)
+{
+    if constexpr (B < BITS) {
+        match_bit<B>(d, lo, hi);
+        match_bits<B + 1, BITS>(d, lo, hi);
     }
-    return peers;
+}
+
+// Lanes of the wave holding the same BITS-bit digit as this lane (and set in lo:hi on entry), four VALU
+// operations per digit bit.
+template <int BITS>
+__device__ __forceinline__ void match_digit(uint32_t d, uint32_t& lo, uint32_t& hi)
+{
+    match_bits<0, BITS>(d, lo, hi);
 }
 
 // s_whist: WAVES * RADIX counters, zeroed before the call (and a __syncthreads() after zeroing).
 // Writes local[j] = rank of key j among the same-digit keys of this wave that precede it.
-template <int BITS, int KPT>
+// CHECK = true: positions at or past tile_count are excluded (neither ranked nor counted). CHECK = false: every
+// lane takes part; a caller padding a partial tile gives its padding the largest digit, so that the padding ranks
+// after every real key of the tile and the real keys keep positions [0, tile_count).
+template <int BITS, int KPT, bool CHECK = true>
 __device__ __forceinline__ void rank_wave_keys(const uint32_t (&digit)[KPT], uint32_t valid_count_base,
                                                uint32_t tile_count, uint32_t* s_wave_row, uint32_t (&local)[KPT])
 {
     const uint32_t lane = lane_id();
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
-        uint32_t pos = valid_count_base + (uint32_t)j * VREN_WAVE + lane;
-        bool valid = pos < tile_count;
-        uint32_t d = digit[j];
-        uint64_t peers = match_digit<BITS>(d, valid);
-        uint32_t before = 0;
-        if (valid) before = s_wave_row[d];
-        uint32_t rank = mbcnt(peers);
+        const uint32_t d = digit[j];
+        bool valid = true;
+        uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
+        if (CHECK) {
+            valid = valid_count_base + (uint32_t)j * VREN_WAVE + lane < tile_count;
+            const uint64_t v = ballot(valid);
+            lo = (uint32_t)v;
+            hi = (uint32_t)(v >> 32);
+        }
+        const uint32_t before = valid ? s_wave_row[d] : 0u;
+        match_digit<BITS>(d, lo, hi);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        const uint64_t peers = ((uint64_t)hi << 32) | lo;
         // the highest peer lane publishes the new running count
         if (valid && (peers >> lane) == 1ull) s_wave_row[d] = before + rank + 1u;
         local[j] = before + rank;

@@ -170,7 +170,6 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[(VALS && !SEQ) ? TILE : 1];
     __shared__ uint32_t s_whist[WAVES * RADIX];
-    __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_goff[RADIX];
     __shared__ uint32_t s_scan[THREADS / VREN_WAVE];
     __shared__ uint32_t s_ticket;
@@ -187,54 +186,66 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     if (!RTS && clear_next) {
         for (uint32_t i = tid; i < (uint32_t)RADIX; i += THREADS) clear_next[(uint64_t)tile * RADIX + i] = 0u;
     }
-    const uint32_t mask = (1u << width) - 1u;
 
-    // warp-striped load
+    // warp-striped load; positions past the end of a partial tile hold 0xFFFFFFFF, whose digit is the largest, so
+    // they rank after every real key of the tile and the real keys keep the staged positions [0, tile_count)
     const uint32_t wbase = wave * KPT * VREN_WAVE;
+    const uint32_t* kin = keys_in + tile_start + wbase + lane;
+    const uint32_t* vin = vals_in + tile_start + wbase + lane;
     uint32_t key[KPT], val[VALS ? KPT : 1], digit[KPT], local[KPT];
+    if (tile_count == (uint32_t)TILE) {
 #pragma unroll
-    for (int j = 0; j < KPT; j++) {
-        uint32_t pos = wbase + j * VREN_WAVE + lane;
-        if (pos < tile_count) {
-            key[j] = __builtin_nontemporal_load(keys_in + tile_start + pos);
-            if (VALS) val[j] = __builtin_nontemporal_load(vals_in + tile_start + pos);
-        } else {
-            key[j] = 0xFFFFFFFFu;
-            if (VALS) val[j] = 0u;
-        }
-        digit[j] = (key[j] >> shift) & mask;
+        for (int j = 0; j < KPT; j++) key[j] = __builtin_nontemporal_load(kin + j * VREN_WAVE);
+    } else {
+#pragma unroll
+        for (int j = 0; j < KPT; j++)
+            key[j] = wbase + j * VREN_WAVE + lane < tile_count ? __builtin_nontemporal_load(kin + j * VREN_WAVE)
+                                                               : 0xFFFFFFFFu;
     }
-    rank_wave_keys<BITS, KPT>(digit, wbase, tile_count, &s_whist[wave * RADIX], local);
+#pragma unroll
+    for (int j = 0; j < KPT; j++) digit[j] = __builtin_amdgcn_ubfe(key[j], shift, width);
+    rank_wave_keys<BITS, KPT, false>(digit, wbase, tile_count, &s_whist[wave * RADIX], local);
+    // the values are read after the ranking: their loads overlap the counts, the look-back and the key stores
+    if (VALS) {
+#pragma unroll
+        for (int j = 0; j < KPT; j++)
+            val[j] = wbase + j * VREN_WAVE + lane < tile_count ? __builtin_nontemporal_load(vin + j * VREN_WAVE) : 0u;
+    }
     __syncthreads();
-    // per digit: exclusive prefix across waves, tile count
+    // per digit: tile count (padding included: it only adds to the largest digit), then the exclusive prefix
+    // across waves offset by the digit's tile-local start, so a key's staged position is s_whist[wave][d] + local
     uint32_t dcount = 0;
+    uint32_t wc[WAVES];
     if (tid < (uint32_t)RADIX) {
-        uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < WAVES; w++) {
-            uint32_t c = s_whist[w * RADIX + tid];
-            s_whist[w * RADIX + tid] = run;
-            run += c;
+            wc[w] = s_whist[w * RADIX + tid];
+            dcount += wc[w];
         }
-        dcount = run;
     }
+    // onesweep: this tile's per-digit counts are visible to the tiles after it before its own look-back
+    if (!RTS && tid < (uint32_t)RADIX)
+        st_agent_u32(status + (uint64_t)tile * RADIX + tid, (tile == 0 ? RS_INCL : RS_AGG) | dcount);
     uint32_t total;
     uint32_t dstart = block_exclusive_sum<THREADS>(dcount, s_scan, &total);
+    if (tid < (uint32_t)RADIX) {
+        uint32_t run = dstart;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+            s_whist[w * RADIX + tid] = run;
+            run += wc[w];
+        }
+    }
     uint32_t prefix = 0;
     if (RTS) {
         if (tid < (uint32_t)RADIX) {
             const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
-            s_start[tid] = dstart;
             s_goff[tid] = digit_base[(uint64_t)tid * tiles + tile] - dstart;
         }
     } else if (tid < (uint32_t)RADIX) {
-        s_start[tid] = dstart;
         // decoupled look-back on this digit's count
         uint32_t* my = status + (uint64_t)tile * RADIX + tid;
-        if (tile == 0) {
-            st_agent_u32(my, RS_INCL | dcount);
-        } else {
-            st_agent_u32(my, RS_AGG | dcount);
+        if (tile != 0) {
             int64_t t = (int64_t)tile - 1;
             while (t >= 0) {
                 const uint32_t* p = status + (uint64_t)t * RADIX + tid;
@@ -253,55 +264,61 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     // global digit base (exclusive scan of this pass's histogram, rs_hexscan_kernel)
     if (!RTS && tid < (uint32_t)RADIX) s_goff[tid] = digit_base[tid] + prefix - dstart;
     __syncthreads();
-    // scatter into LDS in tile-local sorted order
+    // scatter into LDS in tile-local sorted order (every staged position is < TILE, padding included)
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
-        uint32_t pos = wbase + j * VREN_WAVE + lane;
-        if (pos < tile_count) {
-            uint32_t d = digit[j];
-            uint32_t dst = s_start[d] + s_whist[wave * RADIX + d] + local[j];
-            s_keys[dst] = key[j];
-            if (VALS && !SEQ) s_vals[dst] = val[j];
-        }
+        local[j] += s_whist[wave * RADIX + digit[j]];
+        s_keys[local[j]] = key[j];
+        if (VALS && !SEQ) s_vals[local[j]] = val[j];
     }
     __syncthreads();
-    if (!SEQ) {
-        for (uint32_t i = tid; i < tile_count; i += THREADS) {
-            uint32_t k = s_keys[i];
-            uint32_t d = (k >> shift) & mask;
-            uint32_t o = s_goff[d] + i;
-            if (o < n) {
-                keys_out[o] = k;
-                if (VALS) vals_out[o] = s_vals[i];
+    // staged position i = tid + j * THREADS goes to s_goff[digit] + i (< n: the digit's count over the tiles up to
+    // this one never exceeds its count in the pass histogram). A full tile reads all its staged keys before the
+    // first store, so the LDS reads overlap.
+    const bool full = tile_count == (uint32_t)TILE;
+    uint32_t o[KPT];
+    if (full) {
+        uint32_t k[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) k[j] = s_keys[tid + j * THREADS];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) o[j] = s_goff[__builtin_amdgcn_ubfe(k[j], shift, width)] + tid + j * THREADS;
+        if (VALS && !SEQ) {
+            uint32_t v[KPT];
+#pragma unroll
+            for (int j = 0; j < KPT; j++) v[j] = s_vals[tid + j * THREADS];
+#pragma unroll
+            for (int j = 0; j < KPT; j++) {
+                keys_out[o[j]] = k[j];
+                vals_out[o[j]] = v[j];
             }
+        } else {
+#pragma unroll
+            for (int j = 0; j < KPT; j++) keys_out[o[j]] = k[j];
         }
     } else {
-        // keys first, keeping each staged position's destination, then the values through the same buffer
-        uint32_t o[KPT];
 #pragma unroll
         for (int j = 0; j < KPT; j++) {
             const uint32_t i = tid + j * THREADS;
-            o[j] = 0xFFFFFFFFu;
+            o[j] = 0u;
             if (i < tile_count) {
                 const uint32_t k = s_keys[i];
-                o[j] = s_goff[(k >> shift) & mask] + i;
-                if (o[j] < n) keys_out[o[j]] = k;
+                o[j] = s_goff[__builtin_amdgcn_ubfe(k, shift, width)] + i;
+                keys_out[o[j]] = k;
+                if (VALS && !SEQ) vals_out[o[j]] = s_vals[i];
             }
         }
+    }
+    if (SEQ) {
+        // the values through the same staging buffer, to the destinations kept from the keys
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < KPT; j++) {
-            uint32_t pos = wbase + j * VREN_WAVE + lane;
-            if (pos < tile_count) {
-                uint32_t d = digit[j];
-                s_keys[s_start[d] + s_whist[wave * RADIX + d] + local[j]] = val[j];
-            }
-        }
+        for (int j = 0; j < KPT; j++) s_keys[local[j]] = val[j];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < KPT; j++) {
             const uint32_t i = tid + j * THREADS;
-            if (i < tile_count && o[j] < n) vals_out[o[j]] = s_keys[i];
+            if (full || i < tile_count) vals_out[o[j]] = s_keys[i];
         }
     }
 }
