@@ -28,6 +28,21 @@ __device__ __forceinline__ void match_bit(uint32_t d, uint32_t& lo, uint32_t& hi
         : "vcc");
 }
 
+// Bit 0 of a match that starts from every lane: peers = ~(ballot ^ bc) (truth table ~(0xCC ^ 0xAA) = 0x99, S0
+// unused).
+__device__ __forceinline__ void match_bit0_all(uint32_t d, uint32_t& lo, uint32_t& hi)
+{
+    uint32_t bc;
+    asm("v_bfe_i32 %2, %3, 0, 1\n\t"
+        "v_cmp_ne_u32_e32 vcc, 0, %2\n\t"
+        "s_nop 1\n\t"
+        "v_bitop3_b32 %0, %2, vcc_lo, %2 bitop3:0x99\n\t"
+        "v_bitop3_b32 %1, %2, vcc_hi, %2 bitop3:0x99"
+        : "=&v"(lo), "=&v"(hi), "=&v"(bc)
+        : "v"(d)
+        : "vcc");
+}
+
 template <int B, int BITS>
 __device__ __forceinline__ void match_bits(uint32_t d, uint32_t& lo, uint32_t& hi)
 {
@@ -37,12 +52,18 @@ __device__ __forceinline__ void match_bits(uint32_t d, uint32_t& lo, uint32_t& h
     }
 }
 
-// Lanes of the wave holding the same BITS-bit digit as this lane (and set in lo:hi on entry), four VALU
-// operations per digit bit.
-template <int BITS>
+// Lanes of the wave holding the same BITS-bit digit as this lane, four VALU operations per digit bit.
+// ALL = true: every lane of the wave takes part (lo:hi are outputs only); ALL = false: only the lanes set in lo:hi
+// on entry.
+template <int BITS, bool ALL = false>
 __device__ __forceinline__ void match_digit(uint32_t d, uint32_t& lo, uint32_t& hi)
 {
-    match_bits<0, BITS>(d, lo, hi);
+    if constexpr (ALL) {
+        match_bit0_all(d, lo, hi);
+        match_bits<1, BITS>(d, lo, hi);
+    } else {
+        match_bits<0, BITS>(d, lo, hi);
+    }
 }
 
 // s_whist: WAVES * RADIX counters, zeroed before the call (and a __syncthreads() after zeroing).
@@ -67,7 +88,7 @@ __device__ __forceinline__ void rank_wave_keys(const uint32_t (&digit)[KPT], uin
             hi = (uint32_t)(v >> 32);
         }
         const uint32_t before = valid ? s_wave_row[d] : 0u;
-        match_digit<BITS>(d, lo, hi);
+        match_digit<BITS, !CHECK>(d, lo, hi);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
         const uint64_t peers = ((uint64_t)hi << 32) | lo;
         // the highest peer lane publishes the new running count

@@ -19,6 +19,9 @@
 
 namespace {
 
+// predecessor states read per round trip of the onesweep look-back
+constexpr int kLookback = 8;
+
 // status word of (tile, digit): [31:30] state, [29:0] count
 #define RS_AGG (1u << 30)
 #define RS_INCL (2u << 30)
@@ -212,8 +215,8 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             val[j] = wbase + j * VREN_WAVE + lane < tile_count ? __builtin_nontemporal_load(vin + j * VREN_WAVE) : 0u;
     }
     __syncthreads();
-    // per digit: tile count (padding included: it only adds to the largest digit), then the exclusive prefix
-    // across waves offset by the digit's tile-local start, so a key's staged position is s_whist[wave][d] + local
+    // per digit: tile count, then the exclusive prefix across waves offset by the digit's tile-local start, so a
+    // key's staged position is s_whist[wave][d] + local
     uint32_t dcount = 0;
     uint32_t wc[WAVES];
     if (tid < (uint32_t)RADIX) {
@@ -223,11 +226,13 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             dcount += wc[w];
         }
     }
-    // onesweep: this tile's per-digit counts are visible to the tiles after it before its own look-back
+    // onesweep: the tile's per-digit counts (the padding of a partial tile included: only the last tile has
+    // padding, and no tile looks back at the last one) are published before the block scan
     if (!RTS && tid < (uint32_t)RADIX)
         st_agent_u32(status + (uint64_t)tile * RADIX + tid, (tile == 0 ? RS_INCL : RS_AGG) | dcount);
     uint32_t total;
     uint32_t dstart = block_exclusive_sum<THREADS>(dcount, s_scan, &total);
+    uint32_t prefix = 0;
     if (tid < (uint32_t)RADIX) {
         uint32_t run = dstart;
 #pragma unroll
@@ -235,34 +240,41 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             s_whist[w * RADIX + tid] = run;
             run += wc[w];
         }
-    }
-    uint32_t prefix = 0;
-    if (RTS) {
-        if (tid < (uint32_t)RADIX) {
+        if (!RTS && tile != 0) {
+            // windowed decoupled look-back on this digit: the states of the kLookback nearest unread predecessors
+            // are loaded at once and their counts summed up to the first inclusive one; an unpublished state is
+            // waited for and the walk resumes there (tile 0 is always inclusive, so the walk ends at or before it)
+            int64_t t = (int64_t)tile - 1;
+            for (;;) {
+                uint32_t st[kLookback];
+#pragma unroll
+                for (int i = 0; i < kLookback; i++)
+                    st[i] = t - i >= 0 ? ld_agent_u32(status + (uint64_t)(t - i) * RADIX + tid) : RS_INCL;
+                int i = 0;
+                bool done = false;
+                for (; i < kLookback; i++) {
+                    if ((st[i] & ~RS_MASK) == 0u) break;
+                    prefix += st[i] & RS_MASK;
+                    if (st[i] & RS_INCL) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (done) break;
+                t -= i;
+                if (i < kLookback) __builtin_amdgcn_s_sleep(1);
+            }
+            st_agent_u32(status + (uint64_t)tile * RADIX + tid, RS_INCL | (prefix + dcount));
+        }
+        // global destination of staged position i = s_goff[digit] + i: RTS from the scanned digit-major tile
+        // counts, onesweep from the pass histogram's exclusive scan (rs_hexscan_kernel) plus the look-back prefix
+        if (RTS) {
             const uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
             s_goff[tid] = digit_base[(uint64_t)tid * tiles + tile] - dstart;
-        }
-    } else if (tid < (uint32_t)RADIX) {
-        // decoupled look-back on this digit's count
-        uint32_t* my = status + (uint64_t)tile * RADIX + tid;
-        if (tile != 0) {
-            int64_t t = (int64_t)tile - 1;
-            while (t >= 0) {
-                const uint32_t* p = status + (uint64_t)t * RADIX + tid;
-                uint32_t s = ld_agent_u32(p);
-                while ((s & ~RS_MASK) == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    s = ld_agent_u32(p);
-                }
-                prefix += s & RS_MASK;
-                if (s & RS_INCL) break;
-                t--;
-            }
-            st_agent_u32(my, RS_INCL | (prefix + dcount));
+        } else {
+            s_goff[tid] = digit_base[tid] + prefix - dstart;
         }
     }
-    // global digit base (exclusive scan of this pass's histogram, rs_hexscan_kernel)
-    if (!RTS && tid < (uint32_t)RADIX) s_goff[tid] = digit_base[tid] + prefix - dstart;
     __syncthreads();
     // scatter into LDS in tile-local sorted order (every staged position is < TILE, padding included)
 #pragma unroll
