@@ -10,6 +10,7 @@
  *
  * Pins (what checks each function independently of this file; see tests/):
  *   o_pcg32_*            published PCG32 demo vector (seed 42, seq 54) + jump-ahead == sequential
+ *   o_logf_first_decrease  a planted glitch is reported at exactly its bit pattern
  *   o_reduce_tree_*      closed form tree[i] = op over the aligned block ending at i; totals vs numpy
  *   o_exclusive_scan_u32 numpy cumsum (uint32 wrap-around)
  *   o_stable_sort_kv     numpy argsort(kind='stable')
@@ -114,6 +115,28 @@ OEXPORT void o_pcg32_fill_at(uint64_t initstate, uint64_t initseq, uint64_t star
 }
 
 OEXPORT float o_logf(float x) { return vren_logf(x); }
+
+/* First float bit pattern b in [lo, hi) with vren_logf(b + 1) < vren_logf(b) (0xFFFFFFFF when there is none):
+ * the GPU depth-slice threshold table relies on vren_logf being non-decreasing over the non-negative floats.
+ * planted != 0 raises the value at bit pattern `planted` by 1e-4 relative (far more than one step of the log
+ * between neighbouring floats), a decrease the scan must report there. */
+static float logf_scanned(uint32_t b, uint32_t planted)
+{
+    const float v = vren_logf(vren_bits_f32(b));
+    return (planted && b == planted) ? v + std::fabs(v) * 1e-4f + 1e-30f : v;
+}
+
+OEXPORT uint32_t o_logf_first_decrease(uint32_t lo, uint32_t hi, uint32_t planted)
+{
+    float prev = logf_scanned(lo, planted);
+    for (uint32_t b = lo; b < hi; b++) {
+        const float next = logf_scanned(b + 1, planted);
+        if (next < prev) return b;
+        prev = next;
+    }
+    return 0xFFFFFFFFu;
+}
+
 OEXPORT float o_expf(float x) { return vren_expf(x); }
 OEXPORT float o_powi(float a, uint32_t k) { return vren_powi(a, k); }
 

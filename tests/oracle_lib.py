@@ -29,6 +29,8 @@ class Oracle:
         for name, res in [("o_logf", c_f), ("o_expf", c_f)]:
             getattr(L, name).restype = res
             getattr(L, name).argtypes = [c_f]
+        L.o_logf_first_decrease.restype = c_u32
+        L.o_logf_first_decrease.argtypes = [c_u32, c_u32, c_u32]
         L.o_powi.restype = c_f
         L.o_powi.argtypes = [c_f, c_u32]
         L.o_reduce_total_u32.restype = c_u32
@@ -49,6 +51,9 @@ class Oracle:
 
     def logf(self, x):
         return self.lib.o_logf(float(x))
+
+    def logf_first_decrease(self, lo, hi, planted=0):
+        return int(self.lib.o_logf_first_decrease(lo, hi, planted))
 
     def expf(self, x):
         return self.lib.o_expf(float(x))

@@ -133,6 +133,40 @@ def test_unique_clusters(vctx, co, c3, case):
     assert np.array_equal(refs, orefs)
 
 
+@pytest.mark.parametrize("mode", ["reference", "sliced"])
+def test_unique_clusters_slice_boundaries(vctx, co, mode):
+    """Depth buffers of float bit-pattern sweeps, keys and references bit for bit against the oracle's formula:
+    a coarse sweep over (0, 1) reaches every depth slice, consecutive floats just below 1.0 cross the far slices'
+    boundaries between neighbouring floats, depths above 1.0 drive the view depth past far, through +inf to
+    negative values (beyond the threshold table), and NaN, +-inf, -0 and subnormal depths close the buffer."""
+    W, H = 1024, 1024
+    n = W * H
+    cam = ol.camera(W, H)
+    q = n // 4
+    coarse = np.linspace(0x34000000, 0x3F7FFFFF, q).astype(np.uint32)
+    below_one = np.uint32(0x3F7FFFFF) - np.arange(q, dtype=np.uint32)
+    above_one = np.uint32(0x3F800000) + np.arange(q, dtype=np.uint32) * np.uint32(64)
+    # the depth where the inverse projection's w row vanishes, and 1024 floats on each side: view depths of huge
+    # magnitude up to +-inf (log slices >= 1023 wrap in the 10-bit field)
+    inv, _ = co.frame_consts(W, H, cam)
+    d0 = (-np.float32(inv[15]) / np.float32(inv[11])).astype(np.float32).view(np.uint32)
+    pole = (d0.astype(np.int64) + np.arange(-1024, 1024)).astype(np.uint32)
+    special = np.array([0x7FC00000, 0x7F800000, 0xFF800000, 0x80000000, 0x00000001, 0x007FFFFF, 0x00000000,
+                        0x3F800000], np.uint32)
+    rest = np.resize(np.concatenate([pole, special]), n - 3 * q)
+    depth = np.concatenate([coarse, below_one, above_one, rest]).view(np.float32)
+    _, normals = co.synth_frame(777, W, H, cam)
+    cf = ol.cfg(key_mode=1, depth_slices=200) if mode == "sliced" else ol.cfg()
+    _, keys, U, refs, _, _ = gpu_clusters(vctx, W, H, cam, cf, depth, normals)
+    okeys, oU, orefs = co.unique_clusters(W, H, cam, cf, depth, normals)
+    assert U == oU
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(refs, orefs)
+    shift, width = (16, 8) if mode == "sliced" else (16, 10)
+    slices = np.unique((okeys >> shift) & ((1 << width) - 1))
+    assert slices.size == 200 if mode == "sliced" else slices.size > 150, slices.size
+
+
 def test_unique_clusters_capacity_flag(vctx, co, c3):
     from vren_amd import VrenError
     _, keys, U, refs, _, _ = gpu_clusters(vctx, W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)

@@ -48,6 +48,20 @@ def test_vmath_logf_error_bound(oracle):
     assert math.isnan(oracle.logf(-1.0))
 
 
+def test_vmath_logf_nondecreasing_over_all_nonnegative_floats(oracle):
+    """vren_logf(b) <= vren_logf(b + 1) for every bit pattern b of [+0, +inf): the GPU's depth-slice threshold
+    table equals the slice formula only if the log never decreases. Exhaustive, 8 threads (the scan releases the
+    GIL); a planted glitch must be reported at exactly its bit pattern."""
+    from concurrent.futures import ThreadPoolExecutor
+    assert oracle.logf_first_decrease(0x3F000000, 0x3F010000) == 0xFFFFFFFF
+    assert oracle.logf_first_decrease(0x3F000000, 0x3F010000, planted=0x3F00ABCD) == 0x3F00ABCD
+    end = 0x7F800000
+    chunks = [(i * end // 8, (i + 1) * end // 8) for i in range(8)]
+    with ThreadPoolExecutor(8) as ex:
+        found = list(ex.map(lambda c: oracle.logf_first_decrease(c[0], c[1]), chunks))
+    assert found == [0xFFFFFFFF] * 8, [hex(f) for f in found]
+
+
 def test_vmath_expf_and_powi(oracle):
     rng = np.random.default_rng(2)
     worst = 0.0

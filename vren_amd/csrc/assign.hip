@@ -239,11 +239,11 @@ __device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t wor
 // COUNT (assign_lights.comp, count pass): the traversal marks the hit lights in the wave's LDS bitmask; the count
 // is stored, and a list of at most VREN_LIST_SLOT lights is kept in the cluster's slot (ascending), so the writing
 // pass copies it instead of traversing again.
-__global__ void assign_count_kernel(const FrameDev* frames, uint32_t F, uint32_t mask_words)
+__global__ void assign_count_kernel(const FrameDev* __restrict__ frames, uint32_t F, uint32_t mask_words)
 {
     extern __shared__ uint32_t s_mem[];
     const uint32_t wpb = blockDim.x / VREN_WAVE;
-    const uint32_t wave = threadIdx.x / VREN_WAVE;
+    const uint32_t wave = wave_id();
     uint32_t* stack = s_mem + wave * kStack;
     uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
     for (uint32_t i = lane_id(); i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
@@ -264,7 +264,7 @@ __global__ void assign_count_kernel(const FrameDev* frames, uint32_t F, uint32_t
 // exclusive scan of counts[0 .. min(cluster_count, key_cap)) -> offsets and assigned_total: one workgroup per
 // frame walks its counts 4096 at a time with a running carry (no cross-workgroup chaining, no tickets; a frame
 // holds a few thousand clusters, at most key_cap)
-__global__ __launch_bounds__(1024) void assign_offsets_kernel(const FrameDev* frames)
+__global__ __launch_bounds__(1024) void assign_offsets_kernel(const FrameDev* __restrict__ frames)
 {
     __shared__ uint32_t s_wave[16];
     const FrameDev& fd = frames[blockIdx.x];
@@ -295,11 +295,11 @@ __global__ __launch_bounds__(1024) void assign_offsets_kernel(const FrameDev* fr
     }
 }
 
-__global__ void assign_write_kernel(const FrameDev* frames, uint32_t F, uint32_t mask_words)
+__global__ void assign_write_kernel(const FrameDev* __restrict__ frames, uint32_t F, uint32_t mask_words)
 {
     extern __shared__ uint32_t s_mem[];
     const uint32_t wpb = blockDim.x / VREN_WAVE;
-    const uint32_t wave = threadIdx.x / VREN_WAVE;
+    const uint32_t wave = wave_id();
     const uint32_t lane = lane_id();
     uint32_t* stack = s_mem + wave * kStack;
     uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
@@ -395,9 +395,9 @@ extern "C" VREN_EXPORT int vren_assign_lights(vren_ctx* ctx, vren_stream_t strea
     ChainWork w;
     memset(&w, 0, sizeof(w));
     const uint32_t offs_tiles = (key_capacity + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
-    st = vren_ensure_scratch(ctx, chain_work_bytes(0, offs_tiles));
+    st = vren_ensure_scratch(ctx, chain_work_bytes(0, offs_tiles, 0));
     if (st) return st;
-    chain_work_bind(&w, ctx->scratch, 0, offs_tiles);
+    chain_work_bind(&w, ctx->scratch, 0, offs_tiles, 0);
     w.max_L = light_count;
     FrameDev* slot = vren_next_frame_slot(ctx);
     st = vren_upload_frames(ctx, s, slot, &f, 1);

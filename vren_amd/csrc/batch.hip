@@ -141,7 +141,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     }
     hipError_t e = hipMalloc(&b->arena, std::max<size_t>(arena, 256));
     if (e == hipSuccess) e = hipMemset(b->arena, 0, std::max<size_t>(arena, 256));
-    const size_t wbytes = chain_work_bytes(tile_base, offs_base);
+    const size_t wbytes = chain_work_bytes(tile_base, offs_base, n_frames);
     if (e == hipSuccess) e = hipMalloc(&b->work, wbytes);
     if (e == hipSuccess) e = hipMalloc(&b->d_table, sizeof(FrameDev) * n_frames);
     if (e == hipSuccess) e = hipMalloc(&b->d_single, sizeof(FrameDev) * n_frames);
@@ -174,7 +174,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.leaf_pos = f.L ? reinterpret_cast<float*>(b->arena + o[11]) : nullptr;
         f.list_slots = f.slot_cap ? reinterpret_cast<uint32_t*>(b->arena + o[12]) : nullptr;
     }
-    chain_work_bind(&b->w, b->work, tile_base, offs_base);
+    chain_work_bind(&b->w, b->work, tile_base, offs_base, n_frames);
     b->w.tile_frame = b->d_tile_frame;
     b->w.total_leaf_slots = leaf_base;
     b->w.max_L = max_L;
@@ -186,7 +186,8 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         single[i].leaf_base = 0;
         single[i].offs_base = 0;
         ChainWork& wf = b->w_frame[i];
-        chain_work_bind(&wf, b->work, single[i].Sx * single[i].Sy, (single[i].key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE);
+        chain_work_bind(&wf, b->work, single[i].Sx * single[i].Sy, (single[i].key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE,
+                        1);
         wf.total_leaf_slots = b->leaf_slots[i];
         wf.max_L = single[i].L;
     }

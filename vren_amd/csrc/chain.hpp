@@ -17,6 +17,7 @@ struct ChainWork {
     uint64_t* tile_status; // cluster compaction: one word per 32x32 tile of every frame
     uint64_t* offs_status; // light-offset scan: one word per 4096 clusters of capacity of every frame
     uint32_t* tickets;     // [0] cluster tiles, [1] offset tiles
+    float* slice_thr;      // depth-slice threshold table of every frame (VREN_SLICE_TABLE floats each)
     const uint32_t* tile_frame; // frame of every flattened 32x32 tile (null for a one-frame table)
     uint32_t total_tiles;
     uint32_t total_offs_tiles;
@@ -35,11 +36,20 @@ inline void chain_mark(const ChainWork& w, hipStream_t s)
 // kernels of one chain launch, in order (chain_mark follows each)
 #define VREN_CHAIN_KERNELS 9
 
-inline size_t chain_work_bytes(uint32_t total_tiles, uint32_t total_offs_tiles)
+// floats per frame of the depth-slice threshold table (clusters.hip)
+#define VREN_SLICE_TABLE 1024u
+
+// workspace: 256 bytes of tickets, the look-back statuses, then (256-aligned) the slice tables of table_frames
+// frames
+inline size_t chain_work_table_offset(uint32_t total_tiles, uint32_t total_offs_tiles)
 {
-    return 256 + ((size_t)total_tiles + total_offs_tiles) * sizeof(uint64_t);
+    return (256 + ((size_t)total_tiles + total_offs_tiles) * sizeof(uint64_t) + 255) & ~(size_t)255;
 }
-void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t total_offs_tiles);
+inline size_t chain_work_bytes(uint32_t total_tiles, uint32_t total_offs_tiles, uint32_t table_frames)
+{
+    return chain_work_table_offset(total_tiles, total_offs_tiles) + (size_t)table_frames * VREN_SLICE_TABLE * 4;
+}
+void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t total_offs_tiles, uint32_t table_frames);
 
 // stage launchers; d_frames is a device table of F FrameDev records
 int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w);

@@ -64,10 +64,55 @@ __device__ __forceinline__ uint32_t face_of_normal(const float n[3])
     return ax < 3 ? (n[ax] > 0.0f ? 1u : 0u) * 3u + ax : 0u;
 }
 
-__device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t gx, uint32_t gy,
-                                              float depth, uint2 nb)
+// The depth slice of a view depth z by the reference's formula (find_unique_clusters.comp:58-62): log mode
+// f2u_sat(floor(log(z / near) / log_a)) before the 10-bit mask, sliced mode the same over [near, far] in
+// depth_slices steps, clamped to the last slice. A non-positive or NaN z gives 0 (log of a non-positive ratio is
+// NaN or -inf, which f2u_sat maps to 0).
+__device__ __forceinline__ uint32_t slice_exact(const FrameDev& fd, float z)
 {
-    // find_unique_clusters.comp:48-65
+    if (fd.key_mode == VREN_KEYS_SLICED) {
+        uint32_t k = vren_f2u_sat(vren_floorf(vren_logf(z / fd.near_plane) / fd.log_fn * (float)fd.depth_slices));
+        return k >= fd.depth_slices ? fd.depth_slices - 1 : k;
+    }
+    return vren_f2u_sat(vren_floorf(vren_logf(z / fd.near_plane) / fd.log_a));
+}
+
+// Slices kept in the threshold table (thr[1 .. kt - 1] are looked up; z >= thr[kt] takes the formula): every
+// sliced-mode slice, log-mode slices below 1023 (the key keeps k & 0x3FF, so k >= 1023 goes through the formula)
+__device__ __forceinline__ uint32_t slice_table_kt(const FrameDev& fd)
+{
+    return fd.key_mode == VREN_KEYS_SLICED ? fd.depth_slices - 1 : 1023u;
+}
+
+// thr[k] = the smallest positive float z with slice_exact(z) >= k, for k = 1 .. kt (+inf when no finite z reaches
+// k), thr[0] = 0. slice_exact is non-decreasing in z: z / near and the division by log_a (or log_fn) are correctly
+// rounded divisions by positive constants, floor and f2u_sat are monotone, and vren_logf is non-decreasing over
+// every non-negative float (checked exhaustively by the oracle tests). So for 0 < z < thr[kt],
+// slice_exact(z) = #{k in 1 .. kt - 1 : thr[k] <= z}, and positive floats order like their bit patterns, so each
+// threshold is a binary search over bits (slice_exact(+inf) >= kt in both modes).
+__global__ __launch_bounds__(1024) void slice_table_kernel(const FrameDev* __restrict__ frames, float* thr_all)
+{
+    const FrameDev& fd = frames[blockIdx.x];
+    float* thr = thr_all + (uint64_t)blockIdx.x * VREN_SLICE_TABLE;
+    const uint32_t kt = slice_table_kt(fd);
+    const uint32_t k = threadIdx.x;
+    if (k > kt) return;
+    if (k == 0) {
+        thr[0] = 0.0f;
+        return;
+    }
+    uint32_t lo = 0u, hi = 0x7F800000u; // slice_exact(+0) = 0 < k <= slice_exact(+inf)
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (slice_exact(fd, __uint_as_float(mid)) >= k) hi = mid;
+        else lo = mid;
+    }
+    thr[k] = __uint_as_float(hi);
+}
+
+// view depth of a pixel (find_unique_clusters.comp:48-56)
+__device__ __forceinline__ float pixel_view_z(const FrameDev& fd, uint32_t gx, uint32_t gy, float depth)
+{
     float r2, r3;
     if (fd.proj_zw_xy_zero) {
         // the z and w rows of inverse(P) have zero x and y coefficients: ((0 x + 0 y) + m2 depth) + m3 reduces to
@@ -84,16 +129,16 @@ __device__ __forceinline__ uint32_t pixel_key(const FrameDev& fd, uint32_t ti, u
         r2 = r[2];
         r3 = r[3];
     }
-    const float z = r2 / r3;
+    return r2 / r3;
+}
+
+// the cluster key from its slice and normal (find_unique_clusters.comp:58-65)
+__device__ __forceinline__ uint32_t assemble_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t k, uint2 nb)
+{
     const float n[3] = {vren_h2f((uint16_t)(nb.x & 0xFFFFu)), vren_h2f((uint16_t)(nb.x >> 16)),
                         vren_h2f((uint16_t)(nb.y & 0xFFFFu))};
-    if (fd.key_mode == VREN_KEYS_SLICED) {
-        float s = vren_floorf(vren_logf(z / fd.near_plane) / fd.log_fn * (float)fd.depth_slices);
-        uint32_t k = vren_f2u_sat(s);
-        if (k >= fd.depth_slices) k = fd.depth_slices - 1;
+    if (fd.key_mode == VREN_KEYS_SLICED)
         return (ti & 0xFFu) | ((tj & 0xFFu) << 8) | ((k & 0xFFu) << 16) | (face_of_normal(n) << 24);
-    }
-    const uint32_t k = vren_f2u_sat(vren_floorf(vren_logf(z / fd.near_plane) / fd.log_a));
     return (ti & 0xFFu) | ((tj & 0xFFu) << 8) | ((k & 0x3FFu) << 16) | (discretize_normal(n) << 26);
 }
 
@@ -113,31 +158,50 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 
 constexpr uint32_t kClusterWaves = 8;   // waves per workgroup, one 32x32 tile per wave (16 measured slower)
 
-// the 4 horizontally adjacent pixels (x0 .. x0+3, y) of a tile row: one 16-byte depth load and two 16-byte normal
-// loads when they lie inside the frame on an aligned row, otherwise per pixel, wrapping (x mod W, y mod H) like
-// the reference's REPEAT sampler
-__device__ __forceinline__ void load_quad(const FrameDev& fd, uint32_t x0, uint32_t y, float dep[4], uint2 nrm[4])
+// global-address-space views of the G-buffer pointers of a frame record: loads through them compile to global_load
+// instead of flat_load (native vector types, since HIP's float4/uint4 copy through a generic reference)
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) float gfloat;
+typedef const __attribute__((address_space(1))) vf4 gvf4;
+typedef const __attribute__((address_space(1))) vu4 gvu4;
+typedef const __attribute__((address_space(1))) vu2 gvu2;
+
+// The 16 depths and normals of lane `lane` of a tile: pixel i = 4 j + q is (x0 + q, tj * 32 + 8 j + lane / 8).
+// A tile inside the frame with 16-byte aligned rows (uniform across the wave) takes one 16-byte depth load and two
+// 16-byte normal loads per row; other tiles load per pixel, wrapping (x mod W, y mod H) like the reference's
+// REPEAT sampler. Every load is issued before any is used.
+__device__ __forceinline__ void load_tile_pixels(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t lane,
+                                                 float dep[16], uint2 nrm[16])
 {
-    const uint64_t p0 = (uint64_t)y * fd.W + x0;
-    if (y < fd.H && x0 + 3 < fd.W && (p0 & 3) == 0 &&
-        ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0) {
-        const float4 d4 = *reinterpret_cast<const float4*>(fd.depth + p0);
-        const uint4 n01 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0);
-        const uint4 n23 = *reinterpret_cast<const uint4*>(fd.normals + 4 * p0 + 8);
-        dep[0] = d4.x;
-        dep[1] = d4.y;
-        dep[2] = d4.z;
-        dep[3] = d4.w;
-        nrm[0] = make_uint2(n01.x, n01.y);
-        nrm[1] = make_uint2(n01.z, n01.w);
-        nrm[2] = make_uint2(n23.x, n23.y);
-        nrm[3] = make_uint2(n23.z, n23.w);
+    const uint32_t x0 = ti * 32 + (lane & 7) * 4;
+    const bool interior = (ti + 1) * 32 <= fd.W && (tj + 1) * 32 <= fd.H && (fd.W & 3) == 0 &&
+                          ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0;
+    if (interior) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t p0 = (uint64_t)(tj * 32 + j * 8 + (lane >> 3)) * fd.W + x0;
+            const vf4 d4 = *(gvf4*)(fd.depth + p0);
+            const vu4 n01 = *(gvu4*)(fd.normals + 4 * p0);
+            const vu4 n23 = *(gvu4*)(fd.normals + 4 * p0 + 8);
+            dep[4 * j + 0] = d4.x;
+            dep[4 * j + 1] = d4.y;
+            dep[4 * j + 2] = d4.z;
+            dep[4 * j + 3] = d4.w;
+            nrm[4 * j + 0] = make_uint2(n01.x, n01.y);
+            nrm[4 * j + 1] = make_uint2(n01.z, n01.w);
+            nrm[4 * j + 2] = make_uint2(n23.x, n23.y);
+            nrm[4 * j + 3] = make_uint2(n23.z, n23.w);
+        }
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint64_t p = (uint64_t)(y % fd.H) * fd.W + ((x0 + q) % fd.W);
-            dep[q] = fd.depth[p];
-            nrm[q] = reinterpret_cast<const uint2*>(fd.normals)[p];
+        for (int i = 0; i < 16; i++) {
+            const uint32_t x = x0 + (i & 3), y = tj * 32 + (i >> 2) * 8 + (lane >> 3);
+            const uint64_t p = (uint64_t)(y % fd.H) * fd.W + (x % fd.W);
+            dep[i] = *(gfloat*)(fd.depth + p);
+            const vu2 n = *(gvu2*)(fd.normals + 4 * p);
+            nrm[i] = make_uint2(n.x, n.y);
         }
     }
 }
@@ -149,35 +213,101 @@ __device__ __forceinline__ void load_quad(const FrameDev& fd, uint32_t x0, uint3
 // wait ends. (Giving a wave several tiles one after another would make the next workgroup's first tile wait for
 // this workgroup's last step, and serialise the grid.) Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8),
 // q, j = 0..3, of its tile.
-__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* frames,
-                                                                       const uint32_t* tile_frame,
+__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* __restrict__ frames,
+                                                                       const uint32_t* __restrict__ tile_frame,
                                                                        uint32_t total_tiles, uint64_t* tile_status,
-                                                                       uint32_t* ticket)
+                                                                       uint32_t* ticket, const float* __restrict__ slice_thr)
 {
     __shared__ uint32_t s_unique[kClusterWaves][1024];
+    __shared__ float s_thr[2][VREN_SLICE_TABLE];
     __shared__ uint32_t s_t;
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / VREN_WAVE;
+    const uint32_t wave = wave_id();
     if (threadIdx.x == 0) s_t = atomic_add_agent(ticket, 1u);
+    __syncthreads();
+    const uint32_t g0 = s_t * kClusterWaves;
+    if (g0 >= total_tiles) return; // the whole workgroup
+    // the slice tables of the frames of the workgroup's first and last tile (the only frames of its tiles unless
+    // frames have fewer tiles than the workgroup has waves; a wave of any other frame evaluates the formula)
+    const uint32_t fa = frame_of_tile(tile_frame, g0);
+    const uint32_t fb = frame_of_tile(tile_frame, min(g0 + kClusterWaves, total_tiles) - 1);
+    {
+        // both halves of each table are requested before either is stored
+        static_assert(VREN_SLICE_TABLE == 2 * kClusterWaves * 64, "two table entries per thread");
+        const uint32_t kta = slice_table_kt(frames[fa]), ktb = slice_table_kt(frames[fb]);
+        const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + kClusterWaves * 64;
+        const float* ta = slice_thr + (uint64_t)fa * VREN_SLICE_TABLE;
+        const float* tb = slice_thr + (uint64_t)fb * VREN_SLICE_TABLE;
+        const float a0 = i0 <= kta ? ta[i0] : 0.0f, a1 = i1 <= kta ? ta[i1] : 0.0f;
+        float b0 = 0.0f, b1 = 0.0f;
+        if (fb != fa) {
+            b0 = i0 <= ktb ? tb[i0] : 0.0f;
+            b1 = i1 <= ktb ? tb[i1] : 0.0f;
+        }
+        s_thr[0][i0] = a0;
+        s_thr[0][i1] = a1;
+        s_thr[1][i0] = b0;
+        s_thr[1][i1] = b1;
+    }
     __syncthreads();
     uint32_t* uq = s_unique[wave];
     {
-        const uint32_t g = s_t * kClusterWaves + wave;
+        const uint32_t g = g0 + wave;
         if (g >= total_tiles) return;
         const uint32_t f = frame_of_tile(tile_frame, g);
         const FrameDev& fd = frames[f];
         const uint32_t t = g - fd.tile_base;
         const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
         const uint32_t x0 = ti * 32 + (lane & 7) * 4;
+        const bool table = f == fa || f == fb;
+        const float* thr = s_thr[f == fa ? 0 : 1];
+        const uint32_t kt = slice_table_kt(fd);
+        const float thr_kt = table ? thr[kt] : 0.0f;
+        // slice estimate t = log2(z) c1 + c0 ~ ln(z / near) / log_a (log mode) or ln(z / near) S / log_fn (sliced):
+        // its error is orders of magnitude below one slice, so the slice is the estimate's floor k0 moved by at
+        // most one, decided by thr[k0] and thr[k0 + 1]; thr[k0 - 1] and thr[k0 + 2] confirm the move, and a pixel
+        // whose confirmation fails (or beyond the table, or of a frame without a staged table) takes the formula
+        const float c1 = fd.key_mode == VREN_KEYS_SLICED ? 0.69314718f * (float)fd.depth_slices / fd.log_fn
+                                                          : 0.69314718f / fd.log_a;
+        const float c0 = -__builtin_amdgcn_logf(fd.near_plane) * c1;
+        const float emax = (float)(kt > 0u ? kt - 1u : 0u); // k0 <= kt - 1, so every index below is <= kt
         uint32_t key[16];
+        uint32_t pend = 0u;
+        float dep[16];
+        uint2 nrm[16];
+        load_tile_pixels(fd, ti, tj, lane, dep, nrm);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
-            float dep[4];
-            uint2 nrm[4];
-            load_quad(fd, x0, y, dep, nrm);
 #pragma unroll
-            for (int q = 0; q < 4; q++) key[4 * j + q] = pixel_key(fd, ti, tj, x0 + q, y, dep[q], nrm[q]);
+            for (int q = 0; q < 4; q++) {
+                const int i = 4 * j + q;
+                const float z = pixel_view_z(fd, x0 + q, y, dep[i]);
+                uint32_t k = 0u;
+                if (z > 0.0f) {
+                    const float e = fminf(fmaxf(__builtin_amdgcn_logf(z) * c1 + c0, 0.0f), emax);
+                    const uint32_t k0 = table ? (uint32_t)e : 0u;
+                    const float vm = thr[k0 > 0u ? k0 - 1u : 0u], v0 = thr[k0], v1 = thr[k0 + 1u];
+                    const float v2 = thr[min(k0 + 2u, kt)];
+                    const bool dn = z < v0, up = z >= v1;
+                    k = k0 - (dn ? 1u : 0u) + (up ? 1u : 0u);
+                    const bool bad = !table || !(z < thr_kt) || (dn && (k0 == 0u || z < vm)) || (up && z >= v2);
+                    if (bad) pend |= 1u << i;
+                }
+                key[i] = assemble_key(fd, ti, tj, k, nrm[i]);
+            }
+        }
+        // the formula for the flagged pixels, their depth and normal read again (rare: beyond the table)
+        if (pend) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                if ((pend >> i) & 1u) {
+                    const uint32_t x = x0 + (i & 3), y = tj * 32 + (i >> 2) * 8 + (lane >> 3);
+                    const uint64_t p = (uint64_t)(y % fd.H) * fd.W + (x % fd.W);
+                    const float z = pixel_view_z(fd, x, y, fd.depth[p]);
+                    key[i] = assemble_key(fd, ti, tj, slice_exact(fd, z), reinterpret_cast<const uint2*>(fd.normals)[p]);
+                }
+            }
         }
         // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key); lanes
         // with no key left offer 0xFFFFFFFF, which only wins when some lane still holds that very key
@@ -196,6 +326,18 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
                 if (key[q] == m) rem &= ~(1u << q);
         }
         __builtin_amdgcn_wave_barrier();
+        // each pixel's index in the tile's unique list (binary search), found before the look-back so that it
+        // overlaps the wait for the previous tiles
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            uint32_t lo = 0, hi = u;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (uq[mid] <= key[i]) lo = mid;
+                else hi = mid;
+            }
+            key[i] = lo;
+        }
         // decoupled look-back over the frame's tiles (row-major order)
         const uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, u);
         for (uint32_t i = lane; i < u; i += VREN_WAVE) {
@@ -213,15 +355,7 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
             if (y >= fd.H) continue;
             uint32_t ref[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                uint32_t lo = 0, hi = u;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (uq[mid] <= key[4 * j + q]) lo = mid;
-                    else hi = mid;
-                }
-                ref[q] = prefix + lo;
-            }
+            for (int q = 0; q < 4; q++) ref[q] = prefix + key[4 * j + q];
             uint32_t* row = fd.refs + (uint64_t)y * fd.W;
             if (x0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + x0) & 15) == 0)) {
                 *reinterpret_cast<uint4*>(row + x0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
@@ -236,11 +370,14 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
 
 } // namespace
 
-void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t total_offs_tiles)
+void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t total_offs_tiles, uint32_t table_frames)
 {
     w->tickets = static_cast<uint32_t*>(mem);
     w->tile_status = reinterpret_cast<uint64_t*>(static_cast<char*>(mem) + 256);
     w->offs_status = w->tile_status + total_tiles;
+    w->slice_thr = table_frames ? reinterpret_cast<float*>(static_cast<char*>(mem) +
+                                                           chain_work_table_offset(total_tiles, total_offs_tiles))
+                                : nullptr;
     w->total_tiles = total_tiles;
     w->total_offs_tiles = total_offs_tiles;
 }
@@ -257,9 +394,12 @@ int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w)
 
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
+    if (!w.slice_thr) return vren_fail(ctx, VREN_EINVAL, "clusters: chain workspace has no slice table");
+    slice_table_kernel<<<F, 1024, 0, s>>>(d_frames, w.slice_thr);
+    VREN_LAUNCH_CHECK(ctx);
     const uint32_t blocks = (w.total_tiles + kClusterWaves - 1) / kClusterWaves;
     clusters_kernel<<<blocks, kClusterWaves * 64, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.total_tiles,
-                                                          w.tile_status, w.tickets + 0);
+                                                          w.tile_status, w.tickets + 0, w.slice_thr);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
@@ -295,9 +435,9 @@ extern "C" VREN_EXPORT int vren_find_unique_cluster_list(vren_ctx* ctx, vren_str
     ChainWork w;
     memset(&w, 0, sizeof(w));
     uint32_t tiles = f.Sx * f.Sy;
-    st = vren_ensure_scratch(ctx, chain_work_bytes(tiles, 0));
+    st = vren_ensure_scratch(ctx, chain_work_bytes(tiles, 0, 1));
     if (st) return st;
-    chain_work_bind(&w, ctx->scratch, tiles, 0);
+    chain_work_bind(&w, ctx->scratch, tiles, 0, 1);
     FrameDev* slot = vren_next_frame_slot(ctx);
     st = vren_upload_frames(ctx, s, slot, &f, 1);
     if (!st) st = launch_chain_init(ctx, s, w);

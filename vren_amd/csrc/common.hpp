@@ -19,6 +19,10 @@
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// index of the calling wave in its workgroup, as a wave-uniform value: the compiler then keeps what is derived
+// from it (tile and frame indices, frame-record fields) in scalar registers and reads such records with scalar loads
+__device__ __forceinline__ uint32_t wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / VREN_WAVE); }
+
 // popcount of mask bits below this lane
 __device__ __forceinline__ uint32_t mbcnt(uint64_t mask)
 {

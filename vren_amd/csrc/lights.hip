@@ -84,7 +84,7 @@ __device__ void block_sort_pass(const uint2* src, uint2* dst, uint32_t L, uint32
     }
 }
 
-__global__ __launch_bounds__(kLT) void lights_kernel(const FrameDev* frames)
+__global__ __launch_bounds__(kLT) void lights_kernel(const FrameDev* __restrict__ frames)
 {
     __shared__ uint2 s_a[kLdsCap];
     __shared__ uint2 s_b[kLdsCap];
@@ -212,7 +212,7 @@ __device__ __forceinline__ void store_node(vren_bvh_node* n, const float (&mn)[3
 }
 
 // leaves (init_light_array_bvh.comp:42-65) + the first parent level (build_bvh.comp:32-55)
-__global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* frames, uint32_t F, bool from_lights)
+__global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* __restrict__ frames, uint32_t F, bool from_lights)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     const uint32_t f = find_frame(frames, F, blockIdx.x * 256, [](const FrameDev& d) { return d.leaf_base; });
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* frames, u
 }
 
 // remaining parent levels, one workgroup per frame
-__global__ __launch_bounds__(1024) void bvh_upper_kernel(const FrameDev* frames)
+__global__ __launch_bounds__(1024) void bvh_upper_kernel(const FrameDev* __restrict__ frames)
 {
     const FrameDev& fd = frames[blockIdx.x];
     const uint32_t padded = fd.bvh_padded;

@@ -219,7 +219,7 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
 // shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS, and then
 // shades each pixel from LDS instead of chasing index -> light through global memory per light.
 template <int PPT>
-__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* frames, const uint32_t* tile_frame)
+__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__ frames, const uint32_t* __restrict__ tile_frame)
 {
     constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
     __shared__ float4 s_lpos[kStage];
@@ -447,7 +447,7 @@ __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm&
     gl[6] += g_att * datt_dx * (-t.x / t.r);
 }
 
-__global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* frames, uint32_t F, const float* grad_out,
+__global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __restrict__ frames, uint32_t F, const float* grad_out,
                                                              float* g_pos, float* g_light, float* g_ar)
 {
     __shared__ float4 s_lpos[kStage];
