@@ -18,6 +18,8 @@ struct ChainWork {
     uint64_t* offs_status; // light-offset scan: one word per 4096 clusters of capacity of every frame
     uint32_t* tickets;     // [0] cluster tiles, [1] offset tiles
     float* slice_thr;      // depth-slice threshold table of every frame (VREN_SLICE_TABLE floats each)
+    uint2* tile_cl;        // per 32x32 tile: (first cluster index, cluster count), written by the clusters kernel
+    uint2* tile_range;     // per 32x32 tile: [begin, end) of the index-buffer entries of its clusters' light lists
     const uint32_t* tile_frame; // frame of every flattened 32x32 tile (null for a one-frame table)
     uint32_t total_tiles;
     uint32_t total_offs_tiles;
@@ -39,11 +41,12 @@ inline void chain_mark(const ChainWork& w, hipStream_t s)
 // floats per frame of the depth-slice threshold table (clusters.hip)
 #define VREN_SLICE_TABLE 1024u
 
-// workspace: 256 bytes of tickets, the look-back statuses, then (256-aligned) the slice tables of table_frames
-// frames
+// workspace: 256 bytes of tickets, the look-back statuses, the per-tile cluster spans and list ranges, then
+// (256-aligned) the slice tables of table_frames frames
 inline size_t chain_work_table_offset(uint32_t total_tiles, uint32_t total_offs_tiles)
 {
-    return (256 + ((size_t)total_tiles + total_offs_tiles) * sizeof(uint64_t) + 255) & ~(size_t)255;
+    return (256 + ((size_t)total_tiles + total_offs_tiles) * sizeof(uint64_t) + (size_t)total_tiles * 16 + 255) &
+           ~(size_t)255;
 }
 inline size_t chain_work_bytes(uint32_t total_tiles, uint32_t total_offs_tiles, uint32_t table_frames)
 {

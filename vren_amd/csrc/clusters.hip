@@ -216,7 +216,8 @@ __device__ __forceinline__ void load_tile_pixels(const FrameDev& fd, uint32_t ti
 __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* __restrict__ frames,
                                                                        const uint32_t* __restrict__ tile_frame,
                                                                        uint32_t total_tiles, uint64_t* tile_status,
-                                                                       uint32_t* ticket, const float* __restrict__ slice_thr)
+                                                                       uint32_t* ticket, const float* __restrict__ slice_thr,
+                                                                       uint2* __restrict__ tile_cl)
 {
     __shared__ uint32_t s_unique[kClusterWaves][1024];
     __shared__ float s_thr[2][VREN_SLICE_TABLE];
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
         }
         // decoupled look-back over the frame's tiles (row-major order)
         const uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, u);
+        if (lane == 0) tile_cl[g] = make_uint2(prefix, u);
         for (uint32_t i = lane; i < u; i += VREN_WAVE) {
             const uint32_t idx = prefix + i;
             if (idx < fd.key_cap) fd.keys[idx] = uq[i];
@@ -375,6 +377,8 @@ void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t tot
     w->tickets = static_cast<uint32_t*>(mem);
     w->tile_status = reinterpret_cast<uint64_t*>(static_cast<char*>(mem) + 256);
     w->offs_status = w->tile_status + total_tiles;
+    w->tile_cl = reinterpret_cast<uint2*>(w->offs_status + total_offs_tiles);
+    w->tile_range = w->tile_cl + total_tiles;
     w->slice_thr = table_frames ? reinterpret_cast<float*>(static_cast<char*>(mem) +
                                                            chain_work_table_offset(total_tiles, total_offs_tiles))
                                 : nullptr;
@@ -399,7 +403,7 @@ int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint
     VREN_LAUNCH_CHECK(ctx);
     const uint32_t blocks = (w.total_tiles + kClusterWaves - 1) / kClusterWaves;
     clusters_kernel<<<blocks, kClusterWaves * 64, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.total_tiles,
-                                                          w.tile_status, w.tickets + 0, w.slice_thr);
+                                                          w.tile_status, w.tickets + 0, w.slice_thr, w.tile_cl);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

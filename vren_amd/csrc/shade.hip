@@ -213,25 +213,62 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     reinterpret_cast<float4*>(fd.out)[p] = o;
 }
 
+// [begin, end) of the index-buffer entries of a tile's light lists: the tile's clusters are the consecutive indices
+// [c0, c0 + u) (clusters kernel) and their lists are consecutive in the index buffer, so the range runs from the
+// first cluster's offset to the end of the last list (both clamped to the capacities like the per-pixel lists).
+__global__ __launch_bounds__(256) void tile_range_kernel(const FrameDev* __restrict__ frames,
+                                                         const uint32_t* __restrict__ tile_frame, uint32_t total_tiles,
+                                                         const uint2* __restrict__ tile_cl, uint2* __restrict__ tile_range)
+{
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= total_tiles) return;
+    const FrameDev& fd = frames[frame_of_tile(tile_frame, g)];
+    const uint2 c = tile_cl[g];
+    uint32_t a = 0, b = 0;
+    if (fd.L > 0 && c.y > 0 && c.x < fd.key_cap) {
+        const uint32_t c1 = min(c.x + c.y, fd.key_cap) - 1u;
+        a = min(fd.offsets[c.x], fd.index_cap);
+        const uint64_t e = (uint64_t)fd.offsets[c1] + fd.counts[c1];
+        b = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+    }
+    tile_range[g] = make_uint2(a, b);
+}
+
 // A 256-thread workgroup shades PPT 16x16 quadrants of a 32x32 tile (PPT pixels per thread, their loads issued
-// together), flattened over every frame of the table. The clusters of a tile are numbered consecutively and their
-// light lists are consecutive in the index buffer, so the workgroup first reduces the range of list entries its
-// shaded pixels reference, stages that contiguous run of light records (position, colour/radius) in LDS, and then
-// shades each pixel from LDS instead of chasing index -> light through global memory per light.
+// together), flattened over every frame of the table. Every pixel of a tile reads its light list from one
+// contiguous run of the index buffer, which the workgroup stages in LDS as light records (position with 1 / radius,
+// colour with radius) before shading each pixel from LDS. With tile_range (the chain, whose references are the
+// clusters kernel's), the run is known from one load, so the staged indices are requested beside the pixel data and
+// the light records beside the pixels' list bounds: two dependent rounds of loads. Without it (references from the
+// caller), the workgroup first reduces the run from its pixels' lists.
 template <int PPT>
-__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__ frames, const uint32_t* __restrict__ tile_frame)
+__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__ frames,
+                                                    const uint32_t* __restrict__ tile_frame,
+                                                    const uint2* __restrict__ tile_range)
 {
     constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
+    static_assert(kStage == 2 * 256, "two staged entries per thread");
     __shared__ float4 s_lpos[kStage];
     __shared__ float4 s_lcol[kStage];
     __shared__ uint32_t s_min[4], s_max[4];
-    __shared__ uint32_t s_start, s_n, s_staged;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / BPT, sub = blockIdx.x % BPT;
     const uint32_t f = frame_of_tile(tile_frame, g);
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
+    uint32_t start = 0, nstage = 0;
+    uint32_t li[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+    if (tile_range) {
+        const uint2 rg = tile_range[g];
+        start = rg.x;
+        nstage = rg.y > rg.x ? rg.y - rg.x : 0u;
+        if (nstage <= kStage) {
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                if (tid + 256u * k < nstage) li[k] = fd.indices[start + tid + 256u * k];
+        }
+    }
     PixelIn px[PPT];
     uint2 nb[PPT];
     float4 ar[PPT];
@@ -260,10 +297,9 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
             px[q] = reconstruct(fd, gx, gy, depth);
         }
     }
-    // each pixel's light list, then the workgroup's range of list entries
+    // each pixel's light list
     bool has_list[PPT];
     uint32_t my_off[PPT], my_end[PPT];
-    uint32_t rmin = 0xFFFFFFFFu, rmax = 0u;
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         has_list[q] = px[q].shaded && fd.L > 0 && cl_raw[q] < fd.key_cap;
@@ -273,47 +309,54 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
             my_off[q] = min(fd.offsets[cl_raw[q]], fd.index_cap);
             const uint64_t e = (uint64_t)fd.offsets[cl_raw[q]] + fd.counts[cl_raw[q]];
             my_end[q] = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
-            rmin = min(rmin, my_off[q]);
-            rmax = max(rmax, my_end[q]);
         }
     }
+    if (!tile_range) {
+        // the workgroup's range of list entries, reduced from its pixels' lists
+        uint32_t rmin = 0xFFFFFFFFu, rmax = 0u;
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        rmin = min(rmin, shfl_xor(rmin, m));
-        rmax = max(rmax, shfl_xor(rmax, m));
-    }
-    if (lane == 0) {
-        s_min[wave] = rmin;
-        s_max[wave] = rmax;
-    }
-    __syncthreads();
-    {
+        for (int q = 0; q < PPT; q++) {
+            if (has_list[q]) {
+                rmin = min(rmin, my_off[q]);
+                rmax = max(rmax, my_end[q]);
+            }
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            rmin = min(rmin, shfl_xor(rmin, m));
+            rmax = max(rmax, shfl_xor(rmax, m));
+        }
+        if (lane == 0) {
+            s_min[wave] = rmin;
+            s_max[wave] = rmax;
+        }
+        __syncthreads();
         const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
         const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
-        const uint32_t n = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
-        if (tid == 0) {
-            s_start = a == 0xFFFFFFFFu ? 0u : a;
-            s_n = n;
-            s_staged = n <= kStage ? 1u : 0u;
+        start = a == 0xFFFFFFFFu ? 0u : a;
+        nstage = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
+        if (nstage <= kStage) {
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                if (tid + 256u * k < nstage) li[k] = fd.indices[start + tid + 256u * k];
         }
     }
-    __syncthreads();
-    const uint32_t start = s_start, nstage = s_n;
-    const bool staged = s_staged != 0;
+    const bool staged = nstage <= kStage;
     if (staged) {
         const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
         const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
-        for (uint32_t i = tid; i < nstage; i += 256) {
-            const uint32_t li = fd.indices[start + i];
-            if (li < fd.L) {
-                const float4 c = lcol[li];
-                float4 q = lpos[li];
-                q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const uint32_t i = tid + 256u * k;
+            if (i < nstage) {
+                float4 q = make_float4(0.f, 0.f, 0.f, 0.f), c = q; // radius 0: never contributes
+                if (li[k] < fd.L) {
+                    c = lcol[li[k]];
+                    q = lpos[li[k]];
+                    q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
+                }
                 s_lpos[i] = q;
                 s_lcol[i] = c;
-            } else {
-                s_lcol[i] = make_float4(0.f, 0.f, 0.f, 0.f); // radius 0: never contributes
-                s_lpos[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
     }
@@ -630,7 +673,13 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
 
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr);
+    if (w.tile_range) {
+        tile_range_kernel<<<(w.total_tiles + 255) / 256, 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr,
+                                                                    w.total_tiles, w.tile_cl, w.tile_range);
+        VREN_LAUNCH_CHECK(ctx);
+    }
+    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr,
+                                                                           w.tile_range);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
