@@ -113,9 +113,10 @@ __device__ __forceinline__ bool test_leaf(const Box& c, const float4& o, float l
     return vren_sqrtf(vren_dot3(d, d)) < r;
 }
 
-// Traverse the BVH for one cluster with the whole wave. MODE 0 counts hits, MODE 1 sets bits in `bitmask`.
+// Traverse the BVH for one cluster with the whole wave and return the number of hit lights. MODE 0 also writes the
+// first VREN_LIST_SLOT hit light indices (in traversal order) to `out`, MODE 1 sets one bit per hit light in `out`.
 template <int MODE>
-__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* bitmask)
+__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* out)
 {
     const uint32_t lane = lane_id();
     const uint32_t sub = lane & 31u;
@@ -166,11 +167,15 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
                 if (next == VREN_BVH_LEAF_NODE && at_leaf && li < fd.L) {
                     if (!fd.leaf_pos) lp = vpos[li];
                     leaf_hit = test_leaf(box, lp, lo.x, hi.x);
-                    if (MODE == 1 && leaf_hit) atomicOr(&bitmask[li >> 5], 1u << (li & 31u));
+                    if (MODE == 1 && leaf_hit) atomicOr(&out[li >> 5], 1u << (li & 31u));
                 }
             } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {
                 push = test_aabb_aabb(box, lo, hi);
                 child = next | ((level + 1) << kLevelShift);
+            }
+            if (MODE == 0 && leaf_hit) {
+                const uint32_t pos = hits + mbcnt(ballot(true));
+                if (pos < VREN_LIST_SLOT) out[pos] = li;
             }
         }
         hits += (uint32_t)__popcll(ballot(leaf_hit));
@@ -186,11 +191,108 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
     return hits;
 }
 
-// Clusters of every frame of the table are processed as one flattened range: each wave loads the device-side
-// cluster counts of 64 frames at once (one per lane) and scans them, instead of walking the frames one dependent
-// load after another. Calls fn(fd, cluster) for the wave's share (clusters gw, gw + stride, ...).
-template <typename Fn>
-__device__ __forceinline__ void for_each_cluster(const FrameDev* frames, uint32_t F, uint32_t gw, uint32_t stride, Fn fn)
+// The BVH of a frame as one half-wave's traversal needs it (per lane, so the two halves may serve two frames)
+struct TravFrame {
+    const vren_bvh_node* bvh;
+    const uint2* codes;
+    const float4* leaf_pos;
+    const float4* vpos;
+    uint32_t L, leaf_level, root, length;
+};
+
+__device__ __forceinline__ TravFrame trav_frame(const FrameDev& fd)
+{
+    return TravFrame{fd.bvh, reinterpret_cast<const uint2*>(fd.codes), reinterpret_cast<const float4*>(fd.leaf_pos),
+                     reinterpret_cast<const float4*>(fd.view_pos), fd.L, fd.bvh_levels - 1, fd.bvh_root, fd.bvh_length};
+}
+
+// Two traversals at once, one per 32-lane half of the wave (lanes 0-31 and 32-63, each with its own frame, cluster
+// box, stack and hit list; a half with live = false does nothing). Each step pops one 32-node block per half and
+// tests its children with the half's 32 lanes, so the dependent node loads of two clusters overlap. Counts the hit
+// lights of each half (returned to every lane of the half) and keeps the first VREN_LIST_SLOT hit light indices of
+// each half in `list` (traversal order). Same node tests and visiting rules as traverse<0>.
+__device__ uint32_t traverse_halves(const TravFrame& tf, const Box& box, bool live, uint32_t* stack, uint32_t* list)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t sub = lane & 31u;
+    const uint64_t half = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+    // level 0 = the root's children (assign_lights.comp:108-119): the first child of the root
+    uint32_t sp = 0;
+    if (live) {
+        const uint32_t root_first = __float_as_uint(reinterpret_cast<const float4*>(&tf.bvh[tf.root])[0].w);
+        if (root_first != VREN_BVH_INVALID_NODE) {
+            if (sub == 0) stack[0] = root_first;
+            sp = 1;
+        }
+    }
+    uint32_t hits = 0;
+    __builtin_amdgcn_wave_barrier();
+    while (ballot(sp > 0)) {
+        uint32_t e = 0xFFFFFFFFu;
+        if (sp > 0) {
+            e = stack[sp - 1];
+            sp -= 1;
+        }
+        // entries always address a whole 32-node block inside the BVH; anything else is ignored
+        const bool active = e != 0xFFFFFFFFu && (e >> kLevelShift) <= tf.leaf_level &&
+                            (e & kAddrMask) + 32u <= tf.length;
+        bool leaf_hit = false, push = false;
+        uint32_t child = 0, li = 0xFFFFFFFFu;
+        if (active) {
+            const uint32_t level = e >> kLevelShift;
+            const uint32_t addr = (e & kAddrMask) + sub;
+            const bool at_leaf = level == tf.leaf_level && addr < tf.L;
+            float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (at_leaf) {
+                li = tf.codes[addr].y;
+                if (tf.leaf_pos) lp = tf.leaf_pos[addr];
+            }
+            const float4* q = reinterpret_cast<const float4*>(&tf.bvh[addr]);
+            const float4 lo = q[0], hi = q[1];
+            const uint32_t next = __float_as_uint(lo.w);
+            if (level == tf.leaf_level) {
+                if (next == VREN_BVH_LEAF_NODE && at_leaf && li < tf.L) {
+                    if (!tf.leaf_pos) lp = tf.vpos[li];
+                    leaf_hit = test_leaf(box, lp, lo.x, hi.x);
+                }
+            } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {
+                push = test_aabb_aabb(box, lo, hi);
+                child = next | ((level + 1) << kLevelShift);
+            }
+        }
+        const uint64_t hm = ballot(leaf_hit) & half;
+        if (leaf_hit) {
+            const uint32_t pos = hits + mbcnt(hm);
+            if (pos < VREN_LIST_SLOT) list[pos] = li;
+        }
+        hits += (uint32_t)__popcll(hm);
+        const uint64_t pm = ballot(push) & half;
+        if (push) {
+            const uint32_t r = mbcnt(pm);
+            if (sp + r < kStack) stack[sp + r] = child;
+        }
+        sp = min(sp + (uint32_t)__popcll(pm), kStack); // the level bound keeps it far below kStack
+        __builtin_amdgcn_wave_barrier();
+    }
+    return hits;
+}
+
+// Clusters per chunk: about four chunks per wave, strided over the whole range, so that every wave samples every
+// part of the frames (the costly clusters, those covering many lights, sit together in the image), at most 64
+// (one per lane).
+__device__ __forceinline__ uint32_t chunk_size(uint32_t total, uint32_t waves)
+{
+    return max(1u, min(VREN_WAVE, (uint32_t)(((uint64_t)total + 4ull * waves - 1) / (4ull * waves))));
+}
+
+// Clusters of every frame of the table are processed as one flattened range, in chunks of at most 64 consecutive
+// clusters per wave: lane i takes cluster i of the chunk, finds its frame among 64 frames at a time (their
+// device-side cluster counts, one per lane, prefix-summed) and computes that cluster's per-lane prologue `pre` (the
+// cluster AABB, list bounds), so work that is uniform per cluster is done for a whole chunk at once. fn(fd, c, k, v)
+// then runs for each cluster k of the chunk with the whole wave (v = lane k's prologue, read with readlane).
+template <typename T, typename Pre, typename Fn>
+__device__ __forceinline__ void for_each_cluster_chunk(const FrameDev* __restrict__ frames, uint32_t F, uint32_t gw,
+                                                       uint32_t stride, Pre pre, Fn fn)
 {
     const uint32_t lane = lane_id();
     for (uint32_t f0 = 0; f0 < F; f0 += VREN_WAVE) {
@@ -201,14 +303,74 @@ __device__ __forceinline__ void for_each_cluster(const FrameDev* frames, uint32_
             if (d.L != 0 && d.cluster_count != nullptr) nf = min(*d.cluster_count, d.key_cap);
         }
         const uint32_t incl = wave_inclusive_sum(nf);
+        const uint32_t excl = incl - nf;
         const uint32_t total = shfl(incl, VREN_WAVE - 1);
-        for (uint32_t c = gw; c < total; c += stride) {
-            const uint32_t fl = (uint32_t)__builtin_ctzll(ballot(incl > c));
-            const uint32_t base = shfl(incl - nf, (int)fl);
-            fn(frames[f0 + fl], c - base);
+                const uint32_t cs = chunk_size(total, stride);
+        const uint32_t chunks = (uint32_t)(((uint64_t)total + cs - 1) / cs);
+        for (uint32_t chunk = gw; chunk < chunks; chunk += stride) {
+            const uint32_t c0 = chunk * cs;
+            const uint32_t c = c0 + lane;
+            // frame of cluster c: the number of frames whose inclusive count is <= c (counts are non-decreasing)
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t step = VREN_WAVE / 2; step >= 1; step >>= 1)
+                if (shfl(incl, (int)(pos + step - 1)) <= c) pos += step;
+            const uint32_t fl = min(pos, VREN_WAVE - 1);
+            const uint32_t base = shfl(excl, (int)fl);
+            T v{};
+            if (lane < cs && c < total) v = pre(frames[f0 + fl], c - base);
+            const uint32_t n = min(total - c0, cs);
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t fk = __builtin_amdgcn_readlane(fl, k);
+                const uint32_t bk = __builtin_amdgcn_readlane(base, k);
+                fn(frames[f0 + fk], c0 + k - bk, k, v);
+            }
         }
     }
 }
+
+// for_each_cluster_chunk for the paired count pass: pre(fd, c, frame) runs per lane (lane i: cluster i of the
+// chunk), then fn(k, n, cA, cB) for k = 0, 2, 4, ... with cA, cB the frame-local indices of clusters k and k + 1
+template <typename Pre, typename Fn>
+__device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restrict__ frames, uint32_t F, uint32_t gw,
+                                                        uint32_t stride, Pre pre, Fn fn)
+{
+    const uint32_t lane = lane_id();
+    for (uint32_t f0 = 0; f0 < F; f0 += VREN_WAVE) {
+        const uint32_t fi = f0 + lane;
+        uint32_t nf = 0;
+        if (fi < F) {
+            const FrameDev& d = frames[fi];
+            if (d.L != 0 && d.cluster_count != nullptr) nf = min(*d.cluster_count, d.key_cap);
+        }
+        const uint32_t incl = wave_inclusive_sum(nf);
+        const uint32_t excl = incl - nf;
+        const uint32_t total = shfl(incl, VREN_WAVE - 1);
+        const uint32_t cs = chunk_size(total, stride);
+        const uint32_t chunks = (uint32_t)(((uint64_t)total + cs - 1) / cs);
+        for (uint32_t chunk = gw; chunk < chunks; chunk += stride) {
+            const uint32_t c0 = chunk * cs;
+            const uint32_t c = c0 + lane;
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t step = VREN_WAVE / 2; step >= 1; step >>= 1)
+                if (shfl(incl, (int)(pos + step - 1)) <= c) pos += step;
+            const uint32_t fl = min(pos, VREN_WAVE - 1);
+            const uint32_t base = shfl(excl, (int)fl);
+            if (lane < cs && c < total) pre(frames[f0 + fl], c - base, f0 + fl);
+            const uint32_t local = c - base;
+            const uint32_t n = min(total - c0, cs);
+            for (uint32_t k = 0; k < n; k += 2)
+                fn(k, n, __builtin_amdgcn_readlane(local, k), __builtin_amdgcn_readlane(local, min(k + 1, n - 1)));
+        }
+    }
+}
+
+__device__ __forceinline__ float readlane_f(float x, uint32_t k)
+{
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x), k));
+}
+
 
 // Emits the lights set in a wave's bitmask in ascending index to dst[0 ..) (nothing when dst is null), clearing
 // the mask. Returns the number emitted.
@@ -236,29 +398,51 @@ __device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t wor
     return out;
 }
 
-// COUNT (assign_lights.comp, count pass): the traversal marks the hit lights in the wave's LDS bitmask; the count
-// is stored, and a list of at most VREN_LIST_SLOT lights is kept in the cluster's slot (ascending), so the writing
+// COUNT (assign_lights.comp, count pass): two clusters per wave at a time (one per half, traverse_halves), each
+// counting its hit lights and keeping the first VREN_LIST_SLOT of them in the half's LDS list; a list of at most
+// VREN_LIST_SLOT lights is sorted (the rank of each index among the list) into the cluster's slot, so the writing
 // pass copies it instead of traversing again.
-__global__ void assign_count_kernel(const FrameDev* __restrict__ frames, uint32_t F, uint32_t mask_words)
+__global__ __launch_bounds__(256, 8) void assign_count_kernel(const FrameDev* __restrict__ frames, uint32_t F,
+                                                             uint32_t mask_words)
 {
     extern __shared__ uint32_t s_mem[];
     const uint32_t wpb = blockDim.x / VREN_WAVE;
     const uint32_t wave = wave_id();
-    uint32_t* stack = s_mem + wave * kStack;
-    uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
-    for (uint32_t i = lane_id(); i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
-    __builtin_amdgcn_wave_barrier();
+    const uint32_t lane = lane_id();
+    const uint32_t hb = lane < 32 ? 0u : 1u; // half of the wave
+    uint32_t* stack = s_mem + (wave * 2 + hb) * kStack;
+    uint32_t* list = s_mem + wpb * 2 * kStack + (wave * 2 + hb) * VREN_LIST_SLOT;
+    Box* boxes = reinterpret_cast<Box*>(s_mem + wpb * 2 * (kStack + VREN_LIST_SLOT)) + wave * VREN_WAVE;
+    uint32_t* cl_frame = s_mem + wpb * 2 * (kStack + VREN_LIST_SLOT) + wpb * VREN_WAVE * 6 + wave * VREN_WAVE;
     const uint32_t gw = blockIdx.x * wpb + wave;
-    for_each_cluster(frames, F, gw, gridDim.x * wpb, [&](const FrameDev& fd, uint32_t c) {
-        const Box box = cluster_aabb(fd, fd.keys[c]);
-        const uint32_t h = traverse<1>(fd, box, stack, bitmask);
-        __builtin_amdgcn_wave_barrier();
-        const bool keep = fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT;
-        emit_bitmask(bitmask, (fd.L + 31) / 32, keep ? fd.list_slots + (size_t)c * VREN_LIST_SLOT : nullptr,
-                     VREN_LIST_SLOT);
-        if (lane_id() == 0) fd.counts[c] = h;
-        __builtin_amdgcn_wave_barrier();
-    });
+    // the chunk's cluster AABBs and frames wait in LDS (one per lane) while the wave traverses two at a time
+    for_each_cluster_chunk2(
+        frames, F, gw, gridDim.x * wpb,
+        [&](const FrameDev& fdl, uint32_t c, uint32_t f) {
+            boxes[lane] = cluster_aabb(fdl, fdl.keys[c]);
+            cl_frame[lane] = f;
+        },
+        [&](uint32_t k, uint32_t n, uint32_t cA, uint32_t cB) {
+            // half 0: cluster k of the chunk, half 1: cluster k + 1 (when the chunk has one)
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t kk = k + hb;
+            const bool live = kk < n;
+            const uint32_t c = hb ? cB : cA;
+            const FrameDev& fd = frames[cl_frame[live ? kk : k]];
+            const Box box = boxes[live ? kk : k];
+            const uint32_t h = traverse_halves(trav_frame(fd), box, live, stack, list);
+            __builtin_amdgcn_wave_barrier();
+            if (live && fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT) {
+                // ascending light index: each entry's rank among the (distinct) entries of the half's list
+                const uint32_t sub = lane & 31u;
+                const uint32_t li = sub < h ? list[sub] : 0xFFFFFFFFu;
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < h; j++) rank += list[j] < li ? 1u : 0u;
+                if (sub < h) fd.list_slots[(size_t)c * VREN_LIST_SLOT + rank] = li;
+            }
+            if (live && (lane & 31u) == 0) fd.counts[c] = h;
+            __builtin_amdgcn_wave_barrier();
+        });
 }
 
 // exclusive scan of counts[0 .. min(cluster_count, key_cap)) -> offsets and assigned_total: one workgroup per
@@ -295,7 +479,13 @@ __global__ __launch_bounds__(1024) void assign_offsets_kernel(const FrameDev* __
     }
 }
 
-__global__ void assign_write_kernel(const FrameDev* __restrict__ frames, uint32_t F, uint32_t mask_words)
+struct ListBounds {
+    uint32_t count, offset;
+};
+
+// WRITE: each cluster's list at its offset: copied from the slot the counting pass kept, or traversed again
+// (bitmask, emitted in ascending light index) when it was longer than a slot
+__global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __restrict__ frames, uint32_t F, uint32_t mask_words)
 {
     extern __shared__ uint32_t s_mem[];
     const uint32_t wpb = blockDim.x / VREN_WAVE;
@@ -306,23 +496,26 @@ __global__ void assign_write_kernel(const FrameDev* __restrict__ frames, uint32_
     for (uint32_t i = lane; i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
     __builtin_amdgcn_wave_barrier();
     const uint32_t gw = blockIdx.x * wpb + wave;
-    for_each_cluster(frames, F, gw, gridDim.x * wpb, [&](const FrameDev& fd, uint32_t c) {
-        const uint32_t cnt = fd.counts[c];
-        const uint32_t out = fd.offsets[c];
-        if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
-            // the counting pass kept this list
-            if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
-                fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
-            return;
-        }
-        const Box box = cluster_aabb(fd, fd.keys[c]);
-        traverse<1>(fd, box, stack, bitmask);
-        __builtin_amdgcn_wave_barrier();
-        // emit set bits in ascending light index, clearing the mask as we go
-        const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
-        emit_bitmask(bitmask, (fd.L + 31) / 32, fd.indices + out, room);
-        __builtin_amdgcn_wave_barrier();
-    });
+    for_each_cluster_chunk<ListBounds>(
+        frames, F, gw, gridDim.x * wpb,
+        [&](const FrameDev& fdl, uint32_t c) { return ListBounds{fdl.counts[c], fdl.offsets[c]}; },
+        [&](const FrameDev& fd, uint32_t c, uint32_t k, const ListBounds& mine) {
+            const uint32_t cnt = __builtin_amdgcn_readlane(mine.count, k);
+            const uint32_t out = __builtin_amdgcn_readlane(mine.offset, k);
+            if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
+                // the counting pass kept this list
+                if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
+                    fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
+                return;
+            }
+            const Box box = cluster_aabb(fd, fd.keys[c]);
+            traverse<1>(fd, box, stack, bitmask);
+            __builtin_amdgcn_wave_barrier();
+            // emit set bits in ascending light index, clearing the mask as we go
+            const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
+            emit_bitmask(bitmask, (fd.L + 31) / 32, fd.indices + out, room);
+            __builtin_amdgcn_wave_barrier();
+        });
 }
 
 // enough resident waves per CU (up to 32) to hide the dependent node loads of the traversals
@@ -338,7 +531,9 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     uint32_t wpb = 4;
     while (wpb > 1 && per_wave * wpb > 150 * 1024) wpb--;
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
-    assign_count_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
+    // two stacks and two lists per wave (one per half), the chunk's boxes and cluster frames
+    const size_t count_per_wave = (size_t)2 * (kStack + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
+    assign_count_kernel<<<persistent_blocks(ctx, 4), 4 * VREN_WAVE, count_per_wave * 4, s>>>(d_frames, F, mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     assign_offsets_kernel<<<F, 1024, 0, s>>>(d_frames);
