@@ -138,7 +138,7 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     ms = sorted(times)[len(times) // 2]
     achieved = SORT_BYTES_PER_PAIR * n / (ms * 1e-3) / 1e9
     configs = ctx.radix_configs()
-    cfg = configs[cfg_id] if cfg_id is not None else (8, 23, 8, 0)  # the library's untuned default
+    cfg = configs[cfg_id] if cfg_id is not None else (8, 23, 16, 0)  # the library's untuned default
     bits, kpt, waves, algo = cfg
     passes = (32 + bits - 1) // bits
     traffic = profiled_traffic()

@@ -14,8 +14,9 @@
 
 namespace {
 
-// defaults when nothing is tuned or forced: 8-bit digits, 23 keys per thread, 8 waves (the fastest on MI355X at 2^26)
-constexpr int kDefaultBits = 8, kDefaultKpt = 23, kDefaultWaves = 8, kDefaultAlgorithm = 0;
+// defaults when nothing is tuned or forced: 8-bit digits, 23 keys per thread, 16 waves, onesweep (the fastest on
+// MI355X at 2^26 key-value pairs)
+constexpr int kDefaultBits = 8, kDefaultKpt = 23, kDefaultWaves = 16, kDefaultAlgorithm = 0;
 
 uint64_t bucket_key(uint64_t n, bool vals, uint32_t key_bits)
 {
