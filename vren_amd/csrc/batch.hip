@@ -118,7 +118,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.bvh_length = L ? vren_calc_bvh_buffer_length(L) : 0;
         const uint32_t slots = L ? (f.bvh_padded + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN : 0;
         b->leaf_slots[i] = slots;
-        tile_base += tiles;
+        tile_base += (tiles + VREN_TILE_GROUP - 1) / VREN_TILE_GROUP * VREN_TILE_GROUP;
         leaf_base += slots;
         offs_base += (key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
         max_L = std::max(max_L, L);
@@ -148,8 +148,10 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     if (e == hipSuccess) e = hipMalloc(&b->d_tile_frame, sizeof(uint32_t) * std::max<uint32_t>(tile_base, 1));
     if (e == hipSuccess) {
         std::vector<uint32_t> tf(tile_base);
-        for (uint32_t i = 0; i < n_frames; i++)
-            for (uint32_t t = 0; t < b->host[i].Sx * b->host[i].Sy; t++) tf[b->host[i].tile_base + t] = i;
+        for (uint32_t i = 0; i < n_frames; i++) {
+            const uint32_t end = i + 1 < n_frames ? b->host[i + 1].tile_base : tile_base;
+            for (uint32_t t = b->host[i].tile_base; t < end; t++) tf[t] = i; // the frame's tiles and its padding
+        }
         e = hipMemcpy(b->d_tile_frame, tf.data(), sizeof(uint32_t) * tile_base, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {

@@ -10,6 +10,9 @@ int vren_upload_frames(vren_ctx* ctx, hipStream_t s, FrameDev* dst, const FrameD
 FrameDev* vren_next_frame_slot(vren_ctx* ctx);
 
 #define VREN_LEAF_ALIGN 256u
+// a frame's 32x32 tiles start at a multiple of this in the flattened tile numbering of a table (the clusters kernel
+// chains its look-back over groups of this many tiles, one workgroup each); tiles of the padding do nothing
+#define VREN_TILE_GROUP 8u
 #define VREN_SCAN_TILE 4096u
 
 // look-back statuses and ticket counters of one chain launch

@@ -4,8 +4,8 @@
 // of 16 pixels (depth slice from the D32 depth, cube-face normal code from the RGBA16F normal). The reference
 // sorts the tile's 1024 keys with a 55-step LDS bitonic network and 20 barriers; here the wave extracts the tile's
 // unique keys in ascending order by repeated wave64 minima (one round per distinct key -- a tile of a real scene
-// holds a handful), and the tile's unique count is chained to the previous tiles of the same frame by a decoupled
-// look-back. This replaces the reference's atomicAdd allocation (whose order across tiles is nondeterministic,
+// holds a handful), and the unique counts of a group of 8 tiles are chained to the previous groups of the same frame
+// by a decoupled look-back. This replaces the reference's atomicAdd allocation (whose order across tiles is nondeterministic,
 // find_unique_clusters.comp:100-103) by the canonical row-major tile order, and needs no separate scan dispatch.
 // Each pixel finds its key among the tile's unique keys by binary search and stores the frame-wide cluster index.
 //
@@ -206,73 +206,65 @@ __device__ __forceinline__ void load_tile_pixels(const FrameDev& fd, uint32_t ti
     }
 }
 
-// Tiles are flattened over every frame of the table (tile_base). A workgroup takes ONE ticket for kClusterWaves
-// consecutive tiles, one per wave -- a ticket on a single counter costs ~11 ns of serialised atomic throughput, so
-// one per tile would cap the kernel at ~90 M tiles/s. A wave waits (look-back) only on lower tiles: lower waves of
-// its workgroup, which run beside it, and tiles of lower tickets, whose workgroups are already running, so every
-// wait ends. (Giving a wave several tiles one after another would make the next workgroup's first tile wait for
-// this workgroup's last step, and serialise the grid.) Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8),
-// q, j = 0..3, of its tile.
+// Tiles are flattened over every frame of the table (tile_base, a multiple of VREN_TILE_GROUP). A workgroup takes
+// ONE ticket for a group of kClusterWaves consecutive tiles of one frame, one per wave -- a ticket on a single
+// counter costs ~11 ns of serialised atomic throughput, so one per tile would cap the kernel at ~90 M tiles/s.
+// The group's unique counts are summed in LDS and chained to the frame's previous groups by one decoupled
+// look-back per workgroup (per tile, the walk's round trips of ~1.4 us covered 64 tiles each and could not keep up
+// with the tiles finishing). The walk waits only on groups of lower tickets, whose workgroups are already running,
+// so every wait ends. Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8), q, j = 0..3, of its tile.
 __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* __restrict__ frames,
                                                                        const uint32_t* __restrict__ tile_frame,
                                                                        uint32_t total_tiles, uint64_t* tile_status,
                                                                        uint32_t* ticket, const float* __restrict__ slice_thr,
                                                                        uint2* __restrict__ tile_cl)
 {
+    static_assert(kClusterWaves == VREN_TILE_GROUP, "one workgroup per tile group");
+    static_assert(VREN_SLICE_TABLE == 2 * kClusterWaves * 64, "two table entries per thread");
     __shared__ uint32_t s_unique[kClusterWaves][1024];
-    __shared__ float s_thr[2][VREN_SLICE_TABLE];
-    __shared__ uint32_t s_t;
+    __shared__ float s_thr[VREN_SLICE_TABLE];
+    __shared__ uint32_t s_u[kClusterWaves];
+    __shared__ uint32_t s_t, s_prefix;
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
     if (threadIdx.x == 0) s_t = atomic_add_agent(ticket, 1u);
     __syncthreads();
     const uint32_t g0 = s_t * kClusterWaves;
     if (g0 >= total_tiles) return; // the whole workgroup
-    // the slice tables of the frames of the workgroup's first and last tile (the only frames of its tiles unless
-    // frames have fewer tiles than the workgroup has waves; a wave of any other frame evaluates the formula)
-    const uint32_t fa = frame_of_tile(tile_frame, g0);
-    const uint32_t fb = frame_of_tile(tile_frame, min(g0 + kClusterWaves, total_tiles) - 1);
+    // a workgroup's tiles are one group of its frame (frame tile ranges start at multiples of VREN_TILE_GROUP), so
+    // one slice table serves them all
+    const FrameDev& fd = frames[frame_of_tile(tile_frame, g0)];
+    const uint32_t kt = slice_table_kt(fd);
     {
-        // both halves of each table are requested before either is stored
-        static_assert(VREN_SLICE_TABLE == 2 * kClusterWaves * 64, "two table entries per thread");
-        const uint32_t kta = slice_table_kt(frames[fa]), ktb = slice_table_kt(frames[fb]);
+        // both halves of the table are requested before either is stored
         const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + kClusterWaves * 64;
-        const float* ta = slice_thr + (uint64_t)fa * VREN_SLICE_TABLE;
-        const float* tb = slice_thr + (uint64_t)fb * VREN_SLICE_TABLE;
-        const float a0 = i0 <= kta ? ta[i0] : 0.0f, a1 = i1 <= kta ? ta[i1] : 0.0f;
-        float b0 = 0.0f, b1 = 0.0f;
-        if (fb != fa) {
-            b0 = i0 <= ktb ? tb[i0] : 0.0f;
-            b1 = i1 <= ktb ? tb[i1] : 0.0f;
-        }
-        s_thr[0][i0] = a0;
-        s_thr[0][i1] = a1;
-        s_thr[1][i0] = b0;
-        s_thr[1][i1] = b1;
+        const float* tb = slice_thr + (uint64_t)frame_of_tile(tile_frame, g0) * VREN_SLICE_TABLE;
+        const float a0 = i0 <= kt ? tb[i0] : 0.0f, a1 = i1 <= kt ? tb[i1] : 0.0f;
+        s_thr[i0] = a0;
+        s_thr[i1] = a1;
     }
     __syncthreads();
     uint32_t* uq = s_unique[wave];
-    {
-        const uint32_t g = g0 + wave;
-        if (g >= total_tiles) return;
-        const uint32_t f = frame_of_tile(tile_frame, g);
-        const FrameDev& fd = frames[f];
-        const uint32_t t = g - fd.tile_base;
-        const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-        const uint32_t x0 = ti * 32 + (lane & 7) * 4;
-        const bool table = f == fa || f == fb;
-        const float* thr = s_thr[f == fa ? 0 : 1];
-        const uint32_t kt = slice_table_kt(fd);
-        const float thr_kt = table ? thr[kt] : 0.0f;
+    const uint32_t g = g0 + wave;
+    const uint32_t t0 = g0 - fd.tile_base; // the group's first tile in its frame
+    const uint32_t t = t0 + wave;
+    const uint32_t tiles = fd.Sx * fd.Sy;
+    const bool live = g < total_tiles && t < tiles; // not past the table nor on a frame's padding
+    const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
+    const uint32_t x0 = ti * 32 + (lane & 7) * 4;
+    uint32_t key[16];
+    uint32_t u = 0;
+    if (live) {
+        const float* thr = s_thr;
+        const float thr_kt = thr[kt];
         // slice estimate t = log2(z) c1 + c0 ~ ln(z / near) / log_a (log mode) or ln(z / near) S / log_fn (sliced):
         // its error is orders of magnitude below one slice, so the slice is the estimate's floor k0 moved by at
         // most one, decided by thr[k0] and thr[k0 + 1]; thr[k0 - 1] and thr[k0 + 2] confirm the move, and a pixel
-        // whose confirmation fails (or beyond the table, or of a frame without a staged table) takes the formula
+        // whose confirmation fails (or beyond the table) takes the formula
         const float c1 = fd.key_mode == VREN_KEYS_SLICED ? 0.69314718f * (float)fd.depth_slices / fd.log_fn
                                                           : 0.69314718f / fd.log_a;
         const float c0 = -__builtin_amdgcn_logf(fd.near_plane) * c1;
         const float emax = (float)(kt > 0u ? kt - 1u : 0u); // k0 <= kt - 1, so every index below is <= kt
-        uint32_t key[16];
         uint32_t pend = 0u;
         float dep[16];
         uint2 nrm[16];
@@ -287,12 +279,12 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
                 uint32_t k = 0u;
                 if (z > 0.0f) {
                     const float e = fminf(fmaxf(__builtin_amdgcn_logf(z) * c1 + c0, 0.0f), emax);
-                    const uint32_t k0 = table ? (uint32_t)e : 0u;
+                    const uint32_t k0 = (uint32_t)e;
                     const float vm = thr[k0 > 0u ? k0 - 1u : 0u], v0 = thr[k0], v1 = thr[k0 + 1u];
                     const float v2 = thr[min(k0 + 2u, kt)];
                     const bool dn = z < v0, up = z >= v1;
                     k = k0 - (dn ? 1u : 0u) + (up ? 1u : 0u);
-                    const bool bad = !table || !(z < thr_kt) || (dn && (k0 == 0u || z < vm)) || (up && z >= v2);
+                    const bool bad = !(z < thr_kt) || (dn && (k0 == 0u || z < vm)) || (up && z >= v2);
                     if (bad) pend |= 1u << i;
                 }
                 key[i] = assemble_key(fd, ti, tj, k, nrm[i]);
@@ -313,7 +305,6 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
         // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key); lanes
         // with no key left offer 0xFFFFFFFF, which only wins when some lane still holds that very key
         uint32_t rem = 0xFFFFu;
-        uint32_t u = 0;
         while (ballot(rem != 0u)) {
             uint32_t mine = 0xFFFFFFFFu;
 #pragma unroll
@@ -328,7 +319,7 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
         }
         __builtin_amdgcn_wave_barrier();
         // each pixel's index in the tile's unique list (binary search), found before the look-back so that it
-        // overlaps the wait for the previous tiles
+        // overlaps the wait for the previous groups
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             uint32_t lo = 0, hi = u;
@@ -339,33 +330,44 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
             }
             key[i] = lo;
         }
-        // decoupled look-back over the frame's tiles (row-major order)
-        const uint32_t prefix = wave_lookback(tile_status + fd.tile_base, t, u);
-        if (lane == 0) tile_cl[g] = make_uint2(prefix, u);
-        for (uint32_t i = lane; i < u; i += VREN_WAVE) {
-            const uint32_t idx = prefix + i;
-            if (idx < fd.key_cap) fd.keys[idx] = uq[i];
-        }
-        if (lane == 0 && t == fd.Sx * fd.Sy - 1) {
-            const uint32_t total = prefix + u;
-            *fd.cluster_count = total;
-            if (total > fd.key_cap) atomic_or_agent(fd.flags, VREN_FLAG_KEY_CAPACITY);
-        }
+    }
+    if (lane == 0) s_u[wave] = u;
+    __syncthreads();
+    // decoupled look-back over the frame's tile groups (row-major order), one walk per workgroup: a round trip of
+    // the walk covers 64 groups = 512 tiles
+    if (wave == 0) {
+        const uint32_t agg = wave_sum(lane < kClusterWaves ? s_u[lane] : 0u);
+        const uint32_t prefix = wave_lookback(tile_status + fd.tile_base / VREN_TILE_GROUP, t0 / VREN_TILE_GROUP, agg);
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    if (!live) return;
+    uint32_t prefix = s_prefix;
+    for (uint32_t w = 0; w < wave; w++) prefix += s_u[w];
+    if (lane == 0) tile_cl[g] = make_uint2(prefix, u);
+    for (uint32_t i = lane; i < u; i += VREN_WAVE) {
+        const uint32_t idx = prefix + i;
+        if (idx < fd.key_cap) fd.keys[idx] = uq[i];
+    }
+    if (lane == 0 && t == tiles - 1) {
+        const uint32_t total = prefix + u;
+        *fd.cluster_count = total;
+        if (total > fd.key_cap) atomic_or_agent(fd.flags, VREN_FLAG_KEY_CAPACITY);
+    }
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
-            if (y >= fd.H) continue;
-            uint32_t ref[4];
+    for (int j = 0; j < 4; j++) {
+        const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
+        if (y >= fd.H) continue;
+        uint32_t ref[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) ref[q] = prefix + key[4 * j + q];
-            uint32_t* row = fd.refs + (uint64_t)y * fd.W;
-            if (x0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + x0) & 15) == 0)) {
-                *reinterpret_cast<uint4*>(row + x0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-            } else {
+        for (int q = 0; q < 4; q++) ref[q] = prefix + key[4 * j + q];
+        uint32_t* row = fd.refs + (uint64_t)y * fd.W;
+        if (x0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + x0) & 15) == 0)) {
+            *reinterpret_cast<uint4*>(row + x0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+        } else {
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (x0 + q < fd.W) row[x0 + q] = ref[q];
-            }
+            for (int q = 0; q < 4; q++)
+                if (x0 + q < fd.W) row[x0 + q] = ref[q];
         }
     }
 }

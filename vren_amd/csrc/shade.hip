@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void tile_range_kernel(const FrameDev* __restr
     const FrameDev& fd = frames[frame_of_tile(tile_frame, g)];
     const uint2 c = tile_cl[g];
     uint32_t a = 0, b = 0;
-    if (fd.L > 0 && c.y > 0 && c.x < fd.key_cap) {
+    if (g - fd.tile_base < fd.Sx * fd.Sy && fd.L > 0 && c.y > 0 && c.x < fd.key_cap) { // not a padding tile
         const uint32_t c1 = min(c.x + c.y, fd.key_cap) - 1u;
         a = min(fd.offsets[c.x], fd.index_cap);
         const uint64_t e = (uint64_t)fd.offsets[c1] + fd.counts[c1];
@@ -256,6 +256,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
     const uint32_t f = frame_of_tile(tile_frame, g);
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
+    if (t >= fd.Sx * fd.Sy) return; // padding after the frame's tiles (the whole workgroup)
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
     uint32_t start = 0, nstage = 0;
     uint32_t li[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
@@ -504,6 +505,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
+    if (t >= fd.Sx * fd.Sy) return; // padding after the frame's tiles (the whole workgroup)
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
     const uint32_t gx = ti * 32 + (sub & 1) * 16 + (tid & 15), gy = tj * 32 + (sub >> 1) * 16 + (tid >> 4);
     const bool in = gx < fd.W && gy < fd.H;
