@@ -101,7 +101,21 @@ __global__ __launch_bounds__(1024) void slice_table_kernel(const FrameDev* __res
         thr[0] = 0.0f;
         return;
     }
+    // bisection over float bit patterns between lo (slice < k) and hi (slice >= k); it starts from +-1e-4 around
+    // the real threshold near * exp(k log_a) (log mode) or near * exp(k log_fn / S) (sliced), in double, far wider
+    // than the few ulps the float formula's rounding moves it, and from (0, +inf] when that bracket does not hold
+    const double e = fd.key_mode == VREN_KEYS_SLICED ? (double)k * (double)fd.log_fn / (double)fd.depth_slices
+                                                     : (double)k * (double)fd.log_a;
+    const double zk = (double)fd.near_plane * exp(e);
     uint32_t lo = 0u, hi = 0x7F800000u; // slice_exact(+0) = 0 < k <= slice_exact(+inf)
+    if (zk < 3.0e38) {
+        const uint32_t blo = __float_as_uint((float)(zk * (1.0 - 1e-4)));
+        const uint32_t bhi = __float_as_uint((float)(zk * (1.0 + 1e-4)));
+        if (slice_exact(fd, __uint_as_float(blo)) < k && slice_exact(fd, __uint_as_float(bhi)) >= k) {
+            lo = blo;
+            hi = bhi;
+        }
+    }
     while (hi - lo > 1u) {
         const uint32_t mid = lo + (hi - lo) / 2u;
         if (slice_exact(fd, __uint_as_float(mid)) >= k) hi = mid;
