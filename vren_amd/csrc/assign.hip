@@ -4,12 +4,13 @@
 // Reference: one wave32 per unique cluster (indirect dispatch), a DFS over the 32-ary BVH with a per-level
 // ballot stack, run twice (COUNT then WRITE) around a fixed 2^17-entry Blelloch scan of the counts; lists come
 // out in traversal order with reversed lanes inside each leaf block.
-// Here: persistent kernels, one wave64 per cluster, the wave tests the 32 children of TWO pending BVH blocks
-// at once (lanes 0-31 and 32-63) from a per-wave LDS stack, so no lane idles at the 32-wide fan-out; the
-// number of clusters is read on the device (it never returns to the host). COUNT counts leaf hits; one
-// workgroup per frame scans the counts into offsets; WRITE sets one bit per hit
-// light in a per-wave LDS bitmask and emits the list in ascending light index (the canonical order) by a
-// popcount prefix over the bitmask words.
+// Here: persistent kernels over the clusters of every frame (their number is read on the device; it never returns
+// to the host), taken in chunks of up to 64 whose per-cluster work (the cluster AABB, the list bounds) is done one
+// cluster per lane. COUNT runs two traversals per wave, one per 32-lane half (the BVH's fan-out), each from its own
+// LDS stack, and keeps lists of at most 16 lights, sorted by rank, in per-cluster slots; one workgroup per frame
+// scans the counts into offsets; WRITE copies the kept lists and traverses again only for longer ones, with the
+// whole wave testing two BVH blocks per step, one bit per hit light in a per-wave LDS bitmask, emitted in ascending
+// light index (the canonical order) by a popcount prefix over the bitmask words.
 // The cluster AABB and both tests are restated operation for operation (including the inverted y range of
 // the two-corner AABB); padding leaves and invalid nodes (empty AABB) never match.
 #include "chain.hpp"
@@ -113,10 +114,9 @@ __device__ __forceinline__ bool test_leaf(const Box& c, const float4& o, float l
     return vren_sqrtf(vren_dot3(d, d)) < r;
 }
 
-// Traverse the BVH for one cluster with the whole wave and return the number of hit lights. MODE 0 also writes the
-// first VREN_LIST_SLOT hit light indices (in traversal order) to `out`, MODE 1 sets one bit per hit light in `out`.
-template <int MODE>
-__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* out)
+// Traverse the BVH for one cluster with the whole wave, setting one bit per hit light in the wave's LDS bitmask, and
+// return the number of hit lights.
+__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* bitmask)
 {
     const uint32_t lane = lane_id();
     const uint32_t sub = lane & 31u;
@@ -167,15 +167,11 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
                 if (next == VREN_BVH_LEAF_NODE && at_leaf && li < fd.L) {
                     if (!fd.leaf_pos) lp = vpos[li];
                     leaf_hit = test_leaf(box, lp, lo.x, hi.x);
-                    if (MODE == 1 && leaf_hit) atomicOr(&out[li >> 5], 1u << (li & 31u));
+                    if (leaf_hit) atomicOr(&bitmask[li >> 5], 1u << (li & 31u));
                 }
             } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {
                 push = test_aabb_aabb(box, lo, hi);
                 child = next | ((level + 1) << kLevelShift);
-            }
-            if (MODE == 0 && leaf_hit) {
-                const uint32_t pos = hits + mbcnt(ballot(true));
-                if (pos < VREN_LIST_SLOT) out[pos] = li;
             }
         }
         hits += (uint32_t)__popcll(ballot(leaf_hit));
@@ -210,7 +206,7 @@ __device__ __forceinline__ TravFrame trav_frame(const FrameDev& fd)
 // box, stack and hit list; a half with live = false does nothing). Each step pops one 32-node block per half and
 // tests its children with the half's 32 lanes, so the dependent node loads of two clusters overlap. Counts the hit
 // lights of each half (returned to every lane of the half) and keeps the first VREN_LIST_SLOT hit light indices of
-// each half in `list` (traversal order). Same node tests and visiting rules as traverse<0>.
+// each half in `list` (traversal order). Same node tests and visiting rules as traverse.
 __device__ uint32_t traverse_halves(const TravFrame& tf, const Box& box, bool live, uint32_t* stack, uint32_t* list)
 {
     const uint32_t lane = lane_id();
@@ -366,10 +362,6 @@ __device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restri
     }
 }
 
-__device__ __forceinline__ float readlane_f(float x, uint32_t k)
-{
-    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x), k));
-}
 
 
 // Emits the lights set in a wave's bitmask in ascending index to dst[0 ..) (nothing when dst is null), clearing
@@ -509,7 +501,7 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
                 return;
             }
             const Box box = cluster_aabb(fd, fd.keys[c]);
-            traverse<1>(fd, box, stack, bitmask);
+            traverse(fd, box, stack, bitmask);
             __builtin_amdgcn_wave_barrier();
             // emit set bits in ascending light index, clearing the mask as we go
             const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;

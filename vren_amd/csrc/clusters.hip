@@ -182,36 +182,38 @@ typedef const __attribute__((address_space(1))) vf4 gvf4;
 typedef const __attribute__((address_space(1))) vu4 gvu4;
 typedef const __attribute__((address_space(1))) vu2 gvu2;
 
-// The 16 depths and normals of lane `lane` of a tile: pixel i = 4 j + q is (x0 + q, tj * 32 + 8 j + lane / 8).
-// A tile inside the frame with 16-byte aligned rows (uniform across the wave) takes one 16-byte depth load and two
-// 16-byte normal loads per row; other tiles load per pixel, wrapping (x mod W, y mod H) like the reference's
-// REPEAT sampler. Every load is issued before any is used.
-__device__ __forceinline__ void load_tile_pixels(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t lane,
-                                                 float dep[16], uint2 nrm[16])
+// The depths and normals of lane `lane` of a tile's rows 2 h and 2 h + 1 (pixels i = 4 j + q with j = 2 h + jj,
+// stored at [4 jj + q]): pixel (x0 + q, tj * 32 + 8 j + lane / 8). A tile inside the frame with 16-byte aligned
+// rows (uniform across the wave) takes one 16-byte depth load and two 16-byte normal loads per row; other tiles
+// load per pixel, wrapping (x mod W, y mod H) like the reference's REPEAT sampler. Every load of the half is issued
+// before any is used; a tile's two halves are loaded one after the other, which keeps the register peak of a wave
+// below 64 (eight waves per SIMD).
+__device__ __forceinline__ void load_tile_half(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t lane, uint32_t h,
+                                               float dep[8], uint2 nrm[8])
 {
     const uint32_t x0 = ti * 32 + (lane & 7) * 4;
     const bool interior = (ti + 1) * 32 <= fd.W && (tj + 1) * 32 <= fd.H && (fd.W & 3) == 0 &&
                           ((reinterpret_cast<uintptr_t>(fd.depth) | reinterpret_cast<uintptr_t>(fd.normals)) & 15) == 0;
     if (interior) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t p0 = (uint64_t)(tj * 32 + j * 8 + (lane >> 3)) * fd.W + x0;
+        for (int jj = 0; jj < 2; jj++) {
+            const uint64_t p0 = (uint64_t)(tj * 32 + (2 * h + jj) * 8 + (lane >> 3)) * fd.W + x0;
             const vf4 d4 = *(gvf4*)(fd.depth + p0);
             const vu4 n01 = *(gvu4*)(fd.normals + 4 * p0);
             const vu4 n23 = *(gvu4*)(fd.normals + 4 * p0 + 8);
-            dep[4 * j + 0] = d4.x;
-            dep[4 * j + 1] = d4.y;
-            dep[4 * j + 2] = d4.z;
-            dep[4 * j + 3] = d4.w;
-            nrm[4 * j + 0] = make_uint2(n01.x, n01.y);
-            nrm[4 * j + 1] = make_uint2(n01.z, n01.w);
-            nrm[4 * j + 2] = make_uint2(n23.x, n23.y);
-            nrm[4 * j + 3] = make_uint2(n23.z, n23.w);
+            dep[4 * jj + 0] = d4.x;
+            dep[4 * jj + 1] = d4.y;
+            dep[4 * jj + 2] = d4.z;
+            dep[4 * jj + 3] = d4.w;
+            nrm[4 * jj + 0] = make_uint2(n01.x, n01.y);
+            nrm[4 * jj + 1] = make_uint2(n01.z, n01.w);
+            nrm[4 * jj + 2] = make_uint2(n23.x, n23.y);
+            nrm[4 * jj + 3] = make_uint2(n23.z, n23.w);
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t x = x0 + (i & 3), y = tj * 32 + (i >> 2) * 8 + (lane >> 3);
+        for (int i = 0; i < 8; i++) {
+            const uint32_t x = x0 + (i & 3), y = tj * 32 + (2 * h + (i >> 2)) * 8 + (lane >> 3);
             const uint64_t p = (uint64_t)(y % fd.H) * fd.W + (x % fd.W);
             dep[i] = *(gfloat*)(fd.depth + p);
             const vu2 n = *(gvu2*)(fd.normals + 4 * p);
@@ -227,7 +229,7 @@ __device__ __forceinline__ void load_tile_pixels(const FrameDev& fd, uint32_t ti
 // look-back per workgroup (per tile, the walk's round trips of ~1.4 us covered 64 tiles each and could not keep up
 // with the tiles finishing). The walk waits only on groups of lower tickets, whose workgroups are already running,
 // so every wait ends. Lane l owns the pixels (4 (l % 8) + q, 8 j + l / 8), q, j = 0..3, of its tile.
-__global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const FrameDev* __restrict__ frames,
+__global__ __launch_bounds__(kClusterWaves * 64, 8) void clusters_kernel(const FrameDev* __restrict__ frames,
                                                                        const uint32_t* __restrict__ tile_frame,
                                                                        uint32_t total_tiles, uint64_t* tile_status,
                                                                        uint32_t* ticket, const float* __restrict__ slice_thr,
@@ -280,16 +282,19 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
         const float c0 = -__builtin_amdgcn_logf(fd.near_plane) * c1;
         const float emax = (float)(kt > 0u ? kt - 1u : 0u); // k0 <= kt - 1, so every index below is <= kt
         uint32_t pend = 0u;
-        float dep[16];
-        uint2 nrm[16];
-        load_tile_pixels(fd, ti, tj, lane, dep, nrm);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (uint32_t h = 0; h < 2; h++) {
+        float dep[8];
+        uint2 nrm[8];
+        load_tile_half(fd, ti, tj, lane, h, dep, nrm);
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const int j = 2 * h + jj;
             const uint32_t y = tj * 32 + j * 8 + (lane >> 3);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int i = 4 * j + q;
-                const float z = pixel_view_z(fd, x0 + q, y, dep[i]);
+                const float z = pixel_view_z(fd, x0 + q, y, dep[4 * jj + q]);
                 uint32_t k = 0u;
                 if (z > 0.0f) {
                     const float e = fminf(fmaxf(__builtin_amdgcn_logf(z) * c1 + c0, 0.0f), emax);
@@ -301,8 +306,12 @@ __global__ __launch_bounds__(kClusterWaves * 64) void clusters_kernel(const Fram
                     const bool bad = !(z < thr_kt) || (dn && (k0 == 0u || z < vm)) || (up && z >= v2);
                     if (bad) pend |= 1u << i;
                 }
-                key[i] = assemble_key(fd, ti, tj, k, nrm[i]);
+                key[i] = assemble_key(fd, ti, tj, k, nrm[4 * jj + q]);
             }
+        }
+        // the second half's loads stay after the first half's keys (the register peak of one half)
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::: "memory");
         }
         // the formula for the flagged pixels, their depth and normal read again (rare: beyond the table)
         if (pend) {
