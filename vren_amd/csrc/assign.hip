@@ -238,17 +238,21 @@ __device__ uint32_t traverse_halves(const TravFrame& tf, const Box& box, bool li
             const uint32_t level = e >> kLevelShift;
             const uint32_t addr = (e & kAddrMask) + sub;
             const bool at_leaf = level == tf.leaf_level && addr < tf.L;
-            float4 lp = make_float4(0.f, 0.f, 0.f, 0.f);
+            vf4 lpv = {0.f, 0.f, 0.f, 0.f};
             if (at_leaf) {
-                li = tf.codes[addr].y;
-                if (tf.leaf_pos) lp = tf.leaf_pos[addr];
+                li = *((guint*)tf.codes + 2 * (uint64_t)addr + 1);
+                if (tf.leaf_pos) lpv = *((gvf4*)tf.leaf_pos + addr);
             }
-            const float4* q = reinterpret_cast<const float4*>(&tf.bvh[addr]);
-            const float4 lo = q[0], hi = q[1];
+            const vf4 lov = *((gvf4*)&tf.bvh[addr]), hiv = *((gvf4*)&tf.bvh[addr] + 1);
+            const float4 lo = make_float4(lov.x, lov.y, lov.z, lov.w), hi = make_float4(hiv.x, hiv.y, hiv.z, hiv.w);
+            float4 lp = make_float4(lpv.x, lpv.y, lpv.z, lpv.w);
             const uint32_t next = __float_as_uint(lo.w);
             if (level == tf.leaf_level) {
                 if (next == VREN_BVH_LEAF_NODE && at_leaf && li < tf.L) {
-                    if (!tf.leaf_pos) lp = tf.vpos[li];
+                    if (!tf.leaf_pos) {
+                        const vf4 v = *((gvf4*)tf.vpos + li);
+                        lp = make_float4(v.x, v.y, v.z, v.w);
+                    }
                     leaf_hit = test_leaf(box, lp, lo.x, hi.x);
                 }
             } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {

@@ -172,16 +172,6 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 
 constexpr uint32_t kClusterWaves = 8;   // waves per workgroup, one 32x32 tile per wave (16 measured slower)
 
-// global-address-space views of the G-buffer pointers of a frame record: loads through them compile to global_load
-// instead of flat_load (native vector types, since HIP's float4/uint4 copy through a generic reference)
-typedef float vf4 __attribute__((ext_vector_type(4)));
-typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
-typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
-typedef const __attribute__((address_space(1))) float gfloat;
-typedef const __attribute__((address_space(1))) vf4 gvf4;
-typedef const __attribute__((address_space(1))) vu4 gvu4;
-typedef const __attribute__((address_space(1))) vu2 gvu2;
-
 // The depths and normals of lane `lane` of a tile's rows 2 h and 2 h + 1 (pixels i = 4 j + q with j = 2 h + jj,
 // stored at [4 jj + q]): pixel (x0 + q, tj * 32 + 8 j + lane / 8). A tile inside the frame with 16-byte aligned
 // rows (uniform across the wave) takes one 16-byte depth load and two 16-byte normal loads per row; other tiles

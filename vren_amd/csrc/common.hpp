@@ -19,6 +19,18 @@
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Global-address-space views for pointers held in records (frame tables): loads through them compile to global_load
+// instead of flat_load, whose completion also counts against the LDS wait counter. Native vector types, since HIP's
+// float4/uint4 copy through a generic reference.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) float gfloat;
+typedef const __attribute__((address_space(1))) uint32_t guint;
+typedef const __attribute__((address_space(1))) vf4 gvf4;
+typedef const __attribute__((address_space(1))) vu4 gvu4;
+typedef const __attribute__((address_space(1))) vu2 gvu2;
+
 // index of the calling wave in its workgroup, as a wave-uniform value: the compiler then keeps what is derived
 // from it (tile and frame indices, frame-record fields) in scalar registers and reads such records with scalar loads
 __device__ __forceinline__ uint32_t wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / VREN_WAVE); }
