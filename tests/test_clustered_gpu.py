@@ -328,6 +328,16 @@ def test_batch_modes_identical_and_match_oracle(vctx, co):
     got = results[0][0].view(np.float32).reshape(-1, 4)
     err = shade_error_report(got, ref, mask)
     assert err.max() <= SHADE_RTOL
+    # frame 2 (333x200: 77 tiles, padded to a multiple of 8 in the batch's tile numbering): keys and references
+    k = keep[2]
+    W, H = k["W"], k["H"]
+    depth = from_dev(k["d"], np.float32, (W * H,))
+    normals = from_dev(k["n"], np.uint16, (W * H, 4))
+    okeys, oU, orefs = co.unique_clusters(W, H, k["cam"], ol.cfg(), depth, normals)
+    o = b.outputs(2)
+    assert int(_read_ptr(o.cluster_count, 4, np.uint32)[0]) == oU
+    assert np.array_equal(_read_ptr(o.cluster_keys, 4 * oU, np.uint32), okeys)
+    assert np.array_equal(_read_ptr(o.cluster_refs, 4 * W * H, np.uint32), orefs)
     b.close()
 
 
