@@ -36,6 +36,10 @@ SEED_FRAMES = 42 + 1000 * 4
 BYTES_PER_PIXEL = {"clusters": 16.0, "shade": 52.0}
 BYTES_PER_LIGHT = {"lights": 40.0}
 SORT_BYTES_PER_PAIR = 68.0
+# FP32 operations of one (pixel, assigned light) pair of the PBR shade (distance, window, GGX NDF, Smith G,
+# Schlick F, diffuse/specular combine, accumulate), against the MI355X vector FP32 peak
+SHADE_OPS_PER_PAIR = 102.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def profiled_traffic():
@@ -93,6 +97,41 @@ def build_batch(ctx, vren_amd, n_frames):
         keep.append((d, n, a, m, p, l, o))
     torch.cuda.synchronize()
     return vren_amd.Batch(ctx, descs, vren_amd.default_cfg()), keep
+
+
+def _from_device(ptr, nbytes, dtype):
+    """Copy nbytes from a raw device pointer to the host (hipMemcpy through the HIP runtime torch loaded)."""
+    import numpy as np
+    import torch
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    out = np.zeros(max(nbytes, 1), np.uint8)
+    rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes),
+                       ctypes.c_int(2))
+    if rc != 0:
+        raise RuntimeError("hipMemcpy device->host failed (%d)" % rc)
+    return out[:nbytes].view(dtype)
+
+
+def shade_pairs(ctx, vren_amd, batch, keep):
+    """(pixel, assigned light) pairs the shade kernel evaluates over the batch: for every pixel in front of the
+    background threshold (view depth < far - 0.001, the kernel's test), the length of its cluster's light list.
+    Read back from the batch's outputs after a run."""
+    import numpy as np
+    cam = make_camera(vren_amd)
+    _, proj = ctx.camera_matrices(cam)
+    inv = np.linalg.inv(proj.astype(np.float64).reshape(4, 4).T)  # stored column-major
+    bg = float(np.float32(cam.far_plane) - np.float32(0.001))
+    total = 0
+    for f, k in enumerate(keep):
+        o = batch.outputs(f)
+        d = k[0].cpu().numpy().astype(np.float64)
+        z = (inv[2, 2] * d + inv[2, 3]) / (inv[3, 2] * d + inv[3, 3])
+        refs = _from_device(o.cluster_refs, 4 * W * H, np.uint32)
+        counts = _from_device(o.light_counts, 4 * o.key_capacity, np.uint32)
+        ok = (z < bg) & (refs < o.key_capacity)
+        total += int(counts[refs[ok]].sum(dtype=np.int64))
+    return total
 
 
 def time_events(fn, steps):
@@ -161,6 +200,122 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     }
 
 
+def _median_ms(fn, reps, warmup=2, before=None):
+    """Median duration of fn() over reps runs, HIP events on the current stream around each run; before() (not
+    timed) runs ahead of each."""
+    import torch
+    s = torch.cuda.current_stream()
+    times = []
+    for r in range(warmup + reps):
+        if before:
+            before()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        if r >= warmup:
+            times.append(e0.elapsed_time(e1))
+    return sorted(times)[len(times) // 2]
+
+
+def _hbm(ms, nbytes, **extra):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    d = {"ms": round(ms, 4), "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes}
+    d.update(extra)
+    return d
+
+
+def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
+    """The path's other rows at roofline sizes: uint32 add-reduce (the up-sweep tree and the scalar total) and
+    exclusive scan of 2^26 keys, bucket sort of 2^24 pairs into 65536 buckets, and the shade forward and adjoint on
+    one 1920x1080 frame with 4096 lights (smooth windowed attenuation, the differentiable configuration)."""
+    import numpy as np
+    import torch
+    n = 1 << log2n
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.synth_keys(42, 54, n, keys)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    one = torch.empty(1, dtype=torch.int32, device="cuda")
+    res = {}
+    ms = _median_ms(lambda: ctx.reduce_u32(keys, n, out, vren_amd.REDUCE_ADD), reps)
+    res["reduce_tree_u32"] = _hbm(ms, 8 * n, n=n, algorithmic_bytes_per_key=8)
+    ms = _median_ms(lambda: ctx.reduce_total_u32(keys, n, one, vren_amd.REDUCE_ADD), reps)
+    res["reduce_total_u32"] = _hbm(ms, 4 * n, n=n, algorithmic_bytes_per_key=4)
+    ms = _median_ms(lambda: ctx.exclusive_scan_u32(keys, out, n), reps)
+    res["exclusive_scan_u32"] = _hbm(ms, 8 * n, n=n, algorithmic_bytes_per_key=8)
+    del out, one
+    nb = 1 << bucket_log2
+    pairs_in = torch.stack([keys[:nb], torch.arange(nb, dtype=torch.int32, device="cuda")], dim=1).contiguous()
+    pairs_out = torch.empty_like(pairs_in)
+    offs = torch.empty(65536, dtype=torch.int32, device="cuda")
+    ms = _median_ms(lambda: ctx.bucket_sort(pairs_in, nb, pairs_out, offs, 65536, 0), reps)
+    res["bucket_sort_65536"] = _hbm(ms, 16 * nb, n=nb, algorithmic_bytes_per_pair=16)
+    del keys, pairs_in, pairs_out, offs
+
+    # shade forward and adjoint on one frame, lists from the chain's stages
+    cam = make_camera(vren_amd)
+    cf_lists = vren_amd.default_cfg()
+    cf = vren_amd.default_cfg()
+    cf.attenuation = 1
+    npx = W * H
+    key_cap, index_cap = 1 << 20, 1 << 22
+    d = torch.empty(npx, dtype=torch.float32, device="cuda")
+    nrm = torch.empty(npx * 4, dtype=torch.int16, device="cuda")
+    ar = torch.empty(npx * 4, dtype=torch.float32, device="cuda")
+    me = torch.empty(npx, dtype=torch.float32, device="cuda")
+    lp = torch.empty(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+    lc = torch.empty(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+    ctx.synth_frame(SEED_FRAMES, W, H, cam, 64, d, nrm)
+    ctx.synth_materials(SEED_FRAMES, npx, ar, me)
+    ctx.synth_lights(SEED_FRAMES, L_LIGHTS, lp, lc)
+    u8 = lambda nbytes: torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    vp, codes = u8(16 * L_LIGHTS), u8(8 * L_LIGHTS)
+    bvh = u8(32 * ctx.lib.vren_calc_bvh_buffer_length(L_LIGHTS))
+    ctx.construct_point_light_bvh(lp, lc, L_LIGHTS, cam, cf_lists, vp, codes, bvh)
+    ckeys, cnt, refs = u8(4 * key_cap), u8(4), u8(4 * npx)
+    ctx.find_unique_cluster_list(W, H, cam, cf_lists, d, nrm, ckeys, key_cap, cnt, refs)
+    counts, offsets, idx, tot = u8(4 * key_cap), u8(4 * key_cap), u8(4 * index_cap), u8(4)
+    ctx.assign_lights(W, H, cam, cf_lists, ckeys, cnt, key_cap, bvh, L_LIGHTS, codes, vp, counts, offsets, idx,
+                      index_cap, tot)
+    img = torch.empty(npx * 4, dtype=torch.float32, device="cuda")
+    grad = torch.rand(npx * 4, dtype=torch.float32, device="cuda")
+    g_pos = torch.zeros(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+    g_lig = torch.zeros(L_LIGHTS * 4, dtype=torch.float32, device="cuda")
+    g_ar = torch.empty(npx * 4, dtype=torch.float32, device="cuda")
+
+    def clear():
+        g_pos.zero_()
+        g_lig.zero_()
+        g_ar.zero_()
+
+    fwd = lambda: ctx.shade(W, H, cam, cf, d, nrm, ar, me, refs, key_cap, counts, offsets, idx, index_cap, lp, lc,
+                            L_LIGHTS, img)
+    bwd = lambda: ctx.shade_backward(W, H, cam, cf, d, nrm, ar, me, refs, key_cap, counts, offsets, idx, index_cap,
+                                     lp, lc, L_LIGHTS, grad, g_pos, g_lig, g_ar)
+    ms_f = _median_ms(fwd, reps)
+    ms_b = _median_ms(bwd, reps, before=clear)
+    torch.cuda.synchronize()
+    # pairs the two kernels evaluate: every shaded pixel's cluster list
+    _, proj = ctx.camera_matrices(cam)
+    inv = np.linalg.inv(proj.astype(np.float64).reshape(4, 4).T)
+    zd = d.cpu().numpy().astype(np.float64)
+    z = (inv[2, 2] * zd + inv[2, 3]) / (inv[3, 2] * zd + inv[3, 3])
+    r = refs.cpu().numpy().view(np.uint32)
+    c = counts.cpu().numpy().view(np.uint32)
+    ok = (z < float(np.float32(cam.far_plane) - np.float32(0.001))) & (r < key_cap)
+    pairs = int(c[r[ok]].sum(dtype=np.int64))
+    res["shade_forward_1080p"] = _hbm(ms_f, 52 * npx, pixels=npx, algorithmic_bytes_per_pixel=52, pairs=pairs,
+                                      gpairs_per_s=round(pairs / (ms_f * 1e-3) / 1e9, 3))
+    # adjoint: G-buffer, reference and dL/d(rgba) in (52 B), dL/d(albedo, roughness) accumulated (16 B read, 16 B
+    # written) per pixel, plus 28 B of light gradient per light
+    res["shade_adjoint_1080p"] = _hbm(ms_b, 84 * npx + 28 * L_LIGHTS, pixels=npx, algorithmic_bytes_per_pixel=84,
+                                      pairs=pairs, gpairs_per_s=round(pairs / (ms_b * 1e-3) / 1e9, 3),
+                                      lights=L_LIGHTS)
+    return res
+
+
 def cpu_baseline(frames=2, sort_log2=22):
     """The CPU restatement (oracle/) on one host thread: the chain on one 1080p frame with the reference's fp32
     shade formula, and std::stable_sort of configuration-1 pairs. Inputs are generated outside the timing."""
@@ -224,6 +379,7 @@ def main():
     ap.add_argument("--no-sort", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip reduce, scan, bucket sort and the shade adjoint")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB with one plain device copy kernel (scales HBM counters)")
     args = ap.parse_args()
@@ -292,11 +448,23 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                 "traffic": traffic.get(dom + "_kernel") if args.frames == 64 else None, "launch_ms": kernel_ms[dom],
                 "algorithmic_bytes_per_launch": alg_bytes}
+    # the shade kernel's arithmetic view: FP32 operations of the pairs it evaluates per launch
+    pairs = shade_pairs(ctx, vren_amd, batch, keep)
+    tflops = SHADE_OPS_PER_PAIR * pairs / (kernel_ms["shade"] * 1e-3) / 1e12
+    shade_valu = {"kernel": "shade", "bound": "valu", "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                  "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "pairs_per_launch": pairs,
+                  "pairs_per_pixel": round(pairs / pixels, 3), "ops_per_pair": SHADE_OPS_PER_PAIR,
+                  "launch_ms": kernel_ms["shade"]}
 
     sort = None
     if not args.no_sort:
         print("[bench] sort 2^%d pairs" % args.sort_log2, file=sys.stderr, flush=True)
         sort = bench_sort(ctx, args.sort_log2, args.steps, args.warmup, not args.no_tune)
+
+    rows = None
+    if not args.no_rows:
+        print("[bench] reduce, scan, bucket sort, shade forward and adjoint", file=sys.stderr, flush=True)
+        rows = bench_primitives(ctx, vren_amd, max(5, args.steps))
 
     cpu = None
     if not args.no_cpu:
@@ -316,8 +484,10 @@ def main():
         "batched_over_plain": round(value / plain_fps, 3),
         "kernel_ms": kernel_ms,
         "roofline": roofline,
+        "shade_valu": shade_valu,
         "cpu_baseline": cpu,
         "sort": sort,
+        "rows": rows,
     }
     batch.close()
     ctx.close()

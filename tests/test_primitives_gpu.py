@@ -35,6 +35,13 @@ def test_reduce_u32_multi_block_and_in_place(vctx, oracle, c1_keys):
     d = to_dev(y)
     vctx.reduce_u32(d, 4096, d, 0)
     assert np.array_equal(from_dev(d, np.uint32, (4096,)), oracle.reduce_tree_u32(y, 0))
+    # segments longer than one 16384-element tile, at input offsets that are not 16-byte aligned (5001 * 4 bytes)
+    for n, op in ((20000, 0), (5001, 2), (40001, 1)):
+        x = c1_keys[:3 * n]
+        p = 1 << (n - 1).bit_length()
+        out = empty_dev(4 * 3 * p, fill=0xCD)
+        vctx.reduce_u32(to_dev(x), n, out, op, blocks=3)
+        assert np.array_equal(from_dev(out, np.uint32, (3 * p,)), oracle.reduce_tree_u32(x, op, blocks=3)), n
 
 
 @pytest.mark.parametrize("op", [0, 1, 2])
@@ -71,6 +78,11 @@ def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, c1_keys):
     d = to_dev(x)
     vctx.exclusive_scan_u32(d, d, 5000, blocks=3)
     assert np.array_equal(from_dev(d, np.uint32, (15000,)), oracle.exclusive_scan_u32(x, blocks=3))
+    # several 16384-key tiles per segment, segments at offsets that are not 16-byte aligned (40001 * 4 bytes)
+    x = c1_keys[:3 * 40001]
+    o = empty_dev(4 * 3 * 40001)
+    vctx.exclusive_scan_u32(to_dev(x), o, 40001, blocks=3)
+    assert np.array_equal(from_dev(o, np.uint32, (3 * 40001,)), oracle.exclusive_scan_u32(x, blocks=3))
     for e in range(0, 20):  # blelloch_scan.main: all ones, N = 2^0 .. 2^19
         n = 1 << e
         o = empty_dev(4 * n)
@@ -153,10 +165,12 @@ def test_radix_sort_headline_size_properties(vctx, oracle):
     del seen, orig
 
 
-@pytest.mark.parametrize("n_buckets,shift", [(65536, 0), (64, 24), (1, 0)])
-def test_bucket_sort(vctx, oracle, n_buckets, shift):
+@pytest.mark.parametrize("n_buckets,shift,keys_kind", [(65536, 0, "u16"), (64, 24, "u30"), (1, 0, "u16"),
+                                                        (65536, 24, "u32"), (4096, 7, "u32")])
+def test_bucket_sort(vctx, oracle, n_buckets, shift, keys_kind):
     # bucket_sort.main (vren_test/vren_test/primitives/bucket_sort.cpp:182-192): N = 1, then x(4..6) while < 10^7,
-    # key = rand() % 65536, value = i -- here seeded PCG32 draws and a fixed growth sequence
+    # key = rand() % 65536, value = i -- here seeded PCG32 draws and a fixed growth sequence. "u32": full 32-bit
+    # keys with random values (the bucket field of (65536, 24) runs past bit 31 and holds 8 bits)
     n = 1
     sizes = []
     while n < 3_000_000:
@@ -164,14 +178,23 @@ def test_bucket_sort(vctx, oracle, n_buckets, shift):
         n *= 5
     for n in sizes:
         raw = oracle.pcg32(4242, 54, n)
-        keys = raw % 65536 if n_buckets == 65536 else raw >> 2  # 30-bit codes for the 64 coarse buckets
-        pairs = np.stack([keys.astype(np.uint32), np.arange(n, dtype=np.uint32)], axis=1)
+        if keys_kind == "u16":
+            keys, vals = raw % 65536, np.arange(n, dtype=np.uint32)
+        elif keys_kind == "u30":
+            keys, vals = raw >> 2, np.arange(n, dtype=np.uint32)  # 30-bit codes for the 64 coarse buckets
+        else:
+            keys, vals = raw, oracle.pcg32(77, 3, n)
+        pairs = np.stack([keys.astype(np.uint32), vals.astype(np.uint32)], axis=1)
         out = empty_dev(8 * n)
         offs = empty_dev(4 * n_buckets)
         vctx.bucket_sort(to_dev(pairs), n, out, offs, n_buckets, shift)
         opairs, ooffs = oracle.bucket_sort(pairs, n_buckets, shift)
         assert np.array_equal(from_dev(out, np.uint32, (n, 2)), opairs), n
         assert np.array_equal(from_dev(offs, np.uint32, (n_buckets,)), ooffs), n
+    # no pairs: every bucket starts at 0
+    offs = to_dev(np.full(n_buckets, 0xFFFFFFFF, np.uint32))
+    vctx.bucket_sort(empty_dev(16), 0, empty_dev(16), offs, n_buckets, shift)
+    assert not from_dev(offs, np.uint32, (n_buckets,)).any()
 
 
 @pytest.mark.parametrize("n", [0, 1, 10, 100, 1000, 10000])

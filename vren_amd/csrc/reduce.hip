@@ -3,9 +3,9 @@
 // The reference output is the whole in-place Blelloch up-sweep tree of the pow2-padded input:
 //   tree[i] = OP over the aligned block of 2^k elements ending at i, k = number of trailing 1 bits of i,
 // combined as a balanced binary tree (each node = OP(right half, left half), reduce.comp:76).
-// Here one pass handles TILE consecutive elements of a (possibly strided) view per workgroup: every thread
-// up-sweeps ITEMS consecutive elements in registers, waves finish the tile with shuffles, and the tile's
-// elements are written back in place of the ones read. The tile totals (at stride TILE) are up-swept by the
+// Here one pass handles TILE consecutive elements of a (possibly strided) view per workgroup, held in registers
+// with coalesced 16-byte loads and stores (the tree levels are formed in registers, across lanes and across the
+// tile's wave runs), and the tile's elements are written back in place of the ones read. The tile totals (at stride TILE) are up-swept by the
 // next pass over the strided view, and so on: ceil(log2(p) / log2(TILE)) passes instead of the reference's
 // one dispatch per 10 levels with 1024-element LDS trees, and every pass but the first touches only
 // p / TILE^k elements.
@@ -50,98 +50,113 @@ __device__ __forceinline__ float4 shfl_up_t<float4>(float4 v, unsigned d)
     return make_float4(shfl_up(v.x, d), shfl_up(v.y, d), shfl_up(v.z, d), shfl_up(v.w, d));
 }
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256; // scalar total
 
-// One tile of the view: elements e = tile * TILE + t, stored at base[offset + e * stride] (e < count),
-// identity beyond `valid`. Writes back every element with index < count.
-template <typename T, typename Op, int ITEMS>
-__global__ __launch_bounds__(kThreads) void upsweep_tile_kernel(const T* in, T* out,
-                                                                uint64_t in_seg_stride, uint64_t out_seg_stride,
-                                                                uint64_t valid, uint64_t count, uint64_t offset,
-                                                                uint64_t stride)
+// Up-sweep tile: kRows rows of kTreeThreads x E elements, E = 16 / sizeof(T), so that every thread loads and stores
+// E consecutive elements of each row with one 16-byte access (coalesced across the wave). The tree levels inside
+// a thread's E elements are formed in registers, the next six across the lanes with shuffles (a wave covers an
+// aligned run of 64 E elements of a row), and the last six across the tile's 64 (row, wave) runs by one wave.
+constexpr int kTreeThreads = 512;
+constexpr int kTreeWaves = kTreeThreads / VREN_WAVE;
+constexpr int kRows = 8;
+static_assert(kRows * kTreeWaves == VREN_WAVE, "one run per lane of the finishing wave");
+
+template <typename T>
+constexpr uint64_t tree_tile() { return (uint64_t)kTreeThreads * (16 / sizeof(T)) * kRows; }
+
+// One tile of the view: elements e = tile * TILE + i, stored at base[offset + e * stride] (e < count), identity
+// beyond `valid`. Writes back every element with index < count.
+template <typename T, typename Op>
+__global__ __launch_bounds__(kTreeThreads) void upsweep_tile_kernel(const T* in, T* out,
+                                                                    uint64_t in_seg_stride, uint64_t out_seg_stride,
+                                                                    uint64_t valid, uint64_t count, uint64_t offset,
+                                                                    uint64_t stride)
 {
-    constexpr int TILE = kThreads * ITEMS;
-    __shared__ T s_wave[kThreads / VREN_WAVE];
-    const uint32_t tid = threadIdx.x;
+    constexpr int E = 16 / sizeof(T);
+    constexpr uint64_t TILE = tree_tile<T>();
+    static_assert(E * sizeof(T) == 16, "16-byte groups");
+    __shared__ T s_run[kRows * kTreeWaves];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     const uint64_t seg = blockIdx.y;
     const T* src = in + seg * in_seg_stride;
     T* dst = out + seg * out_seg_stride;
     const uint64_t tile_start = (uint64_t)blockIdx.x * TILE;
-    const uint64_t first = tile_start + (uint64_t)tid * ITEMS;
+    const bool vin = stride == 1 && offset == 0 && tile_start + TILE <= valid &&
+                     (reinterpret_cast<uintptr_t>(src + tile_start) & 15) == 0;
+    const bool vout = stride == 1 && offset == 0 && tile_start + TILE <= count &&
+                      (reinterpret_cast<uintptr_t>(dst + tile_start) & 15) == 0;
 
-    T v[ITEMS];
-    if (stride == 1 && offset == 0 && first + ITEMS <= valid && (sizeof(T) * ITEMS) % 16 == 0 &&
-        (reinterpret_cast<uintptr_t>(src + first) & 15) == 0) {
-        // vectorised contiguous load
-        const uint4* p = reinterpret_cast<const uint4*>(src + first);
+    T v[kRows][E];
 #pragma unroll
-        for (int k = 0; k < (int)(sizeof(T) * ITEMS / 16); k++) {
-            uint4 q = p[k];
-            memcpy(reinterpret_cast<char*>(v) + 16 * k, &q, 16);
-        }
-    } else {
+    for (int j = 0; j < kRows; j++) {
+        const uint64_t first = tile_start + ((uint64_t)j * kTreeThreads + tid) * E;
+        if (vin) {
+            const uint4 q = *reinterpret_cast<const uint4*>(src + first);
+            memcpy(&v[j][0], &q, 16);
+        } else {
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            uint64_t e = first + k;
-            v[k] = e < valid ? src[offset + e * stride] : Op::identity();
-        }
-    }
-    // levels inside the thread's ITEMS elements
-#pragma unroll
-    for (int l = 1; l < ITEMS; l <<= 1) {
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            if ((k & (2 * l - 1)) == 2 * l - 1) v[k] = Op::apply(v[k], v[k - l]);
-        }
-    }
-    // levels across threads: the node value of thread t is v[ITEMS - 1]
-    T x = v[ITEMS - 1];
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (unsigned l = 1; l < VREN_WAVE; l <<= 1) {
-        T o = shfl_up_t(x, l);
-        if ((lane & (2 * l - 1)) == 2 * l - 1) x = Op::apply(x, o);
-    }
-    const uint32_t wave = tid / VREN_WAVE;
-    if (lane == VREN_WAVE - 1) s_wave[wave] = x;
-    __syncthreads();
-    if (lane == VREN_WAVE - 1) {
-        // levels across the waves of the block (kThreads / 64 = 4 -> 2 levels)
-        T w[kThreads / VREN_WAVE];
-#pragma unroll
-        for (int i = 0; i < kThreads / VREN_WAVE; i++) w[i] = s_wave[i];
-#pragma unroll
-        for (int l = 1; l < kThreads / VREN_WAVE; l <<= 1) {
-#pragma unroll
-            for (int i = kThreads / VREN_WAVE - 1; i >= 0; i--) {
-                if ((i & (2 * l - 1)) == 2 * l - 1) w[i] = Op::apply(w[i], w[i - l]);
+            for (int c = 0; c < E; c++) {
+                const uint64_t e = first + c;
+                v[j][c] = e < valid ? src[offset + e * stride] : Op::identity();
             }
         }
-        x = w[wave];
     }
-    v[ITEMS - 1] = x;
-    if (stride == 1 && offset == 0 && first + ITEMS <= count && (sizeof(T) * ITEMS) % 16 == 0 &&
-        (reinterpret_cast<uintptr_t>(dst + first) & 15) == 0) {
-        uint4* p = reinterpret_cast<uint4*>(dst + first);
 #pragma unroll
-        for (int k = 0; k < (int)(sizeof(T) * ITEMS / 16); k++) {
-            uint4 q;
-            memcpy(&q, reinterpret_cast<const char*>(v) + 16 * k, 16);
-            p[k] = q;
+    for (int j = 0; j < kRows; j++) {
+        // levels inside the thread's E elements
+#pragma unroll
+        for (int l = 1; l < E; l <<= 1) {
+#pragma unroll
+            for (int c = 0; c < E; c++)
+                if ((c & (2 * l - 1)) == 2 * l - 1) v[j][c] = Op::apply(v[j][c], v[j][c - l]);
         }
-    } else {
+        // levels across the lanes: the node of lane i is its last element
+        T x = v[j][E - 1];
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            uint64_t e = first + k;
-            if (e < count) dst[offset + e * stride] = v[k];
+        for (unsigned l = 1; l < VREN_WAVE; l <<= 1) {
+            T o = shfl_up_t(x, l);
+            if ((lane & (2 * l - 1)) == 2 * l - 1) x = Op::apply(x, o);
+        }
+        v[j][E - 1] = x;
+        if (lane == VREN_WAVE - 1) s_run[j * kTreeWaves + wave] = x;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // levels across the 64 runs, in element order (row-major over (row, wave))
+        T x = s_run[lane];
+#pragma unroll
+        for (unsigned l = 1; l < VREN_WAVE; l <<= 1) {
+            T o = shfl_up_t(x, l);
+            if ((lane & (2 * l - 1)) == 2 * l - 1) x = Op::apply(x, o);
+        }
+        s_run[lane] = x;
+    }
+    __syncthreads();
+    if (lane == VREN_WAVE - 1) {
+#pragma unroll
+        for (int j = 0; j < kRows; j++) v[j][E - 1] = s_run[j * kTreeWaves + wave];
+    }
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const uint64_t first = tile_start + ((uint64_t)j * kTreeThreads + tid) * E;
+        if (vout) {
+            uint4 q;
+            memcpy(&q, &v[j][0], 16);
+            *reinterpret_cast<uint4*>(dst + first) = q;
+        } else {
+#pragma unroll
+            for (int c = 0; c < E; c++) {
+                const uint64_t e = first + c;
+                if (e < count) dst[offset + e * stride] = v[j][c];
+            }
         }
     }
 }
 
-template <typename T, typename Op, int ITEMS>
+template <typename T, typename Op>
 int reduce_tree(vren_ctx* ctx, hipStream_t s, const T* in, uint32_t n, T* out, uint32_t blocks)
 {
-    constexpr uint64_t TILE = (uint64_t)kThreads * ITEMS;
+    constexpr uint64_t TILE = tree_tile<T>();
     const uint64_t p = next_pow2_u32(n);
     // pass 0: in (n valid, segment stride n) -> out (p elements, segment stride p)
     uint64_t count = p, offset = 0, stride = 1;
@@ -150,9 +165,9 @@ int reduce_tree(vren_ctx* ctx, hipStream_t s, const T* in, uint32_t n, T* out, u
         uint64_t tiles = ceil_div_u64(count, TILE);
         dim3 grid((uint32_t)tiles, blocks);
         if (first) {
-            upsweep_tile_kernel<T, Op, ITEMS><<<grid, kThreads, 0, s>>>(in, out, n, p, n, count, 0, 1);
+            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(in, out, n, p, n, count, 0, 1);
         } else {
-            upsweep_tile_kernel<T, Op, ITEMS><<<grid, kThreads, 0, s>>>(out, out, p, p, count, count, offset, stride);
+            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(out, out, p, p, count, count, offset, stride);
         }
         VREN_LAUNCH_CHECK(ctx);
         if (count <= TILE) break;
@@ -243,9 +258,9 @@ VREN_EXPORT int vren_reduce_u32(vren_ctx* ctx, vren_stream_t stream, const uint3
     if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "reduce_u32: context has no device");
     hipStream_t s = (hipStream_t)stream;
     switch (op) {
-    case VREN_REDUCE_ADD: return reduce_tree<uint32_t, OpAddU, 16>(ctx, s, in, n, out_tree, blocks);
-    case VREN_REDUCE_MIN: return reduce_tree<uint32_t, OpMinU, 16>(ctx, s, in, n, out_tree, blocks);
-    case VREN_REDUCE_MAX: return reduce_tree<uint32_t, OpMaxU, 16>(ctx, s, in, n, out_tree, blocks);
+    case VREN_REDUCE_ADD: return reduce_tree<uint32_t, OpAddU>(ctx, s, in, n, out_tree, blocks);
+    case VREN_REDUCE_MIN: return reduce_tree<uint32_t, OpMinU>(ctx, s, in, n, out_tree, blocks);
+    case VREN_REDUCE_MAX: return reduce_tree<uint32_t, OpMaxU>(ctx, s, in, n, out_tree, blocks);
     default: return vren_fail(ctx, VREN_EINVAL, "reduce_u32: unknown op %d", op);
     }
 }
@@ -263,9 +278,9 @@ VREN_EXPORT int vren_reduce_vec4(vren_ctx* ctx, vren_stream_t stream, const floa
     const float4* vin = reinterpret_cast<const float4*>(in);
     float4* vout = reinterpret_cast<float4*>(out_tree);
     switch (op) {
-    case VREN_REDUCE_ADD: return reduce_tree<float4, OpAddV, 4>(ctx, s, vin, n, vout, blocks);
-    case VREN_REDUCE_MIN: return reduce_tree<float4, OpMinV, 4>(ctx, s, vin, n, vout, blocks);
-    case VREN_REDUCE_MAX: return reduce_tree<float4, OpMaxV, 4>(ctx, s, vin, n, vout, blocks);
+    case VREN_REDUCE_ADD: return reduce_tree<float4, OpAddV>(ctx, s, vin, n, vout, blocks);
+    case VREN_REDUCE_MIN: return reduce_tree<float4, OpMinV>(ctx, s, vin, n, vout, blocks);
+    case VREN_REDUCE_MAX: return reduce_tree<float4, OpMaxV>(ctx, s, vin, n, vout, blocks);
     default: return vren_fail(ctx, VREN_EINVAL, "reduce_vec4: unknown op %d", op);
     }
 }

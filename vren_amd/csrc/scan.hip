@@ -2,16 +2,23 @@
 //
 // The reference runs an in-place up-sweep (reduce) and then one down-sweep dispatch per level above 10 plus an
 // LDS down-sweep, i.e. ~2 log2(N) / 10 + log2(N) - 10 strided passes over the buffer. Here each element is read
-// once and written once: tiles of 4096 uint32 are taken from a ticket counter, scanned in registers/LDS, and
+// once and written once: tiles of 16384 uint32 are taken from a ticket counter, scanned in registers, and
 // chained with a decoupled look-back over 64-bit (state | value) status words published at agent scope, so the
 // chain works across the 8 XCDs' separate L2s. Result: exclusive prefix sums mod 2^32, segment by segment.
 #include "scan_device.hpp"
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kItems = 16;
-constexpr int kTile = kThreads * kItems;
+// A tile is kRows rows of kThreads x 4 elements; thread `tid` holds 4 consecutive elements of every row, loaded
+// and stored as one 16-byte access per row (coalesced across the wave). The tile's 64 (row, wave) segments of 256
+// elements are scanned by one wave from their totals, so the data never leaves registers. Tiles of 16384 keys keep
+// the look-back chain short (4096 tiles at 2^26): the inclusive prefix moves back one 64-tile window per
+// agent-scope round trip, which for 4-KiB tiles was the bound.
+constexpr int kThreads = 512;
+constexpr int kWaves = kThreads / VREN_WAVE;
+constexpr int kRows = 8;
+constexpr int kTile = kThreads * 4 * kRows;
+static_assert(kRows * kWaves == VREN_WAVE, "one segment per lane of the scanning wave");
 
 __global__ void scan_init_kernel(uint64_t* status, uint64_t n_status, uint32_t* counter)
 {
@@ -23,11 +30,9 @@ __global__ void scan_init_kernel(uint64_t* status, uint64_t n_status, uint32_t* 
 __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t tiles_per_seg,
                                                         uint64_t* status, uint32_t* counter)
 {
-    __shared__ uint32_t s_stage[kTile];
-    __shared__ uint32_t s_wave[kThreads / VREN_WAVE];
+    __shared__ uint32_t s_part[kRows * kWaves]; // (row, wave) segment totals, then their exclusive prefixes
     __shared__ uint32_t s_ticket;
-    __shared__ uint32_t s_prefix;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     if (tid == 0) s_ticket = atomic_add_agent(counter, 1u);
     __syncthreads();
     const uint32_t g = s_ticket;
@@ -41,51 +46,57 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
     const bool vec = cnt == kTile && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
                      (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
 
-    // striped, coalesced load into LDS
+    uint4 v[kRows];
     if (vec) {
 #pragma unroll
-        for (int j = 0; j < kItems / 4; j++) {
-            uint32_t q = j * kThreads + tid;
-            uint4 v = ntload4(reinterpret_cast<const uint4*>(src) + q);
-            s_stage[4 * q + 0] = v.x;
-            s_stage[4 * q + 1] = v.y;
-            s_stage[4 * q + 2] = v.z;
-            s_stage[4 * q + 3] = v.w;
-        }
+        for (int j = 0; j < kRows; j++) v[j] = ntload4(reinterpret_cast<const uint4*>(src) + j * kThreads + tid);
     } else {
-        for (uint32_t i = tid; i < (uint32_t)kTile; i += kThreads) s_stage[i] = i < cnt ? src[i] : 0u;
-    }
-    __syncthreads();
-    uint32_t v[kItems];
-    uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        v[k] = s_stage[tid * kItems + k];
-        sum += v[k];
-    }
-    uint32_t block_total;
-    uint32_t excl = block_exclusive_sum<kThreads>(sum, s_wave, &block_total);
-    if (tid < VREN_WAVE) {
-        uint32_t prefix = wave_lookback(status + (uint64_t)seg * tiles_per_seg, t, block_total);
-        if (tid == 0) s_prefix = prefix;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + excl;
-#pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        s_stage[tid * kItems + k] = run;
-        run += v[k];
-    }
-    __syncthreads();
-    if (vec) {
-#pragma unroll
-        for (int j = 0; j < kItems / 4; j++) {
-            uint32_t q = j * kThreads + tid;
-            uint4 w = make_uint4(s_stage[4 * q + 0], s_stage[4 * q + 1], s_stage[4 * q + 2], s_stage[4 * q + 3]);
-            ntstore4(reinterpret_cast<uint4*>(dst) + q, w);
+        for (int j = 0; j < kRows; j++) {
+            const uint32_t e = (j * kThreads + tid) * 4;
+            v[j].x = e + 0 < cnt ? src[e + 0] : 0u;
+            v[j].y = e + 1 < cnt ? src[e + 1] : 0u;
+            v[j].z = e + 2 < cnt ? src[e + 2] : 0u;
+            v[j].w = e + 3 < cnt ? src[e + 3] : 0u;
         }
-    } else {
-        for (uint32_t i = tid; i < cnt; i += kThreads) dst[i] = s_stage[i];
+    }
+    uint32_t excl[kRows]; // exclusive prefix of the thread's 4 elements within its wave's segment of each row
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const uint32_t sum = (v[j].x + v[j].y) + (v[j].z + v[j].w);
+        const uint32_t incl = wave_inclusive_sum(sum);
+        excl[j] = incl - sum;
+        if (lane == VREN_WAVE - 1) s_part[j * kWaves + wave] = incl;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t x = s_part[lane];
+        const uint32_t xi = wave_inclusive_sum(x);
+        const uint32_t agg = shfl(xi, VREN_WAVE - 1);
+        const uint32_t prefix = wave_lookback(status + (uint64_t)seg * tiles_per_seg, t, agg);
+        s_part[lane] = prefix + xi - x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        uint32_t run = s_part[j * kWaves + wave] + excl[j];
+        uint4 o;
+        o.x = run;
+        run += v[j].x;
+        o.y = run;
+        run += v[j].y;
+        o.z = run;
+        run += v[j].z;
+        o.w = run;
+        if (vec) {
+            ntstore4(reinterpret_cast<uint4*>(dst) + j * kThreads + tid, o);
+        } else {
+            const uint32_t e = (j * kThreads + tid) * 4;
+            if (e + 0 < cnt) dst[e + 0] = o.x;
+            if (e + 1 < cnt) dst[e + 1] = o.y;
+            if (e + 2 < cnt) dst[e + 2] = o.z;
+            if (e + 3 < cnt) dst[e + 3] = o.w;
+        }
     }
 }
 

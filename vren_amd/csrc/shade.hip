@@ -210,7 +210,8 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     o.y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[1] + acc[1]) * gexp);
     o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[2] + acc[2]) * gexp);
     o.w = 1.0f;
-    reinterpret_cast<float4*>(fd.out)[p] = o;
+    const vf4 ov = {o.x, o.y, o.z, o.w};
+    *((__attribute__((address_space(1))) vf4*)fd.out + p) = ov;
 }
 
 // [begin, end) of the index-buffer entries of a tile's light lists: the tile's clusters are the consecutive indices
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
         if (nstage <= kStage) {
 #pragma unroll
             for (int k = 0; k < 2; k++)
-                if (tid + 256u * k < nstage) li[k] = fd.indices[start + tid + 256u * k];
+                if (tid + 256u * k < nstage) li[k] = *((guint*)fd.indices + start + tid + 256u * k);
         }
     }
     PixelIn px[PPT];
@@ -290,11 +291,13 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
         metal[q] = 0.0f;
         cl_raw[q] = 0xFFFFFFFFu;
         if (in[q]) {
-            const float depth = fd.depth[p[q]];
-            nb[q] = reinterpret_cast<const uint2*>(fd.normals)[p[q]];
-            ar[q] = reinterpret_cast<const float4*>(fd.albedo_rough)[p[q]];
-            metal[q] = fd.metallic[p[q]];
-            cl_raw[q] = fd.refs[p[q]];
+            const float depth = *((gfloat*)fd.depth + p[q]);
+            const vu2 nv = *((gvu2*)fd.normals + p[q]);
+            const vf4 av = *((gvf4*)fd.albedo_rough + p[q]);
+            nb[q] = make_uint2(nv.x, nv.y);
+            ar[q] = make_float4(av.x, av.y, av.z, av.w);
+            metal[q] = *((gfloat*)fd.metallic + p[q]);
+            cl_raw[q] = *((guint*)fd.refs + p[q]);
             px[q] = reconstruct(fd, gx, gy, depth);
         }
     }
@@ -307,8 +310,9 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
         my_off[q] = 0;
         my_end[q] = 0;
         if (has_list[q]) {
-            my_off[q] = min(fd.offsets[cl_raw[q]], fd.index_cap);
-            const uint64_t e = (uint64_t)fd.offsets[cl_raw[q]] + fd.counts[cl_raw[q]];
+            const uint32_t o = *((guint*)fd.offsets + cl_raw[q]);
+            my_off[q] = min(o, fd.index_cap);
+            const uint64_t e = (uint64_t)o + *((guint*)fd.counts + cl_raw[q]);
             my_end[q] = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
         }
     }
@@ -339,7 +343,7 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
         if (nstage <= kStage) {
 #pragma unroll
             for (int k = 0; k < 2; k++)
-                if (tid + 256u * k < nstage) li[k] = fd.indices[start + tid + 256u * k];
+                if (tid + 256u * k < nstage) li[k] = *((guint*)fd.indices + start + tid + 256u * k);
         }
     }
     const bool staged = nstage <= kStage;
@@ -352,8 +356,9 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
             if (i < nstage) {
                 float4 q = make_float4(0.f, 0.f, 0.f, 0.f), c = q; // radius 0: never contributes
                 if (li[k] < fd.L) {
-                    c = lcol[li[k]];
-                    q = lpos[li[k]];
+                    const vf4 cv = *((gvf4*)fd.lights + li[k]), pv = *((gvf4*)fd.light_pos + li[k]);
+                    c = make_float4(cv.x, cv.y, cv.z, cv.w);
+                    q = make_float4(pv.x, pv.y, pv.z, pv.w);
                     q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
                 }
                 s_lpos[i] = q;
