@@ -230,7 +230,7 @@ def _hbm(ms, nbytes, **extra):
 def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     """The path's other rows at roofline sizes: uint32 add-reduce (the up-sweep tree and the scalar total) and
     exclusive scan of 2^26 keys, bucket sort of 2^24 pairs into 65536 buckets, and the shade forward and adjoint on
-    one 1920x1080 frame with 4096 lights (smooth windowed attenuation, the differentiable configuration)."""
+    one 1920x1080 frame with 4096 lights (bench_shade_frame)."""
     import numpy as np
     import torch
     n = 1 << log2n
@@ -253,8 +253,16 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     ms = _median_ms(lambda: ctx.bucket_sort(pairs_in, nb, pairs_out, offs, 65536, 0), reps)
     res["bucket_sort_65536"] = _hbm(ms, 16 * nb, n=nb, algorithmic_bytes_per_pair=16)
     del keys, pairs_in, pairs_out, offs
+    res.update(bench_shade_frame(ctx, vren_amd, reps))
+    return res
 
-    # shade forward and adjoint on one frame, lists from the chain's stages
+
+def bench_shade_frame(ctx, vren_amd, reps):
+    """Shade forward and adjoint on one 1920x1080 frame with 4096 lights (smooth windowed attenuation), light
+    lists from the chain's stages."""
+    import numpy as np
+    import torch
+    res = {}
     cam = make_camera(vren_amd)
     cf_lists = vren_amd.default_cfg()
     cf = vren_amd.default_cfg()

@@ -1,7 +1,7 @@
 // shade.hip -- shade (clustered_shading.cpp:709-811, shade.comp:69-141, pbr.glsl:6-112) with resolved per-pixel
 // materials, for gfx950.
 //
-// One thread per pixel, one 256-thread workgroup per 16x16 quarter tile, flattened over every frame of a batch. The
+// One thread per pixel, one 256-thread workgroup per 32x8 strip of a tile, flattened over every frame of a batch. The
 // reference evaluates everything in fp32, including inverse(P) and inverse(V) per invocation; that formula loses
 // ~1e-4 relative accuracy wherever a light sits close to the surface or a low roughness makes the GGX
 // denominator cancel. This kernel keeps the reference's light model (constants PI = 3.14, EPSILON = 0.001,
@@ -235,8 +235,10 @@ __global__ __launch_bounds__(256) void tile_range_kernel(const FrameDev* __restr
     tile_range[g] = make_uint2(a, b);
 }
 
-// A 256-thread workgroup shades PPT 16x16 quadrants of a 32x32 tile (PPT pixels per thread, their loads issued
-// together), flattened over every frame of the table. Every pixel of a tile reads its light list from one
+// A 256-thread workgroup shades PPT 32x8 strips of a 32x32 tile (PPT pixels per thread, their loads issued
+// together), flattened over every frame of the table. A wave covers two whole 32-pixel rows of the tile, so each
+// 4-byte G-buffer plane is read in whole 128-byte lines: with 16-pixel-wide pieces, the other half of every line
+// belonged to the neighbouring workgroup, which runs on another XCD (another L2) and fetched the line again. Every pixel of a tile reads its light list from one
 // contiguous run of the index buffer, which the workgroup stages in LDS as light records (position with 1 / radius,
 // colour with radius) before shading each pixel from LDS. With tile_range (the chain, whose references are the
 // clusters kernel's), the run is known from one load, so the staged indices are requested beside the pixel data and
@@ -281,8 +283,8 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
     // every per-pixel load is issued before any of them is used
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
-        const uint32_t quad = sub * PPT + q;
-        const uint32_t gx = ti * 32 + (quad & 1) * 16 + (tid & 15), gy = tj * 32 + (quad >> 1) * 16 + (tid >> 4);
+        const uint32_t strip = sub * PPT + q;
+        const uint32_t gx = ti * 32 + (tid & 31), gy = tj * 32 + strip * 8 + (tid >> 5);
         in[q] = gx < fd.W && gy < fd.H;
         p[q] = in[q] ? (uint64_t)gy * fd.W + gx : 0;
         px[q].shaded = false;
@@ -380,10 +382,12 @@ __global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__
 struct LightTerm {
     float L[3];       // unit direction light -> surface
     float dist, r, x, w, att;
+    float inv_r, inv_dist;
     bool live;
 };
 
-// forward quantities of one (pixel, light) pair shared by the two passes of the adjoint
+// forward quantities of one (pixel, light) pair shared by the two passes of the adjoint; quotients through the
+// hardware reciprocal (1 ulp), |d| in double from one Newton step on the float square root
 __device__ __forceinline__ LightTerm light_term(const double P[3], const float4& lp, const float4& lc, uint32_t atten)
 {
     LightTerm t;
@@ -392,30 +396,34 @@ __device__ __forceinline__ LightTerm light_term(const double P[3], const float4&
     const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
     t.r = lc.w;
     if (!(dist2 > 0.0) || !(t.r > 0.0f) || !(dist2 < (double)t.r * (double)t.r)) return t;
-    const float s0 = sqrtf((float)dist2);
-    const double dist = 0.5 * ((double)s0 + dist2 / (double)s0);
-    const float omx = (float)((double)t.r - dist) * (1.0f / t.r);
+    const float s0 = __builtin_amdgcn_sqrtf((float)dist2);
+    // |d| = s0 + (|d|^2 - s0^2) / (2 s0): the correction is ~1e-7 s0, so a float reciprocal leaves ~1e-14
+    const double dist = (double)s0 + (dist2 - (double)s0 * (double)s0) * (double)(0.5f * frcp(s0));
+    t.inv_r = frcp(t.r);
+    const float omx = (float)((double)t.r - dist) * t.inv_r;
     t.x = 1.0f - omx;
     if (atten == VREN_ATTEN_SMOOTH) {
         t.w = omx * (1.0f + t.x) * (1.0f + t.x * t.x);
         t.att = t.w * t.w;
     } else {
         t.w = 0.0f;
-        t.att = omx * (1.0f + t.x + t.x * t.x) / (1.0f + t.x * sqrtf(t.x));
+        t.att = omx * (1.0f + t.x + t.x * t.x) * frcp(1.0f + t.x * __builtin_amdgcn_sqrtf(t.x));
     }
     t.dist = (float)dist;
-    const float inv = 1.0f / t.dist;
-    t.L[0] = (float)d[0] * inv;
-    t.L[1] = (float)d[1] * inv;
-    t.L[2] = (float)d[2] * inv;
+    t.inv_dist = frcp(t.dist);
+    t.L[0] = (float)d[0] * t.inv_dist;
+    t.L[1] = (float)d[1] * t.inv_dist;
+    t.L[2] = (float)d[2] * t.inv_dist;
     t.live = t.att > 0.0f;
     return t;
 }
 
 // Adjoint of one pair's contribution (pbr.glsl:42-93 with the smooth window). gT = dLoss/d(contribution rgb).
 // Accumulates d/d(light position), d/d(light rgb), d/d(radius) into gl[7] and d/d(albedo, roughness) into gp[4].
+// Every quotient goes through one of four hardware reciprocals (1 / inner, 1 / (n.l (1 - k) + k), 1 / D and the
+// per-pixel 1 / (n.v (1 - k) + k)) or the pair's 1 / r and 1 / |d|.
 __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm& t, const float4& lc, const float gT[3],
-                                              float gl[7], float gp[4])
+                                              float i_gv, float gl[7], float gp[4])
 {
     const float mL[3] = {-t.L[0], -t.L[1], -t.L[2]};
     const float NdotL = dot3(s.N, mL);
@@ -423,7 +431,7 @@ __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm&
     const float h[3] = {s.V[0] + mL[0], s.V[1] + mL[1], s.V[2] + mL[2]};
     const float hh = dot3(h, h);
     if (!(hh > 0.0f)) return;
-    const float ih = rsqrtf(hh);
+    const float ih = __builtin_amdgcn_rsqf(hh);
     const float H[3] = {h[0] * ih, h[1] * ih, h[2] * ih};
     const float Nh = dot3(s.N, h);
     float NdotH = 0.0f, inner = 1.0f;
@@ -433,10 +441,12 @@ __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm&
         NdotH = Nh * ih;
         inner = s.nn + dot3(c, c) * inv_hh + s.a2 * (Nh * Nh) * inv_hh;
     }
-    const float NDF = s.a2 / (kPi * inner * inner);
+    const float i_inner = frcp(inner);
+    const float NDF = s.a2 * kInvPi * i_inner * i_inner;
     const float k = s.rough;
     const float gl_den = NdotL * (1.0f - k) + k;
-    const float glv = NdotL / gl_den;
+    const float i_gl = frcp(gl_den);
+    const float glv = NdotL * i_gl;
     const float G = s.ggxV * glv;
     const float HdotVraw = dot3(h, s.V) * ih;
     const float HdotV = fmaxf(HdotVraw, 0.0f);
@@ -444,36 +454,36 @@ __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm&
     const float om2 = om * om;
     const float f5 = om2 * om2 * om;
     const float D = 4.0f * s.NdotV * NdotL + 0.0001f;
-    const float spec = NDF * G / D;
+    const float i_D = frcp(D);
+    const float spec = NDF * G * i_D;
     const float km = 1.0f - s.metal;
     float g_att = 0.0f, g_NdotL = 0.0f, g_spec = 0.0f, g_f5 = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const float F = s.F0[c] + (1.0f - s.F0[c]) * f5;
         const float kD = (1.0f - F) * km;
-        const float B = kD * s.albedo[c] / kPi + F * spec;
+        const float B = kD * s.albedo[c] * kInvPi + F * spec;
         const float lcc = c == 0 ? lc.x : (c == 1 ? lc.y : lc.z);
         gl[3 + c] += gT[c] * B * t.att * NdotL;
         const float gB = gT[c] * lcc * t.att * NdotL;
         g_att += gT[c] * B * lcc * NdotL;
         g_NdotL += gT[c] * B * lcc * t.att;
         g_spec += gB * F;
-        const float gF = gB * (spec - s.albedo[c] * km / kPi);
-        gp[c] += gB * kD / kPi + gF * (1.0f - f5) * s.metal;
+        const float gF = gB * (spec - s.albedo[c] * km * kInvPi);
+        gp[c] += gB * kD * kInvPi + gF * (1.0f - f5) * s.metal;
         g_f5 += gF * (1.0f - s.F0[c]);
     }
     const float g_HdotV = HdotVraw > 0.0f ? g_f5 * (-5.0f * om2 * om2) : 0.0f;
-    const float g_NDF = g_spec * G / D;
-    const float g_G = g_spec * NDF / D;
-    g_NdotL += -g_spec * spec / D * 4.0f * s.NdotV;
+    const float g_NDF = g_spec * G * i_D;
+    const float g_G = g_spec * NDF * i_D;
+    g_NdotL += -g_spec * spec * i_D * 4.0f * s.NdotV;
     const float g_gl = g_G * s.ggxV;
     const float g_ggxV = g_G * glv;
-    g_NdotL += g_gl * k / (gl_den * gl_den);
-    const float gv_den = s.NdotV * (1.0f - k) + k;
-    float g_k = g_gl * (-NdotL * (1.0f - NdotL) / (gl_den * gl_den)) +
-                g_ggxV * (-s.NdotV * (1.0f - s.NdotV) / (gv_den * gv_den));
-    const float g_inner = g_NDF * (-2.0f * NDF / inner);
-    const float g_a2 = g_NDF / (kPi * inner * inner) + g_inner * NdotH * NdotH;
+    g_NdotL += g_gl * k * (i_gl * i_gl);
+    float g_k = g_gl * (-NdotL * (1.0f - NdotL) * (i_gl * i_gl)) +
+                g_ggxV * (-s.NdotV * (1.0f - s.NdotV) * (i_gv * i_gv));
+    const float g_inner = g_NDF * (-2.0f * NDF * i_inner);
+    const float g_a2 = g_NDF * kInvPi * i_inner * i_inner + g_inner * NdotH * NdotH;
     const float g_NdotH = Nh > 0.0f ? g_inner * 2.0f * NdotH * (s.a2 - 1.0f) : 0.0f;
     gp[3] += g_a2 * 2.0f * s.rough + g_k;
     // geometry: h = V - L, NdotL = -N.L, NdotH = N.h/|h|, HdotV = V.h/|h|
@@ -485,15 +495,34 @@ __device__ __forceinline__ void light_adjoint(const Surface& s, const LightTerm&
     }
     // att(x), x = |d| / r (smooth window: att = w^2, w = 1 - x^4)
     const float datt_dx = -8.0f * t.x * t.x * t.x * t.w;
-    const float g_dist = g_att * datt_dx / t.r;
+    const float g_dist = g_att * datt_dx * t.inv_r;
     const float gLL = dot3(g_L, t.L);
-    const float idist = 1.0f / t.dist;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const float g_d = (g_L[i] - gLL * t.L[i]) * idist + g_dist * t.L[i];
+        const float g_d = (g_L[i] - gLL * t.L[i]) * t.inv_dist + g_dist * t.L[i];
         gl[i] -= g_d; // d = p - light
     }
-    gl[6] += g_att * datt_dx * (-t.x / t.r);
+    gl[6] += g_att * datt_dx * (-t.x * t.inv_r);
+}
+
+// Sums each of the 7 values over the wave with 10 shuffles instead of 42: the first three butterfly stages (lane
+// bits 5, 4, 3) each keep half of the remaining components, so lane l ends with component (l >> 3) & 7 summed
+// over its 8-lane group, and three more stages sum the group. Every lane of the wave must be active.
+__device__ __forceinline__ float wave_sum_components(const float g[7])
+{
+    const uint32_t lane = lane_id();
+    const float v[8] = {g[0], g[1], g[2], g[3], g[4], g[5], g[6], 0.0f};
+    const bool h32 = (lane & 32u) != 0, h16 = (lane & 16u) != 0, h8 = (lane & 8u) != 0;
+    float w4[4], w2[2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w4[i] = (h32 ? v[i + 4] : v[i]) + shfl_xor(h32 ? v[i] : v[i + 4], 32);
+#pragma unroll
+    for (int i = 0; i < 2; i++) w2[i] = (h16 ? w4[i + 2] : w4[i]) + shfl_xor(h16 ? w4[i] : w4[i + 2], 16);
+    float w = (h8 ? w2[1] : w2[0]) + shfl_xor(h8 ? w2[0] : w2[1], 8);
+    w += shfl_xor(w, 4);
+    w += shfl_xor(w, 2);
+    w += shfl_xor(w, 1);
+    return w;
 }
 
 __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __restrict__ frames, uint32_t F, const float* grad_out,
@@ -512,7 +541,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     const uint32_t t = g - fd.tile_base;
     if (t >= fd.Sx * fd.Sy) return; // padding after the frame's tiles (the whole workgroup)
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
-    const uint32_t gx = ti * 32 + (sub & 1) * 16 + (tid & 15), gy = tj * 32 + (sub >> 1) * 16 + (tid >> 4);
+    const uint32_t gx = ti * 32 + (tid & 31), gy = tj * 32 + sub * 8 + (tid >> 5); // 32x8 strip `sub`
     const bool in = gx < fd.W && gy < fd.H;
     const uint64_t p = in ? (uint64_t)gy * fd.W + gx : 0;
     PixelIn px;
@@ -572,8 +601,12 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
         for (uint32_t i = tid; i < nstage * 7; i += 256) s_g[i] = 0.0f;
     }
     __syncthreads();
+    Surface s;
+    uint32_t off = 0, end = 0, len = 0;
+    float i_gv = 0.0f;
+    float gT[3] = {0.0f, 0.0f, 0.0f};
+    float gp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (px.shaded) {
-        Surface s;
         {
             const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
             const float il = rsqrtf(dot3(V, V));
@@ -591,8 +624,8 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
         s.a2 = s.rough * s.rough;
         for (int c = 0; c < 3; c++) s.F0[c] = 0.04f * (1.0f - s.metal) + s.albedo[c] * s.metal;
         s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
-        s.ggxV = s.NdotV / (s.NdotV * (1.0f - s.rough) + s.rough);
-        uint32_t off = 0, end = 0;
+        i_gv = frcp(s.NdotV * (1.0f - s.rough) + s.rough);
+        s.ggxV = s.NdotV * i_gv;
         if (has_list) {
             off = fd.offsets[cl];
             const uint64_t e = (uint64_t)off + fd.counts[cl];
@@ -618,42 +651,72 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
             apply_light(s, lt.L, rad, acc);
         }
         const float gexp = 1.0f / 2.2f;
-        float gT[3];
         const float goc[3] = {go.x, go.y, go.z};
-        float gp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int c = 0; c < 3; c++) {
+            // col^(1/2.2 - 1) as 2^(log2(col) (1/2.2 - 1)) on the transcendental unit (~1e-6 relative)
             const float col = 0.03f * s.albedo[c] + acc[c];
-            gT[c] = goc[c] * gexp * powf(col, gexp - 1.0f);
+            gT[c] = goc[c] * gexp * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(col) * (gexp - 1.0f));
             gp[c] += 0.03f * gT[c];
         }
-        // pass 2: adjoint of every contribution
-        for (uint32_t e = off; e < end; e++) {
-            uint32_t li = 0xFFFFFFFFu;
+        len = end > off ? end - off : 0u;
+    }
+    // pass 2: adjoint of every contribution, in a loop that is uniform over the wave (iteration k handles entry
+    // off + k of every lane whose list is that long), so that when every such lane is at the same entry -- the
+    // lanes of one cluster -- the wave sums its 7 light-gradient terms and one lane adds them to the staged slot
+    uint32_t kmax = len;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) kmax = max(kmax, shfl_xor(kmax, m));
+    kmax = __builtin_amdgcn_readfirstlane(kmax);
+    for (uint32_t k = 0; k < kmax; k++) {
+        const bool act = k < len;
+        const uint32_t e = off + k;
+        uint32_t li = 0xFFFFFFFFu;
+        float gl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        bool live = false;
+        if (act) {
             float4 lp, lc;
+            bool ok = true;
             if (staged) {
                 lp = s_lpos[e - start];
                 lc = s_lcol[e - start];
             } else {
                 li = fd.indices[e];
-                if (li >= fd.L) continue;
-                lp = lpos[li];
-                lc = lcol[li];
+                ok = li < fd.L;
+                if (ok) {
+                    lp = lpos[li];
+                    lc = lcol[li];
+                }
             }
-            const LightTerm lt = light_term(px.P, lp, lc, fd.attenuation);
-            if (!lt.live) continue;
-            float gl[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            light_adjoint(s, lt, lc, gT, gl, gp);
-            if (staged) {
+            if (ok) {
+                const LightTerm lt = light_term(px.P, lp, lc, fd.attenuation);
+                if (lt.live) {
+                    light_adjoint(s, lt, lc, gT, i_gv, gl, gp);
+                    live = true;
+                }
+            }
+        }
+        const uint64_t lm = ballot(live);
+        if (lm == 0) continue;
+        if (staged) {
+            const uint32_t first = (uint32_t)__builtin_ctzll(lm);
+            const uint32_t ef = __builtin_amdgcn_readlane(e, first);
+            if (ballot(live && e != ef) == 0) {
+                const float sum = wave_sum_components(gl);
+                const uint32_t c = (lane >> 3) & 7u;
+                if ((lane & 7u) == 0 && c < 7 && sum != 0.0f) atomicAdd(&s_g[(ef - start) * 7 + c], sum);
+            } else if (live) {
                 float* slot = s_g + (e - start) * 7;
 #pragma unroll
                 for (int i = 0; i < 7; i++) atomicAdd(&slot[i], gl[i]);
-            } else {
-#pragma unroll
-                for (int i = 0; i < 3; i++) atomicAdd(&g_pos[4 * li + i], gl[i]);
-#pragma unroll
-                for (int i = 0; i < 4; i++) atomicAdd(&g_light[4 * li + i], gl[3 + i]);
             }
+        } else if (live) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) atomicAdd(&g_pos[4 * li + i], gl[i]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) atomicAdd(&g_light[4 * li + i], gl[3 + i]);
         }
+    }
+    if (px.shaded) {
         float4* pa = reinterpret_cast<float4*>(g_ar) + p;
         float4 cur = *pa;
         cur.x += gp[0];
