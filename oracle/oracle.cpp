@@ -11,6 +11,8 @@
  * Pins (what checks each function independently of this file; see tests/):
  *   o_pcg32_*            published PCG32 demo vector (seed 42, seq 54) + jump-ahead == sequential
  *   o_logf_first_decrease  a planted glitch is reported at exactly its bit pattern
+ *   o_rcp_f16_newton_mismatches  the planted variant (no Newton step) is reported; exact rational arithmetic
+ *                        agrees (the FMA and the division here are correctly rounded by the C library)
  *   o_reduce_tree_*      closed form tree[i] = op over the aligned block ending at i; totals vs numpy
  *   o_exclusive_scan_u32 numpy cumsum (uint32 wrap-around)
  *   o_stable_sort_kv     numpy argsort(kind='stable')
@@ -135,6 +137,25 @@ OEXPORT uint32_t o_logf_first_decrease(uint32_t lo, uint32_t hi, uint32_t plante
         prev = next;
     }
     return 0xFFFFFFFFu;
+}
+
+/* The cluster kernel's 1 / a for a (> 0) a finite f16 normal component widened to float: a reciprocal within
+ * 1 ulp of 1 / a (every candidate: the correctly rounded value and its two neighbours) refined by one FMA Newton
+ * step. Returns how many (a, candidate) pairs do not land on the correctly rounded 1 / a (IEEE division here);
+ * `planted` skips the Newton step, which must then be caught. */
+OEXPORT uint32_t o_rcp_f16_newton_mismatches(uint32_t planted)
+{
+    uint32_t bad = 0;
+    for (uint32_t h = 1; h < 0x7C00u; h++) {
+        const float a = vren_half_to_float((uint16_t)h);
+        const float r = 1.0f / a;
+        const float cand[3] = {std::nextafter(r, 0.0f), r, std::nextafter(r, INFINITY)};
+        for (float y : cand) {
+            const float y1 = planted ? y : std::fma(y, std::fma(-a, y, 1.0f), y);
+            if (y1 != r) bad++;
+        }
+    }
+    return bad;
 }
 
 OEXPORT float o_expf(float x) { return vren_expf(x); }

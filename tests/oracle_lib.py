@@ -31,6 +31,8 @@ class Oracle:
             getattr(L, name).argtypes = [c_f]
         L.o_logf_first_decrease.restype = c_u32
         L.o_logf_first_decrease.argtypes = [c_u32, c_u32, c_u32]
+        L.o_rcp_f16_newton_mismatches.restype = c_u32
+        L.o_rcp_f16_newton_mismatches.argtypes = [c_u32]
         L.o_powi.restype = c_f
         L.o_powi.argtypes = [c_f, c_u32]
         L.o_reduce_total_u32.restype = c_u32
@@ -54,6 +56,9 @@ class Oracle:
 
     def logf_first_decrease(self, lo, hi, planted=0):
         return int(self.lib.o_logf_first_decrease(lo, hi, planted))
+
+    def rcp_f16_newton_mismatches(self, planted=0):
+        return int(self.lib.o_rcp_f16_newton_mismatches(planted))
 
     def expf(self, x):
         return self.lib.o_expf(float(x))

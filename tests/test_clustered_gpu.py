@@ -167,6 +167,37 @@ def test_unique_clusters_slice_boundaries(vctx, co, mode):
     assert slices.size == 200 if mode == "sliced" else slices.size > 150, slices.size
 
 
+@pytest.mark.parametrize("mode", ["reference", "sliced"])
+def test_unique_clusters_every_f16_normal_component(vctx, co, mode):
+    """Normals holding every f16 bit pattern (zeros, subnormals, the largest finite, +-inf, NaNs) as one component,
+    rotating over the three axes, with the other two components random and mostly smaller in magnitude (so that
+    pattern is the dominant axis and the key divides by it), a quarter of them random over the whole f16 range;
+    keys and references bit for bit against the oracle."""
+    W = H = 256
+    n = W * H
+    cam = ol.camera(W, H)
+    rng = np.random.default_rng(4321)
+    h = np.arange(n, dtype=np.uint32).astype(np.uint16)
+    hv = h.view(np.float16).astype(np.float32)
+    finite = np.isfinite(hv)
+    normals = np.zeros((n, 4), np.uint16)
+    axis = np.arange(n) % 3
+    for o in (1, 2):
+        with np.errstate(all="ignore"):
+            small = (rng.uniform(-1.0, 1.0, n).astype(np.float32) * np.where(finite, hv, 1.0)).astype(np.float16)
+        wide = rng.integers(0, 1 << 16, n, dtype=np.uint32).astype(np.uint16)
+        other = np.where(rng.uniform(size=n) < 0.25, wide, small.view(np.uint16))
+        normals[np.arange(n), (axis + o) % 3] = other
+    normals[np.arange(n), axis] = h
+    depth, _ = co.synth_frame(778, W, H, cam)
+    cf = ol.cfg(key_mode=1, depth_slices=24) if mode == "sliced" else ol.cfg()
+    _, keys, U, refs, _, _ = gpu_clusters(vctx, W, H, cam, cf, depth, normals)
+    okeys, oU, orefs = co.unique_clusters(W, H, cam, cf, depth, normals)
+    assert U == oU
+    assert np.array_equal(keys, okeys)
+    assert np.array_equal(refs, orefs)
+
+
 def test_unique_clusters_capacity_flag(vctx, co, c3):
     from vren_amd import VrenError
     _, keys, U, refs, _, _ = gpu_clusters(vctx, W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)

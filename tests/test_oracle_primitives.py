@@ -62,6 +62,20 @@ def test_vmath_logf_nondecreasing_over_all_nonnegative_floats(oracle):
     assert found == [0xFFFFFFFF] * 8, [hex(f) for f in found]
 
 
+def test_reciprocal_of_f16_divisors_is_correctly_rounded(oracle):
+    """The cluster-key kernel divides by the dominant normal component, an f16 value, as v_rcp_f32 (within 1 ulp)
+    plus one FMA Newton step. For every finite positive f16 divisor and every reciprocal within 1 ulp, the step
+    lands on the correctly rounded quotient; without the step (planted) the neighbours do not. Exact rational
+    arithmetic agrees on a sample of divisors."""
+    from fractions import Fraction
+    assert oracle.rcp_f16_newton_mismatches() == 0
+    assert oracle.rcp_f16_newton_mismatches(planted=1) > 0
+    for h in (1, 0x3C01, 0x3555, 0x5A5A, 0x7BFF):
+        a = float(np.array(h, np.uint16).view(np.float16))
+        r = np.float32(1.0) / np.float32(a)
+        assert abs(Fraction(float(r)) - 1 / Fraction(a)) <= abs(Fraction(float(np.nextafter(r, np.float32(0)))) - 1 / Fraction(a))
+
+
 def test_vmath_expf_and_powi(oracle):
     rng = np.random.default_rng(2)
     worst = 0.0

@@ -37,6 +37,17 @@ __device__ __forceinline__ uint32_t dominant_axis(const float n[3])
     return axis;
 }
 
+// 1 / a, correctly rounded, for a > 0 an f16 value widened to float: v_rcp_f32 (within 1 ulp) and one FMA Newton
+// step. A divisor with an 11-bit significand keeps 1 / a at least 2^-36 (relative) away from every rounding
+// boundary of float, far beyond the step's ~2^-46 error; the CPU tests check every finite f16 divisor against
+// every reciprocal within 1 ulp. a = +inf gives +0, as the division does (the Newton step would give NaN).
+__device__ __forceinline__ float rcp_f16_exact(float a)
+{
+    const float y = __builtin_amdgcn_rcpf(a);
+    const float r = __builtin_fmaf(y, __builtin_fmaf(-a, y, 1.0f), y);
+    return a < __builtin_inff() ? r : 0.0f;
+}
+
 __device__ __forceinline__ uint32_t discretize_normal(const float n[3])
 {
     // clustered_shading.glsl:7-43
@@ -46,7 +57,8 @@ __device__ __forceinline__ uint32_t discretize_normal(const float n[3])
     uint32_t axis = 0, face_idx = 0;
     if (ax < 3) {
         axis = ax;
-        min_t = (n[ax] > 0.0f ? 1.0f : -1.0f) / n[ax];
+        // sign(n) / n = 1 / |n|, |n[ax]| > 0 being the largest magnitude (finite or +inf)
+        min_t = rcp_f16_exact(fabsf(n[ax]));
         face_idx = (n[ax] > 0.0f ? 1u : 0u) * 3u + ax;
     }
     float p[3] = {n[0] * min_t, n[1] * min_t, n[2] * min_t};
@@ -147,10 +159,19 @@ __device__ __forceinline__ float pixel_view_z(const FrameDev& fd, uint32_t gx, u
 }
 
 // the cluster key from its slice and normal (find_unique_clusters.comp:58-65)
+// f16 -> f32 on the conversion unit. A key depends on a NaN component only through comparisons and through
+// f2u_sat(floor(NaN)) = 0, never on its payload, so the conversion's NaN payload need not be the oracle's.
+__device__ __forceinline__ float key_h2f(uint32_t h)
+{
+    const uint16_t b = (uint16_t)h;
+    _Float16 x;
+    __builtin_memcpy(&x, &b, 2);
+    return (float)x;
+}
+
 __device__ __forceinline__ uint32_t assemble_key(const FrameDev& fd, uint32_t ti, uint32_t tj, uint32_t k, uint2 nb)
 {
-    const float n[3] = {vren_h2f((uint16_t)(nb.x & 0xFFFFu)), vren_h2f((uint16_t)(nb.x >> 16)),
-                        vren_h2f((uint16_t)(nb.y & 0xFFFFu))};
+    const float n[3] = {key_h2f(nb.x & 0xFFFFu), key_h2f(nb.x >> 16), key_h2f(nb.y & 0xFFFFu)};
     if (fd.key_mode == VREN_KEYS_SLICED)
         return (ti & 0xFFu) | ((tj & 0xFFu) << 8) | ((k & 0xFFu) << 16) | (face_of_normal(n) << 24);
     return (ti & 0xFFu) | ((tj & 0xFFu) << 8) | ((k & 0x3FFu) << 16) | (discretize_normal(n) << 26);
