@@ -195,6 +195,16 @@ VREN_HD uint32_t vren_f2u_sat(float x)
     return (uint32_t)x;
 }
 
+/* vren_f2u_sat(vren_floorf((u + 1) / 2 * 3)), the normal code's cell index along one face axis
+ * (clustered_shading.glsl:37-38), for u < 1.5 or NaN, as three comparisons: the float chain is non-decreasing in
+ * u, so the index reaches c exactly when u reaches the smallest float that does (found by bisection over float
+ * bit patterns; the CPU tests compare both forms on every float below 1.5). A face coordinate is a component
+ * divided by the largest one, so |u| <= 1 up to rounding. */
+VREN_HD uint32_t vren_third_index(float u)
+{
+    return (uint32_t)(u >= -0x1.555554p-2f) + (uint32_t)(u >= 0x1.555556p-2f) + (uint32_t)(u >= 0x1.fffffep-1f);
+}
+
 VREN_HD float vren_dot3(const float a[3], const float b[3])
 {
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];

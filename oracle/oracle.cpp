@@ -13,6 +13,7 @@
  *   o_logf_first_decrease  a planted glitch is reported at exactly its bit pattern
  *   o_rcp_f16_newton_mismatches  the planted variant (no Newton step) is reported; exact rational arithmetic
  *                        agrees (the FMA and the division here are correctly rounded by the C library)
+ *   o_third_index_mismatches  a planted threshold one float off is reported
  *   o_reduce_tree_*      closed form tree[i] = op over the aligned block ending at i; totals vs numpy
  *   o_exclusive_scan_u32 numpy cumsum (uint32 wrap-around)
  *   o_stable_sort_kv     numpy argsort(kind='stable')
@@ -154,6 +155,24 @@ OEXPORT uint32_t o_rcp_f16_newton_mismatches(uint32_t planted)
             const float y1 = planted ? y : std::fma(y, std::fma(-a, y, 1.0f), y);
             if (y1 != r) bad++;
         }
+    }
+    return bad;
+}
+
+/* Bit patterns b in [lo, hi) whose float u (< 1.5 or NaN) gets a different cell index from the comparison form
+ * (vren_third_index) than from the formula; `planted` moves the first threshold up one float in the compared
+ * copy, which must then be reported. */
+OEXPORT uint32_t o_third_index_mismatches(uint64_t lo, uint64_t hi, uint32_t planted)
+{
+    const float t1 = planted ? std::nextafter(-0x1.555554p-2f, 1.0f) : -0x1.555554p-2f;
+    uint32_t bad = 0;
+    for (uint64_t b = lo; b < hi; b++) {
+        const float u = vren_bits_f32((uint32_t)b);
+        if (u >= 1.5f) continue;
+        const uint32_t formula = vren_f2u_sat(vren_floorf((u + 1.0f) / 2.0f * 3.0f));
+        const uint32_t cmp = planted ? (uint32_t)(u >= t1) + (uint32_t)(u >= 0x1.555556p-2f) + (uint32_t)(u >= 0x1.fffffep-1f)
+                                     : vren_third_index(u);
+        if (formula != cmp) bad++;
     }
     return bad;
 }

@@ -33,6 +33,8 @@ class Oracle:
         L.o_logf_first_decrease.argtypes = [c_u32, c_u32, c_u32]
         L.o_rcp_f16_newton_mismatches.restype = c_u32
         L.o_rcp_f16_newton_mismatches.argtypes = [c_u32]
+        L.o_third_index_mismatches.restype = c_u32
+        L.o_third_index_mismatches.argtypes = [ctypes.c_uint64, ctypes.c_uint64, c_u32]
         L.o_powi.restype = c_f
         L.o_powi.argtypes = [c_f, c_u32]
         L.o_reduce_total_u32.restype = c_u32
@@ -59,6 +61,9 @@ class Oracle:
 
     def rcp_f16_newton_mismatches(self, planted=0):
         return int(self.lib.o_rcp_f16_newton_mismatches(planted))
+
+    def third_index_mismatches(self, lo, hi, planted=0):
+        return int(self.lib.o_third_index_mismatches(lo, hi, planted))
 
     def expf(self, x):
         return self.lib.o_expf(float(x))

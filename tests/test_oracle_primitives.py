@@ -76,6 +76,18 @@ def test_reciprocal_of_f16_divisors_is_correctly_rounded(oracle):
         assert abs(Fraction(float(r)) - 1 / Fraction(a)) <= abs(Fraction(float(np.nextafter(r, np.float32(0)))) - 1 / Fraction(a))
 
 
+def test_third_index_comparisons_match_formula_on_every_float(oracle):
+    """The normal code's cell index f2u_sat(floor((u + 1) / 2 * 3)) as three threshold comparisons
+    (vren_third_index), on every float bit pattern below 1.5 and every NaN; 8 threads. A threshold moved by one
+    float must be caught."""
+    from concurrent.futures import ThreadPoolExecutor
+    assert oracle.third_index_mismatches(0xBEAAAA00, 0xBEAAAB00, planted=1) > 0
+    chunks = [(i << 29, (i + 1) << 29) for i in range(8)]  # every 32-bit pattern
+    with ThreadPoolExecutor(8) as ex:
+        bad = list(ex.map(lambda c: oracle.third_index_mismatches(c[0], c[1]), chunks))
+    assert bad == [0] * 8, bad
+
+
 def test_vmath_expf_and_powi(oracle):
     rng = np.random.default_rng(2)
     worst = 0.0

@@ -64,8 +64,8 @@ __device__ __forceinline__ uint32_t discretize_normal(const float n[3])
     float p[3] = {n[0] * min_t, n[1] * min_t, n[2] * min_t};
     float u = p[(axis + 1) % 3];
     float v = p[(axis + 2) % 3];
-    uint32_t du = vren_f2u_sat(vren_floorf((u + 1.0f) / 2.0f * 3.0f));
-    uint32_t dv = vren_f2u_sat(vren_floorf((v + 1.0f) / 2.0f * 3.0f));
+    uint32_t du = vren_third_index(u);
+    uint32_t dv = vren_third_index(v);
     return (face_idx * 9u + du * 3u + dv) & 0x3Fu;
 }
 
