@@ -95,12 +95,12 @@ VREN_EXPORT int vren_autotune_radix_sort(vren_ctx* ctx, vren_stream_t stream, ui
     int best = -1;
     float best_t = 0.f;
     auto cleanup = [&]() {
-        hipStreamSynchronize(s);
-        if (keys) hipFree(keys);
-        if (vals) hipFree(vals);
-        if (scratch) hipFree(scratch);
-        if (e0) hipEventDestroy(e0);
-        if (e1) hipEventDestroy(e1);
+        vren_teardown(hipStreamSynchronize(s));
+        if (keys) vren_teardown(hipFree(keys));
+        if (vals) vren_teardown(hipFree(vals));
+        if (scratch) vren_teardown(hipFree(scratch));
+        if (e0) vren_teardown(hipEventDestroy(e0));
+        if (e1) vren_teardown(hipEventDestroy(e1));
     };
     if (hipMalloc(&keys, n * 4) != hipSuccess || (with_values && hipMalloc(&vals, n * 4) != hipSuccess) ||
         hipMalloc(&scratch, scratch_bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
@@ -113,12 +113,15 @@ VREN_EXPORT int vren_autotune_radix_sort(vren_ctx* ctx, vren_stream_t stream, ui
         float acc = 0.f;
         for (int r = 0; r <= reps && status == VREN_OK; r++) {
             fill_random_kernel<<<1024, 256, 0, s>>>(keys, vals, n, (uint32_t)(c * 16 + r));
-            hipEventRecord(e0, s);
+            if (hipEventRecord(e0, s) != hipSuccess) {
+                status = vren_fail(ctx, VREN_EHIP, "autotune: event record failed");
+                break;
+            }
             status = vren_radix_sort_with_config(ctx, s, c, keys, vals, n, 0, key_bits, scratch, scratch_bytes);
-            hipEventRecord(e1, s);
-            if (hipEventSynchronize(e1) != hipSuccess) status = vren_fail(ctx, VREN_EHIP, "autotune: sync failed");
             float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
+            if (status == VREN_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                status = vren_fail(ctx, VREN_EHIP, "autotune: timing the configuration failed");
             if (r > 0) acc += ms; // first run warms the configuration up
         }
         float t = acc / reps;

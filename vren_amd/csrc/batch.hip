@@ -253,9 +253,10 @@ VREN_EXPORT int vren_batch_run_profiled(vren_batch* b, vren_stream_t stream, flo
     if (!st) VREN_HIP(ctx, hipEventSynchronize(ev[n]));
     for (int i = 0; i < VREN_CHAIN_KERNELS; i++) {
         kernel_ms[i] = 0.0f;
-        if (!st && i < n) hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]);
+        if (!st && i < n && hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]) != hipSuccess)
+            st = vren_fail(ctx, VREN_EHIP, "batch_run_profiled: no time between events %d and %d", i, i + 1);
     }
-    for (int i = 0; i <= VREN_CHAIN_KERNELS; i++) hipEventDestroy(ev[i]);
+    for (int i = 0; i <= VREN_CHAIN_KERNELS; i++) vren_teardown(hipEventDestroy(ev[i]));
     return st;
 }
 
@@ -282,15 +283,15 @@ VREN_EXPORT int vren_batch_frame_outputs(vren_batch* b, uint32_t frame, vren_fra
 VREN_EXPORT void vren_batch_destroy(vren_batch* b)
 {
     if (!b) return;
-    hipDeviceSynchronize();
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    if (b->capture) hipStreamDestroy(b->capture);
-    if (b->arena) hipFree(b->arena);
-    if (b->work) hipFree(b->work);
-    if (b->d_table) hipFree(b->d_table);
-    if (b->d_single) hipFree(b->d_single);
-    if (b->d_tile_frame) hipFree(b->d_tile_frame);
+    vren_teardown(hipDeviceSynchronize());
+    if (b->exec) vren_teardown(hipGraphExecDestroy(b->exec));
+    if (b->graph) vren_teardown(hipGraphDestroy(b->graph));
+    if (b->capture) vren_teardown(hipStreamDestroy(b->capture));
+    if (b->arena) vren_teardown(hipFree(b->arena));
+    if (b->work) vren_teardown(hipFree(b->work));
+    if (b->d_table) vren_teardown(hipFree(b->d_table));
+    if (b->d_single) vren_teardown(hipFree(b->d_single));
+    if (b->d_tile_frame) vren_teardown(hipFree(b->d_tile_frame));
     delete b;
 }
 

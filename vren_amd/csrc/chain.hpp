@@ -33,9 +33,10 @@ struct ChainWork {
     int* n_events = nullptr;
 };
 
+// a failed record surfaces at the checked synchronisation on the last event (vren_batch_run_profiled)
 inline void chain_mark(const ChainWork& w, hipStream_t s)
 {
-    if (w.events) hipEventRecord(w.events[(*w.n_events)++], s);
+    if (w.events) vren_teardown(hipEventRecord(w.events[(*w.n_events)++], s));
 }
 
 // kernels of one chain launch, in order (chain_mark follows each)

@@ -187,6 +187,10 @@ int vren_ensure_scratch(vren_ctx* ctx, size_t bytes);
 
 #define VREN_LAUNCH_CHECK(ctx) VREN_HIP(ctx, hipGetLastError())
 
+// Result of a teardown call (free, destroy, the synchronisation before them) made where nothing can report it: a
+// destroy function returns nothing, and a sticky device error resurfaces at the next checked call.
+static inline void vren_teardown(hipError_t) {}
+
 static inline uint64_t ceil_div_u64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 static inline uint32_t next_pow2_u32(uint32_t v)
 {

@@ -81,11 +81,11 @@ VREN_EXPORT void vren_destroy(vren_ctx* ctx)
         delete ctx;
         return;
     }
-    hipSetDevice(ctx->device);
-    hipDeviceSynchronize();
-    if (ctx->d_flags) hipFree(ctx->d_flags);
-    if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->frame_slots) hipFree(ctx->frame_slots);
+    vren_teardown(hipSetDevice(ctx->device));
+    vren_teardown(hipDeviceSynchronize());
+    if (ctx->d_flags) vren_teardown(hipFree(ctx->d_flags));
+    if (ctx->scratch) vren_teardown(hipFree(ctx->scratch));
+    if (ctx->frame_slots) vren_teardown(hipFree(ctx->frame_slots));
     delete ctx;
 }
 
