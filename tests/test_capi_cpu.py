@@ -66,8 +66,11 @@ def test_calc_bvh_known_answers_library(lib, oracle):
         assert lib.vren_calc_bvh_root_index(n) == oracle.calc("root_index", n)
         assert lib.vren_calc_bvh_level_count(n) == oracle.calc("level_count", n)
         assert lib.vren_calc_bvh_buffer_size(n) == 32 * oracle.calc("buffer_length", n)
-    assert lib.vren_calc_bvh_padded_leaf_count(2193819) == 33554432
-    assert lib.vren_calc_bvh_level_count(2193819) == 5
+    from test_oracle_primitives import golden
+    kat = golden("build_bvh_utils.json")
+    for fn in ("padded_leaf_count", "buffer_length", "root_index", "level_count"):
+        for n, v in kat[fn]:
+            assert getattr(lib, "vren_calc_bvh_" + fn)(n) == v, (fn, n)
 
 
 CAMERAS = [dict(W=1920, H=1080), dict(W=640, H=360, fov_deg=60.0, pos=(3.0, -2.0, 7.5)),

@@ -13,9 +13,9 @@ BVH_DTYPE = np.dtype([("min", "<f4", 3), ("next", "<u4"), ("max", "<f4", 3), ("p
 
 
 def test_pcg32_published_vector(oracle):
-    # pcg32-demo, initstate 42, initseq 54 (O'Neill's reference implementation output)
-    expect = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
-    assert list(oracle.pcg32(42, 54, 6)) == expect
+    # pcg32-demo, initstate 42, initseq 54 (O'Neill's reference implementation output), from tests/golden
+    g = golden("pcg32_demo.json")
+    assert list(oracle.pcg32(g["initstate"], g["initseq"], len(g["outputs"]))) == g["outputs"]
 
 
 def test_pcg32_jump_ahead_matches_sequential(oracle):
@@ -206,18 +206,20 @@ def test_bucket_sort_vs_numpy(oracle):
         assert np.array_equal(offs, np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.uint32))
 
 
+def golden(name):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)) as f:
+        return json.load(f)
+
+
 def test_calc_bvh_known_answers(oracle):
-    # build_bvh.utils (vren_test/vren_test/primitives/build_bvh.cpp:223-254)
-    for n, v in [(0, 32), (1, 32), (17, 32), (32, 32), (129, 1024), (582, 1024), (1024, 1024), (2193819, 33554432)]:
-        assert oracle.calc("padded_leaf_count", n) == v
-    L3 = 32 * 32 + 32 + 1
-    L5 = 32 ** 5 + 32 ** 4 + 32 ** 3 + 32 ** 2 + 32 + 1
-    for n, v in [(0, 33), (1, 33), (17, 33), (32, 33), (129, L3), (582, L3), (1024, L3), (2193819, L5)]:
-        assert oracle.calc("buffer_length", n) == v
-    for n, v in [(0, 32), (1, 32), (129, L3 - 1), (582, L3 - 1), (2193819, L5 - 1)]:
-        assert oracle.calc("root_index", n) == v
-    for n, v in [(0, 1), (1, 1), (129, 2), (582, 2), (2193819, 5)]:
-        assert oracle.calc("level_count", n) == v
+    # build_bvh.utils (vren_test/vren_test/primitives/build_bvh.cpp:223-254), from tests/golden
+    kat = golden("build_bvh_utils.json")
+    for fn in ("padded_leaf_count", "buffer_length", "root_index", "level_count"):
+        assert len(kat[fn]) >= 5
+        for n, v in kat[fn]:
+            assert oracle.calc(fn, n) == v, (fn, n)
 
 
 def random_leaves(n, padded, length, rng):
