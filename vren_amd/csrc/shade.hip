@@ -24,6 +24,16 @@ constexpr float kInvPi = 1.0f / 3.14f;
 // quotient of the forward shade goes through it, which keeps it within ~1e-6 of the FP64 evaluation
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// f16 -> f32 of a normal component on the conversion unit: exact for every finite value and for +-inf; a NaN
+// component gives a NaN radiance either way (only its payload could differ from the oracle's software conversion)
+__device__ __forceinline__ float normal_h2f(uint32_t h)
+{
+    const uint16_t b = (uint16_t)h;
+    _Float16 x;
+    __builtin_memcpy(&x, &b, 2);
+    return (float)x;
+}
+
 struct PixelIn {
     double P[3];
     bool shaded;
@@ -126,12 +136,12 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     Surface s;
     {
         const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
-        const float il = rsqrtf(dot3(V, V));
+        const float il = __builtin_amdgcn_rsqf(dot3(V, V)); // |V| is scene-sized, never subnormal
         for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
     }
-    s.N[0] = vren_h2f((uint16_t)(nb.x & 0xFFFFu));
-    s.N[1] = vren_h2f((uint16_t)(nb.x >> 16));
-    s.N[2] = vren_h2f((uint16_t)(nb.y & 0xFFFFu));
+    s.N[0] = normal_h2f(nb.x & 0xFFFFu);
+    s.N[1] = normal_h2f(nb.x >> 16);
+    s.N[2] = normal_h2f(nb.y & 0xFFFFu);
     s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
     s.albedo[0] = ar.x;
     s.albedo[1] = ar.y;
@@ -609,12 +619,12 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     if (px.shaded) {
         {
             const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
-            const float il = rsqrtf(dot3(V, V));
+            const float il = __builtin_amdgcn_rsqf(dot3(V, V)); // |V| is scene-sized, never subnormal
             for (int i = 0; i < 3; i++) s.V[i] = V[i] * il;
         }
-        s.N[0] = vren_h2f((uint16_t)(nb.x & 0xFFFFu));
-        s.N[1] = vren_h2f((uint16_t)(nb.x >> 16));
-        s.N[2] = vren_h2f((uint16_t)(nb.y & 0xFFFFu));
+        s.N[0] = normal_h2f(nb.x & 0xFFFFu);
+        s.N[1] = normal_h2f(nb.x >> 16);
+        s.N[2] = normal_h2f(nb.y & 0xFFFFu);
         s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
         s.albedo[0] = ar.x;
         s.albedo[1] = ar.y;
