@@ -372,6 +372,59 @@ def test_batch_modes_identical_and_match_oracle(vctx, co):
     b.close()
 
 
+def test_batch_config4_64_heterogeneous_frames(vctx, co):
+    """The batched configuration at full size: 64 frames, resolutions drawn from {640x360, 1280x720, 1920x1080,
+    2560x1440, 3840x2160}, light counts log-uniform in [256, 65536]. Per-frame, batched and graph launches give the
+    same bytes for every frame (compared on the device); the frame with the most lights is checked against the
+    oracle (keys, references, light lists)."""
+    import torch
+    import vren_amd
+    rng = np.random.default_rng(4444)
+    res = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+    specs = []
+    for f in range(64):
+        W, H = res[rng.integers(0, len(res))]
+        L = int(round(float(np.exp(rng.uniform(np.log(256.0), np.log(65536.0))))))
+        specs.append((W, H, L, 7000 + f))
+    print("\n[batch64] %d frames, %.1f M pixels, lights %d..%d" % (
+        len(specs), sum(w * h for w, h, _, _ in specs) / 1e6, min(s[2] for s in specs), max(s[2] for s in specs)),
+          flush=True)
+    descs, keep = make_frames_batch(vctx, co, specs)
+    b = vren_amd.Batch(vctx, descs, ol.cfg())
+    first = None
+    for mode in (b.PER_FRAME, b.BATCHED, b.GRAPH, b.GRAPH):
+        for k in keep:
+            k["o"].fill_(0xFF)
+        b.run(mode)
+        torch.cuda.synchronize()
+        if first is None:
+            first = [k["o"].clone() for k in keep]
+        else:
+            for i, (a, k) in enumerate(zip(first, keep)):
+                assert torch.equal(a, k["o"]), (mode, i)
+    vctx.sync_status()
+    del first
+    i = max(range(len(specs)), key=lambda j: (specs[j][2], -specs[j][0]))
+    k = keep[i]
+    W, H, L = k["W"], k["H"], k["L"]
+    depth = from_dev(k["d"], np.float32, (W * H,))
+    normals = from_dev(k["n"], np.uint16, (W * H, 4))
+    pos = from_dev(k["p"], np.float32, (L, 4))
+    lig = from_dev(k["l"], np.float32, (L, 4))
+    okeys, oU, orefs = co.unique_clusters(W, H, k["cam"], ol.cfg(), depth, normals)
+    ovp, ocodes, obvh = co.light_bvh(pos, lig, k["cam"], ol.cfg())
+    oc, oo, oi, ot = co.assign(W, H, k["cam"], ol.cfg(), okeys, oU, obvh, ocodes, ovp)
+    o = b.outputs(i)
+    assert int(_read_ptr(o.cluster_count, 4, np.uint32)[0]) == oU
+    assert np.array_equal(_read_ptr(o.cluster_keys, 4 * oU, np.uint32), okeys)
+    assert np.array_equal(_read_ptr(o.cluster_refs, 4 * W * H, np.uint32), orefs)
+    assert np.array_equal(_read_ptr(o.light_counts, 4 * oU, np.uint32), oc)
+    assert np.array_equal(_read_ptr(o.light_offsets, 4 * oU, np.uint32), oo)
+    assert int(_read_ptr(o.assigned_total, 4, np.uint32)[0]) == int(ot)
+    assert np.array_equal(_read_ptr(o.light_indices, 4 * int(ot), np.uint32), oi)
+    b.close()
+
+
 def _read_ptr(ptr, nbytes, dtype):
     """Copy nbytes from a raw device pointer to the host (hipMemcpy through the HIP runtime torch loaded)."""
     import ctypes
