@@ -72,6 +72,8 @@ def make_camera(vren_amd):
 
 
 def build_batch(ctx, vren_amd, n_frames):
+    """The bench's frames and a Batch over them (reference key encoder); `keep` holds each frame's tensors and,
+    last, the list of frame descriptors (for another Batch over the same frames)."""
     import torch
     cam = make_camera(vren_amd)
     keep, descs = [], []
@@ -96,6 +98,7 @@ def build_batch(ctx, vren_amd, n_frames):
         descs.append(fd)
         keep.append((d, n, a, m, p, l, o))
     torch.cuda.synchronize()
+    keep.append(descs)
     return vren_amd.Batch(ctx, descs, vren_amd.default_cfg()), keep
 
 
@@ -123,7 +126,7 @@ def shade_pairs(ctx, vren_amd, batch, keep):
     inv = np.linalg.inv(proj.astype(np.float64).reshape(4, 4).T)  # stored column-major
     bg = float(np.float32(cam.far_plane) - np.float32(0.001))
     total = 0
-    for f, k in enumerate(keep):
+    for f, k in enumerate(keep[:batch.n]):
         o = batch.outputs(f)
         d = k[0].cpu().numpy().astype(np.float64)
         z = (inv[2, 2] * d + inv[2, 3]) / (inv[3, 2] * d + inv[3, 3])
@@ -464,6 +467,27 @@ def main():
                   "pairs_per_pixel": round(pairs / pixels, 3), "ops_per_pair": SHADE_OPS_PER_PAIR,
                   "launch_ms": kernel_ms["shade"]}
 
+    # the same frames with the configuration's cluster-key encoder (32 log-depth slices x 6 normal-cone bins)
+    # instead of the reference's (1024 log-depth slices x 64 normal codes)
+    cfg32 = vren_amd.default_cfg(key_mode=vren_amd.KEYS_SLICED, depth_slices=32)
+    batch32 = vren_amd.Batch(ctx, keep[-1], cfg32)
+    for _ in range(max(1, args.warmup)):
+        batch32.run(batch32.GRAPH)
+    torch.cuda.synchronize()
+    ms32, _ = time_events(lambda: batch32.run(batch32.GRAPH), args.steps)
+    for _ in range(max(1, args.warmup)):
+        batch32.run(batch32.PER_FRAME)
+    _, wall32 = time_events(lambda: batch32.run(batch32.PER_FRAME), plain_steps)
+    ctx.sync_status()
+    fps32 = args.frames * args.steps / (ms32 * 1e-3)
+    plain32 = args.frames * plain_steps / (wall32 * 1e-3)
+    sliced = {"key_encoder": "32x32 tile x 32 log-depth slices x 6 normal-cone bins", "frames_per_s": round(fps32, 2),
+              "ms_per_step": round(ms32 / args.steps, 4), "plain_frames_per_s": round(plain32, 2),
+              "batched_over_plain": round(fps32 / plain32, 3)}
+    batch32.close()
+    print("[bench] 32-slice keys: %.3f ms/step batched, %.1f frames/s plain" % (ms32 / args.steps, plain32),
+          file=sys.stderr, flush=True)
+
     sort = None
     if not args.no_sort:
         print("[bench] sort 2^%d pairs" % args.sort_log2, file=sys.stderr, flush=True)
@@ -487,9 +511,11 @@ def main():
                                "(light BVH + unique clusters + BVH light assignment + PBR shade), hipGraph replay"
                                % args.frames,
                    "frames_per_step": args.frames, "resolution": "1920x1080", "lights_per_frame": L_LIGHTS,
+                   "key_encoder": "reference (32x32 tile x 1024 log-depth slices x 64 normal codes)",
                    "parallelism": "1 GPU"},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(value / plain_fps, 3),
+        "config_key_encoder": sliced,
         "kernel_ms": kernel_ms,
         "roofline": roofline,
         "shade_valu": shade_valu,
