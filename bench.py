@@ -305,6 +305,11 @@ def bench_shade_frame(ctx, vren_amd, reps):
                             L_LIGHTS, img)
     bwd = lambda: ctx.shade_backward(W, H, cam, cf, d, nrm, ar, me, refs, key_cap, counts, offsets, idx, index_cap,
                                      lp, lc, L_LIGHTS, grad, g_pos, g_lig, g_ar)
+    # Karras LBVH over the frame's 4096 sorted (Morton code, light index) pairs: 4095 internal nodes
+    lb_nodes, lb_parents = u8(32 * (2 * L_LIGHTS - 1)), u8(4 * (2 * L_LIGHTS - 1))
+    ms_lb = _median_ms(lambda: ctx.build_lbvh(codes, L_LIGHTS, vp, lc, lb_nodes, lb_parents), reps)
+    res["lbvh_4096_lights"] = {"ms": round(ms_lb, 4), "lights": L_LIGHTS, "nodes": 2 * L_LIGHTS - 1,
+                               "bound": "latency (bottom-up refit along the tree depth)"}
     ms_f = _median_ms(fwd, reps)
     ms_b = _median_ms(bwd, reps, before=clear)
     torch.cuda.synchronize()
