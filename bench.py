@@ -385,6 +385,30 @@ def _cpu_model():
     return "unknown"
 
 
+def bench_key_variant(ctx, vren_amd, descs, args, plain_steps):
+    """Batched and plain frames/s of the bench's frames with the 32x32 tile x 32 log-depth slices x 6 normal-cone
+    bins cluster-key encoder."""
+    import torch
+    cfg32 = vren_amd.default_cfg(key_mode=vren_amd.KEYS_SLICED, depth_slices=32)
+    batch32 = vren_amd.Batch(ctx, descs, cfg32)
+    for _ in range(max(1, args.warmup)):
+        batch32.run(batch32.GRAPH)
+    torch.cuda.synchronize()
+    ms32, _ = time_events(lambda: batch32.run(batch32.GRAPH), args.steps)
+    for _ in range(max(1, args.warmup)):
+        batch32.run(batch32.PER_FRAME)
+    _, wall32 = time_events(lambda: batch32.run(batch32.PER_FRAME), plain_steps)
+    ctx.sync_status()
+    batch32.close()
+    fps32 = args.frames * args.steps / (ms32 * 1e-3)
+    plain32 = args.frames * plain_steps / (wall32 * 1e-3)
+    print("[bench] 32-slice keys: %.3f ms/step batched, %.1f frames/s plain" % (ms32 / args.steps, plain32),
+          file=sys.stderr, flush=True)
+    return {"key_encoder": "32x32 tile x 32 log-depth slices x 6 normal-cone bins", "frames_per_s": round(fps32, 2),
+            "ms_per_step": round(ms32 / args.steps, 4), "plain_frames_per_s": round(plain32, 2),
+            "batched_over_plain": round(fps32 / plain32, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +420,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip reduce, scan, bucket sort and the shade adjoint")
+    ap.add_argument("--no-key-variant", action="store_true",
+                    help="skip the second batch with the 32-slice x 6-bin cluster-key encoder")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB with one plain device copy kernel (scales HBM counters)")
     args = ap.parse_args()
@@ -474,24 +500,9 @@ def main():
 
     # the same frames with the configuration's cluster-key encoder (32 log-depth slices x 6 normal-cone bins)
     # instead of the reference's (1024 log-depth slices x 64 normal codes)
-    cfg32 = vren_amd.default_cfg(key_mode=vren_amd.KEYS_SLICED, depth_slices=32)
-    batch32 = vren_amd.Batch(ctx, keep[-1], cfg32)
-    for _ in range(max(1, args.warmup)):
-        batch32.run(batch32.GRAPH)
-    torch.cuda.synchronize()
-    ms32, _ = time_events(lambda: batch32.run(batch32.GRAPH), args.steps)
-    for _ in range(max(1, args.warmup)):
-        batch32.run(batch32.PER_FRAME)
-    _, wall32 = time_events(lambda: batch32.run(batch32.PER_FRAME), plain_steps)
-    ctx.sync_status()
-    fps32 = args.frames * args.steps / (ms32 * 1e-3)
-    plain32 = args.frames * plain_steps / (wall32 * 1e-3)
-    sliced = {"key_encoder": "32x32 tile x 32 log-depth slices x 6 normal-cone bins", "frames_per_s": round(fps32, 2),
-              "ms_per_step": round(ms32 / args.steps, 4), "plain_frames_per_s": round(plain32, 2),
-              "batched_over_plain": round(fps32 / plain32, 3)}
-    batch32.close()
-    print("[bench] 32-slice keys: %.3f ms/step batched, %.1f frames/s plain" % (ms32 / args.steps, plain32),
-          file=sys.stderr, flush=True)
+    sliced = None
+    if not args.no_key_variant:
+        sliced = bench_key_variant(ctx, vren_amd, keep[-1], args, plain_steps)
 
     sort = None
     if not args.no_sort:
