@@ -125,14 +125,72 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
 constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup
 constexpr int kShadePPT = 1;     // pixels per thread of the forward shade (2 measured slower)
 
+// The pixel's list entries [off, end): light records from the LDS stage (STAGED, entry e at e - start, the
+// position's w holding 1 / radius) or from global memory; SMOOTH selects the windowed attenuation.
+template <bool STAGED, bool SMOOTH>
+__device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px, const Surface& s, uint32_t off,
+                                           uint32_t end, uint32_t start, const float4* s_lpos, const float4* s_lcol,
+                                           float acc[3])
+{
+    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
+    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
+    const float Pf[3] = {(float)px.P[0], (float)px.P[1], (float)px.P[2]};
+    for (uint32_t e = off; e < end; e++) {
+        float4 lp, lc;
+        if constexpr (STAGED) {
+            lp = s_lpos[e - start];
+            lc = s_lcol[e - start];
+        } else {
+            const uint32_t li = fd.indices[e];
+            if (li >= fd.L) continue;
+            lp = lpos[li];
+            lc = lcol[li];
+            lp.w = frcp(lc.w);
+        }
+        // lights clearly beyond their radius contribute exactly 0: a float distance (error < 1e-3 at scene
+        // scale) rejects them before the double evaluation
+        {
+            const float fx = Pf[0] - lp.x, fy = Pf[1] - lp.y, fz = Pf[2] - lp.z;
+            if (fx * fx + fy * fy + fz * fz > lc.w * lc.w * 1.0001f + 1e-3f) continue;
+        }
+        // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
+        // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
+        const double d[3] = {px.P[0] - (double)lp.x, px.P[1] - (double)lp.y, px.P[2] - (double)lp.z};
+        const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+        const float r = lc.w;
+        const double rr = (double)r * (double)r;
+        if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < rr)) continue;
+        const float d2f = (float)dist2;
+        const float inv = __builtin_amdgcn_rsqf(d2f); // 1 / |d|
+        const float s0 = d2f * inv;                     // |d|
+        // 1 - x = (r - |d|) / r = (r^2 - |d|^2) / ((r + |d|) r): the difference that cancels near the radius is
+        // taken in double, the rest needs only float accuracy
+        const float num = (float)(rr - dist2);
+        float att;
+        if constexpr (SMOOTH) {
+            // (1 - x^4)^2 = ((1 - x)(1 + x)(1 + x^2))^2
+            const float omx = num * frcp((r + s0) * r);
+            const float x = 1.0f - omx;
+            const float w = omx * (1.0f + x) * (1.0f + x * x);
+            att = w * w;
+        } else {
+            // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5), with x = |d| / r (lp.w holds 1 / r)
+            const float x = s0 * lp.w;
+            att = num * (1.0f + x + x * x) * frcp((r + s0) * r * (1.0f + x * __builtin_amdgcn_sqrtf(x)));
+        }
+        if (!(att > 0.0f)) continue;
+        const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
+        const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
+        apply_light(s, L, rad, acc);
+    }
+}
+
 // Shades one pixel whose inputs are loaded: its list entries [off, end) come from the LDS stage (when `staged`,
 // entry e at e - start) or from global memory.
 __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar, float metal,
                                             bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
                                             const float4* s_lpos, const float4* s_lcol, uint64_t p)
 {
-    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
-    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
     Surface s;
     {
         const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
@@ -155,54 +213,15 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
 
     float acc[3] = {0.0f, 0.0f, 0.0f};
     if (has_list) {
-        const float Pf[3] = {(float)px.P[0], (float)px.P[1], (float)px.P[2]};
-        for (uint32_t e = off; e < end; e++) {
-            float4 lp, lc;
-            if (staged) {
-                lp = s_lpos[e - start];
-                lc = s_lcol[e - start];
-            } else {
-                const uint32_t li = fd.indices[e];
-                if (li >= fd.L) continue;
-                lp = lpos[li];
-                lc = lcol[li];
-                lp.w = frcp(lc.w);
-            }
-            // lights clearly beyond their radius contribute exactly 0: a float distance (error < 1e-3 at scene
-            // scale) rejects them before the double evaluation
-            {
-                const float fx = Pf[0] - lp.x, fy = Pf[1] - lp.y, fz = Pf[2] - lp.z;
-                if (fx * fx + fy * fy + fz * fz > lc.w * lc.w * 1.0001f + 1e-3f) continue;
-            }
-            // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
-            // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
-            const double d[3] = {px.P[0] - (double)lp.x, px.P[1] - (double)lp.y, px.P[2] - (double)lp.z};
-            const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-            const float r = lc.w;
-            const double rr = (double)r * (double)r;
-            if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < rr)) continue;
-            const float d2f = (float)dist2;
-            const float inv = __builtin_amdgcn_rsqf(d2f); // 1 / |d|
-            const float s0 = d2f * inv;                     // |d|
-            // 1 - x = (r - |d|) / r = (r^2 - |d|^2) / ((r + |d|) r): the difference that cancels near the radius is
-            // taken in double, the rest needs only float accuracy
-            const float num = (float)(rr - dist2);
-            float att;
-            if (fd.attenuation == VREN_ATTEN_SMOOTH) {
-                // (1 - x^4)^2 = ((1 - x)(1 + x)(1 + x^2))^2
-                const float omx = num * frcp((r + s0) * r);
-                const float x = 1.0f - omx;
-                const float w = omx * (1.0f + x) * (1.0f + x * x);
-                att = w * w;
-            } else {
-                // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5), with x = |d| / r (lp.w holds 1 / r)
-                const float x = s0 * lp.w;
-                att = num * (1.0f + x + x * x) * frcp((r + s0) * r * (1.0f + x * __builtin_amdgcn_sqrtf(x)));
-            }
-            if (!(att > 0.0f)) continue;
-            const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
-            const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
-            apply_light(s, L, rad, acc);
+        // one loop per (staging, attenuation) pair: both are uniform over the workgroup, so no entry pays for the
+        // other branch's registers or its test
+        const bool smooth = fd.attenuation == VREN_ATTEN_SMOOTH;
+        if (staged) {
+            if (smooth) light_loop<true, true>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
+            else light_loop<true, false>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
+        } else {
+            if (smooth) light_loop<false, true>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
+            else light_loop<false, false>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
         }
     }
     for (uint32_t i = 0; i < fd.n_dir; i++) {
@@ -255,7 +274,8 @@ __global__ __launch_bounds__(256) void tile_range_kernel(const FrameDev* __restr
 // the light records beside the pixels' list bounds: two dependent rounds of loads. Without it (references from the
 // caller), the workgroup first reduces the run from its pixels' lists.
 template <int PPT>
-__global__ __launch_bounds__(256) void shade_kernel(const FrameDev* __restrict__ frames,
+// at most 64 VGPRs: eight waves per SIMD (the split light loops would otherwise take 66 and seven)
+__global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restrict__ frames,
                                                     const uint32_t* __restrict__ tile_frame,
                                                     const uint2* __restrict__ tile_range)
 {
