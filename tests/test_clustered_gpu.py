@@ -319,7 +319,8 @@ def make_frames_batch(vctx, co, specs):
 def test_batch_modes_identical_and_match_oracle(vctx, co):
     import torch
     import vren_amd
-    specs = [(640, 360, 256, 4000), (1280, 720, 5000, 4001), (333, 200, 1, 4002), (1920, 1080, 4096, 4003)]
+    specs = [(640, 360, 256, 4000), (1280, 720, 5000, 4001), (333, 200, 1, 4002), (1920, 1080, 4096, 4003),
+             (160, 90, 0, 4004)]
     descs, keep = make_frames_batch(vctx, co, specs)
     b = vren_amd.Batch(vctx, descs, ol.cfg())
     results = []
@@ -369,6 +370,24 @@ def test_batch_modes_identical_and_match_oracle(vctx, co):
     assert int(_read_ptr(o.cluster_count, 4, np.uint32)[0]) == oU
     assert np.array_equal(_read_ptr(o.cluster_keys, 4 * oU, np.uint32), okeys)
     assert np.array_equal(_read_ptr(o.cluster_refs, 4 * W * H, np.uint32), orefs)
+    # frame 4 has no lights: the radiance is the ambient term alone, (0.03 albedo)^(1/2.2) on every shaded pixel,
+    # and the background stays untouched
+    k = keep[4]
+    W, H = k["W"], k["H"]
+    depth = from_dev(k["d"], np.float32, (W * H,))
+    normals = from_dev(k["n"], np.uint16, (W * H, 4))
+    ar = from_dev(k["a"], np.float32, (W * H, 4))
+    me = from_dev(k["m"], np.float32, (W * H,))
+    _, _, orefs = co.unique_clusters(W, H, k["cam"], ol.cfg(), depth, normals)
+    kc = 1 << 20
+    zero = np.zeros(kc, np.uint32)
+    ref, mask = co.shade(W, H, k["cam"], ol.cfg(), depth, normals, ar, me, orefs, kc, zero, zero,
+                         np.zeros(1, np.uint32), np.zeros((0, 4), np.float32), np.zeros((0, 4), np.float32))
+    got = results[0][4].view(np.float32).reshape(-1, 4)
+    assert np.all(np.isnan(got[~mask])) and mask.sum() > 0
+    amb = (0.03 * ar[mask, :3].astype(np.float64)) ** (1.0 / 2.2)
+    assert np.abs(got[mask, :3] - amb).max() <= SHADE_RTOL * amb.max()
+    assert shade_error_report(got, ref, mask).max() <= SHADE_RTOL
     b.close()
 
 
