@@ -26,6 +26,8 @@ struct vren_batch {
     ChainWork w;                    // batch-wide
     std::vector<ChainWork> w_frame; // per-frame views of the same memory
     std::vector<uint32_t> leaf_slots;
+    std::vector<uint32_t> live_slots; // per frame: leaf slots of the groups that hold lights (256-aligned)
+    uint32_t total_live_slots = 0;
     uint32_t* d_tile_frame = nullptr; // frame of every flattened tile
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -36,13 +38,15 @@ namespace {
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// live_slots: the leaf-kernel grid over the frames' light-holding leaf groups (their padding groups were written
+// when the batch was created)
 int run_chain(vren_batch* b, hipStream_t s, const FrameDev* table, uint32_t F, const ChainWork& w,
-              uint32_t leaf_slots)
+              uint32_t live_slots)
 {
     vren_ctx* ctx = b->ctx;
     int st = launch_chain_init(ctx, s, w);
     if (!st) st = launch_lights(ctx, s, table, F, w);
-    if (!st && leaf_slots) st = launch_bvh(ctx, s, table, F, leaf_slots, true, &w);
+    if (!st && live_slots) st = launch_bvh(ctx, s, table, F, live_slots, true, &w, kLeavesLive);
     if (!st) st = launch_clusters(ctx, s, table, F, w);
     if (!st) st = launch_assign(ctx, s, table, F, w);
     if (!st) st = launch_shade(ctx, s, table, F, w);
@@ -81,8 +85,9 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->F = n_frames;
     b->host.resize(n_frames);
     b->leaf_slots.resize(n_frames);
+    b->live_slots.resize(n_frames);
     // sizes and arena layout
-    uint32_t tile_base = 0, leaf_base = 0, offs_base = 0, max_L = 0;
+    uint32_t tile_base = 0, leaf_base = 0, live_base = 0, offs_base = 0, max_L = 0;
     std::vector<size_t> off(n_frames * 14);
     size_t arena = 0;
     for (uint32_t i = 0; i < n_frames; i++) {
@@ -98,6 +103,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         if (index_cap == 0) index_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)key_cap * L, 1), 1u << 24);
         f.tile_base = tile_base;
         f.leaf_base = leaf_base;
+        f.live_leaf_base = live_base;
         f.offs_base = offs_base;
         f.L = L;
         f.light_pos = d.light_positions;
@@ -118,8 +124,11 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.bvh_length = L ? vren_calc_bvh_buffer_length(L) : 0;
         const uint32_t slots = L ? (f.bvh_padded + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN : 0;
         b->leaf_slots[i] = slots;
+        const uint32_t live = (std::min(slots, (L + 31u) & ~31u) + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN;
+        b->live_slots[i] = live;
         tile_base += (tiles + VREN_TILE_GROUP - 1) / VREN_TILE_GROUP * VREN_TILE_GROUP;
         leaf_base += slots;
+        live_base += live;
         offs_base += (key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
         max_L = std::max(max_L, L);
         const uint32_t slot_cap = std::min<uint32_t>(key_cap, 1u << 16);
@@ -179,6 +188,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     chain_work_bind(&b->w, b->work, tile_base, offs_base, n_frames);
     b->w.tile_frame = b->d_tile_frame;
     b->w.total_leaf_slots = leaf_base;
+    b->total_live_slots = live_base;
     b->w.max_L = max_L;
     // per-frame tables: same buffers, bases 0, the workspace viewed from its start
     std::vector<FrameDev> single(b->host);
@@ -186,6 +196,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     for (uint32_t i = 0; i < n_frames; i++) {
         single[i].tile_base = 0;
         single[i].leaf_base = 0;
+        single[i].live_leaf_base = 0;
         single[i].offs_base = 0;
         ChainWork& wf = b->w_frame[i];
         chain_work_bind(&wf, b->work, single[i].Sx * single[i].Sy, (single[i].key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE,
@@ -196,6 +207,8 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     hipStream_t s = nullptr;
     st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);
     if (!st) st = vren_upload_frames(ctx, s, b->d_single, single.data(), n_frames);
+    // the BVH leaf groups past each frame's last light, written once: every run of the chain skips them
+    if (!st && leaf_base) st = launch_bvh(ctx, s, b->d_table, n_frames, leaf_base, true, nullptr, kLeavesPadding);
     if (!st && hipStreamSynchronize(s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch: upload failed");
     if (st) {
         vren_batch_destroy(b);
@@ -212,18 +225,18 @@ VREN_EXPORT int vren_batch_run(vren_batch* b, vren_stream_t stream, int mode)
     hipStream_t s = (hipStream_t)stream;
     if (mode == VREN_RUN_PER_FRAME) {
         for (uint32_t i = 0; i < b->F; i++) {
-            int st = run_chain(b, s, b->d_single + i, 1, b->w_frame[i], b->leaf_slots[i]);
+            int st = run_chain(b, s, b->d_single + i, 1, b->w_frame[i], b->live_slots[i]);
             if (st) return st;
             VREN_HIP(ctx, hipStreamSynchronize(s));
         }
         return VREN_OK;
     }
-    if (mode == VREN_RUN_BATCHED) return run_chain(b, s, b->d_table, b->F, b->w, b->w.total_leaf_slots);
+    if (mode == VREN_RUN_BATCHED) return run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots);
     if (mode == VREN_RUN_GRAPH) {
         if (!b->exec) {
             VREN_HIP(ctx, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
             VREN_HIP(ctx, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-            int st = run_chain(b, b->capture, b->d_table, b->F, b->w, b->w.total_leaf_slots);
+            int st = run_chain(b, b->capture, b->d_table, b->F, b->w, b->total_live_slots);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->capture, &g);
             if (st) return st;
@@ -249,7 +262,7 @@ VREN_EXPORT int vren_batch_run_profiled(vren_batch* b, vren_stream_t stream, flo
     w.events = ev + 1;
     w.n_events = &n;
     VREN_HIP(ctx, hipEventRecord(ev[0], s));
-    int st = run_chain(b, s, b->d_table, b->F, w, b->w.total_leaf_slots);
+    int st = run_chain(b, s, b->d_table, b->F, w, b->total_live_slots);
     if (!st) VREN_HIP(ctx, hipEventSynchronize(ev[n]));
     for (int i = 0; i < VREN_CHAIN_KERNELS; i++) {
         kernel_ms[i] = 0.0f;

@@ -61,8 +61,10 @@ void chain_work_bind(ChainWork* w, void* mem, uint32_t total_tiles, uint32_t tot
 // stage launchers; d_frames is a device table of F FrameDev records
 int launch_chain_init(vren_ctx* ctx, hipStream_t s, const ChainWork& w);
 int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
+// which 32-leaf groups the leaf kernel writes: all, those holding a light, or the all-padding ones
+enum : uint32_t { kLeavesAll = 0, kLeavesLive = 1, kLeavesPadding = 2 };
 int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_leaf_slots,
-               bool leaves_from_lights, const ChainWork* w);
+               bool leaves_from_lights, const ChainWork* w, uint32_t span);
 int launch_clusters(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);
 int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w);

@@ -16,6 +16,7 @@ struct FrameDev {
     uint32_t W, H, Sx, Sy;
     uint32_t tile_base;     // first flattened 32x32 tile of this frame in the table
     uint32_t leaf_base;     // first flattened leaf slot (256-aligned) of this frame's BVH leaves
+    uint32_t live_leaf_base; // the same over the leaf groups that hold lights only (a batch's chain launch)
     uint32_t offs_base;     // first flattened tile of this frame's light-offset scan
     // camera constants
     float view[16];         // camera::get_view()

@@ -211,15 +211,32 @@ __device__ __forceinline__ void store_node(vren_bvh_node* n, const float (&mn)[3
     q[1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
 }
 
-// leaves (init_light_array_bvh.comp:42-65) + the first parent level (build_bvh.comp:32-55)
-__global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* __restrict__ frames, uint32_t F, bool from_lights)
+// leaves (init_light_array_bvh.comp:42-65) + the first parent level (build_bvh.comp:32-55). `span` selects the
+// 32-leaf groups written: every group, only those holding a light (the groups past the last light are all INVALID
+// leaves under an INVALID parent, which depend on the light count alone: a batch writes them once, with
+// kLeavesPadding, when it is created), or only those padding groups.
+__global__ __launch_bounds__(256) void bvh_leaf_kernel(const FrameDev* __restrict__ frames, uint32_t F, bool from_lights,
+                                                       uint32_t span)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t f = find_frame(frames, F, blockIdx.x * 256, [](const FrameDev& d) { return d.leaf_base; });
+    // the live span's grid covers each frame's light-holding groups only (numbered by live_leaf_base)
+    const bool live_grid = span == kLeavesLive;
+    const uint32_t f = live_grid ? find_frame(frames, F, blockIdx.x * 256, [](const FrameDev& d) { return d.live_leaf_base; })
+                                 : find_frame(frames, F, blockIdx.x * 256, [](const FrameDev& d) { return d.leaf_base; });
     const FrameDev& fd = frames[f];
-    const uint32_t i = g - fd.leaf_base;
+    const uint32_t i = g - (live_grid ? fd.live_leaf_base : fd.leaf_base);
     const uint32_t padded = fd.bvh_padded;
     if (i >= padded) return; // whole half-waves leave together (padded is a multiple of 32)
+    const uint32_t live_end = (fd.L + 31u) & ~31u; // first 32-leaf group without a light
+    if (span == kLeavesLive && i >= live_end) return;
+    if (span == kLeavesPadding) {
+        if (i < live_end) return;
+        const float inf[3] = {VREN_F32_INF, VREN_F32_INF, VREN_F32_INF};
+        const float ninf[3] = {-VREN_F32_INF, -VREN_F32_INF, -VREN_F32_INF};
+        store_node(&fd.bvh[i], inf, ninf, VREN_BVH_INVALID_NODE);
+        if ((lane_id() & 31) == 0) store_node(&fd.bvh[padded + i / 32], inf, ninf, VREN_BVH_INVALID_NODE);
+        return;
+    }
     float mn[3], mx[3];
     bool valid;
     if (from_lights) {
@@ -309,11 +326,13 @@ int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     return VREN_OK;
 }
 
+// total_leaf_slots: the grid in leaf slots, the frames' live slots (live_leaf_base numbering) for kLeavesLive
 int launch_bvh(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, uint32_t total_leaf_slots,
-               bool leaves_from_lights, const ChainWork* w)
+               bool leaves_from_lights, const ChainWork* w, uint32_t span)
 {
-    bvh_leaf_kernel<<<total_leaf_slots / 256, 256, 0, s>>>(d_frames, F, leaves_from_lights);
+    bvh_leaf_kernel<<<total_leaf_slots / 256, 256, 0, s>>>(d_frames, F, leaves_from_lights, span);
     VREN_LAUNCH_CHECK(ctx);
+    if (span == kLeavesPadding) return VREN_OK;
     if (w) chain_mark(*w, s);
     bvh_upper_kernel<<<F, 1024, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
@@ -348,7 +367,7 @@ VREN_EXPORT int vren_build_bvh(vren_ctx* ctx, vren_stream_t stream, vren_bvh_nod
     int st = vren_upload_frames(ctx, s, slot, &f, 1);
     if (st) return st;
     uint32_t slots = (padded_leaf_count + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN;
-    return launch_bvh(ctx, s, slot, 1, slots, false, nullptr);
+    return launch_bvh(ctx, s, slot, 1, slots, false, nullptr, kLeavesAll);
 }
 
 VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stream, const float* light_positions,
@@ -393,7 +412,7 @@ VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stre
     st = launch_lights(ctx, s, slot, 1, w);
     if (st) return st;
     uint32_t slots = (f.bvh_padded + VREN_LEAF_ALIGN - 1) / VREN_LEAF_ALIGN * VREN_LEAF_ALIGN;
-    return launch_bvh(ctx, s, slot, 1, slots, true, nullptr);
+    return launch_bvh(ctx, s, slot, 1, slots, true, nullptr, kLeavesAll);
 }
 
 } // extern "C"
