@@ -254,7 +254,8 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     pairs_out = torch.empty_like(pairs_in)
     offs = torch.empty(65536, dtype=torch.int32, device="cuda")
     ms = _median_ms(lambda: ctx.bucket_sort(pairs_in, nb, pairs_out, offs, 65536, 0), reps)
-    res["bucket_sort_65536"] = _hbm(ms, 16 * nb, n=nb, algorithmic_bytes_per_pair=16)
+    # counting-sort unit: count read 8 B + scatter read 8 B + write 8 B per pair (pairs in and out alone: 16 B)
+    res["bucket_sort_65536"] = _hbm(ms, 24 * nb, n=nb, algorithmic_bytes_per_pair=24)
     del keys, pairs_in, pairs_out, offs
     res.update(bench_shade_frame(ctx, vren_amd, reps))
     return res

@@ -151,7 +151,8 @@ VREN_EXPORT int vren_autotune_load(vren_ctx* ctx, const char* path);
 /* bucket_sort::operator() (bucket_sort.hpp:9-45, bucket_sort.cpp:72-161): counting sort of (key, value)
  * uint32 pairs into n_buckets (a power of 2, <= 65536) buckets by (key >> key_shift) & (n_buckets - 1).
  * Stable (the reference's atomic scatter is not). bucket_offsets[n_buckets] receives the exclusive bucket
- * starts (the reference appends them to its output buffer, bucket_sort.cpp:67-70). */
+ * starts (the reference appends them to its output buffer, bucket_sort.cpp:67-70). Pair buffers are 8-byte
+ * aligned and distinct (VREN_EINVAL otherwise). */
 VREN_EXPORT int vren_bucket_sort(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in_pairs, uint32_t n,
                                  uint32_t* out_pairs, uint32_t* bucket_offsets, uint32_t n_buckets,
                                  uint32_t key_shift);

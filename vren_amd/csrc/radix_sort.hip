@@ -33,16 +33,24 @@ __global__ void rs_init_kernel(uint32_t* hist, uint32_t n_hist, uint32_t* counte
     for (uint32_t i = threadIdx.x; i < n_counters; i += blockDim.x) counters[i] = 0u;
 }
 
-template <int BITS>
-__global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint64_t n, uint32_t begin_bit,
-                                                      uint32_t end_bit, uint32_t passes, uint32_t* __restrict__ hist,
-                                                      uint32_t* __restrict__ clear, uint64_t clear_words)
+// one workgroup of 1024 threads per CU: the partial histograms are merged into the global one with agent-scope
+// atomics that all land in a few cache lines, so the number of workgroups is kept small
+constexpr uint32_t kHistThreads = 1024;
+
+// PAIRS: the keys are the first words of interleaved (key, value) pairs (key i at keys[2 i])
+template <int BITS, bool PAIRS = false>
+__global__ __launch_bounds__(kHistThreads) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint64_t n,
+                                                               uint32_t begin_bit, uint32_t end_bit, uint32_t passes,
+                                                               uint32_t* __restrict__ hist,
+                                                               uint32_t* __restrict__ clear, uint64_t clear_words)
 {
+    constexpr uint32_t KPV = PAIRS ? 2 : 4; // keys per 16-byte load
     constexpr int RADIX = 1 << BITS;
     __shared__ uint32_t s_h[8 * RADIX];
-    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) s_h[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kHistThreads) s_h[i] = 0u;
     // clear the first pass's look-back statuses while we are here
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < clear_words; i += (uint64_t)gridDim.x * 256)
+    for (uint64_t i = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; i < clear_words;
+         i += (uint64_t)gridDim.x * kHistThreads)
         clear[i] = 0u;
     __syncthreads();
     auto count = [&](uint32_t k) {
@@ -53,11 +61,17 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict
             atomicAdd(&s_h[p * RADIX + d], 1u);
         }
     };
+    auto count_vec = [&](uint4 q) {
+        count(q.x);
+        if (!PAIRS) count(q.y);
+        count(q.z);
+        if (!PAIRS) count(q.w);
+    };
     const bool aligned = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
-    const uint64_t n4 = aligned ? n / 4 : 0;
+    const uint64_t n4 = aligned ? n / KPV : 0;
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    const uint64_t stride = (uint64_t)gridDim.x * 256;
-    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
+    uint64_t i = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x;
     // eight 128-bit loads in flight per thread before the LDS counting
     constexpr int U = 8;
     for (; i + (U - 1) * stride < n4; i += U * stride) {
@@ -65,23 +79,13 @@ __global__ __launch_bounds__(256) void rs_hist_kernel(const uint32_t* __restrict
 #pragma unroll
         for (int u = 0; u < U; u++) q[u] = ntload4(k4 + i + u * stride);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            count(q[u].x);
-            count(q[u].y);
-            count(q[u].z);
-            count(q[u].w);
-        }
+        for (int u = 0; u < U; u++) count_vec(q[u]);
     }
-    for (; i < n4; i += stride) {
-        uint4 q = k4[i];
-        count(q.x);
-        count(q.y);
-        count(q.z);
-        count(q.w);
-    }
-    for (uint64_t j = n4 * 4 + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += stride) count(keys[j]);
+    for (; i < n4; i += stride) count_vec(k4[i]);
+    for (uint64_t j = n4 * KPV + (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; j < n; j += stride)
+        count(keys[PAIRS ? 2 * j : j]);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += 256) {
+    for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kHistThreads) {
         uint32_t c = s_h[i];
         if (c) atomic_add_agent(&hist[i], c);
     }
@@ -155,7 +159,9 @@ constexpr bool rs_sequential_staging()
 // RTS = false: onesweep (tiles from a ticket counter, per-digit decoupled look-back, digit_base = the digit's
 // first output position). RTS = true: reduce-then-scan scatter pass (tile = workgroup id; digit_base = the scanned
 // digit-major tile counts of rs_upsweep_kernel), no tickets and no waiting.
-template <int BITS, int KPT, int WAVES, bool VALS, bool RTS = false>
+// IN_PAIRS: keys_in holds interleaved (key, value) pairs and vals_in is unused. OUT_PAIRS: the pass writes
+// interleaved pairs to keys_out and vals_out is unused. Both need VALS.
+template <int BITS, int KPT, int WAVES, bool VALS, bool RTS = false, bool IN_PAIRS = false, bool OUT_PAIRS = false>
 __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                                  const uint32_t* __restrict__ vals_in,
                                                                  uint32_t* __restrict__ keys_out,
@@ -170,6 +176,7 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     constexpr int TILE = THREADS * KPT;
     constexpr bool SEQ = rs_sequential_staging<KPT, WAVES, VALS>();
     static_assert(THREADS >= RADIX, "one thread per digit");
+    static_assert(VALS || !(IN_PAIRS || OUT_PAIRS), "interleaved pairs carry values");
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[(VALS && !SEQ) ? TILE : 1];
     __shared__ uint32_t s_whist[WAVES * RADIX];
@@ -196,7 +203,17 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     const uint32_t* kin = keys_in + tile_start + wbase + lane;
     const uint32_t* vin = vals_in + tile_start + wbase + lane;
     uint32_t key[KPT], val[VALS ? KPT : 1], digit[KPT], local[KPT];
-    if (tile_count == (uint32_t)TILE) {
+    if (IN_PAIRS) {
+        const vu2* pin = reinterpret_cast<const vu2*>(keys_in) + tile_start + wbase + lane;
+        const bool full_tile = tile_count == (uint32_t)TILE;
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            vu2 q = {0xFFFFFFFFu, 0u};
+            if (full_tile || wbase + j * VREN_WAVE + lane < tile_count) q = __builtin_nontemporal_load(pin + j * VREN_WAVE);
+            key[j] = q.x;
+            val[j] = q.y;
+        }
+    } else if (tile_count == (uint32_t)TILE) {
 #pragma unroll
         for (int j = 0; j < KPT; j++) key[j] = __builtin_nontemporal_load(kin + j * VREN_WAVE);
     } else {
@@ -209,7 +226,7 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     for (int j = 0; j < KPT; j++) digit[j] = __builtin_amdgcn_ubfe(key[j], shift, width);
     rank_wave_keys<BITS, KPT, false>(digit, wbase, tile_count, &s_whist[wave * RADIX], local);
     // the values are read after the ranking: their loads overlap the counts, the look-back and the key stores
-    if (VALS) {
+    if (VALS && !IN_PAIRS) {
 #pragma unroll
         for (int j = 0; j < KPT; j++)
             val[j] = wbase + j * VREN_WAVE + lane < tile_count ? __builtin_nontemporal_load(vin + j * VREN_WAVE) : 0u;
@@ -288,6 +305,32 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     // this one never exceeds its count in the pass histogram). A full tile reads all its staged keys before the
     // first store, so the LDS reads overlap.
     const bool full = tile_count == (uint32_t)TILE;
+    if constexpr (OUT_PAIRS) {
+        // keys and values of staged position i leave together as one 8-byte pair (positions past the end of a
+        // partial tile hold padding and are not stored)
+        uint32_t k[KPT], o[KPT], v[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            k[j] = s_keys[tid + j * THREADS];
+            o[j] = s_goff[__builtin_amdgcn_ubfe(k[j], shift, width)] + tid + j * THREADS;
+        }
+        if (SEQ) {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < KPT; j++) s_keys[local[j]] = val[j];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < KPT; j++) v[j] = s_keys[tid + j * THREADS];
+        } else {
+#pragma unroll
+            for (int j = 0; j < KPT; j++) v[j] = s_vals[tid + j * THREADS];
+        }
+        vu2* pout = reinterpret_cast<vu2*>(keys_out);
+#pragma unroll
+        for (int j = 0; j < KPT; j++)
+            if (full || tid + j * THREADS < tile_count) pout[o[j]] = vu2{k[j], v[j]};
+        return;
+    }
     uint32_t o[KPT];
     if (full) {
         uint32_t k[KPT];
@@ -422,8 +465,8 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
 
     rs_init_kernel<<<1, 256, 0, s>>>(hist, passes * RADIX, counters, passes);
     VREN_LAUNCH_CHECK(ctx);
-    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
-    rs_hist_kernel<BITS><<<hb, 256, 0, s>>>(keys, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
+    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus, std::max<uint64_t>(1, ceil_div_u64(n, 16384)));
+    rs_hist_kernel<BITS><<<hb, kHistThreads, 0, s>>>(keys, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
     VREN_LAUNCH_CHECK(ctx);
     rs_hexscan_kernel<BITS><<<1, 256, 0, s>>>(hist, passes);
     VREN_LAUNCH_CHECK(ctx);
@@ -447,6 +490,48 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
         VREN_HIP(ctx, hipMemcpyAsync(keys, ksrc, n * 4, hipMemcpyDeviceToDevice, s));
         if (vals) VREN_HIP(ctx, hipMemcpyAsync(vals, vsrc, n * 4, hipMemcpyDeviceToDevice, s));
     }
+    return VREN_OK;
+}
+
+// stable sort of interleaved (key, value) pairs on bits [begin_bit, end_bit), at most two digit passes: the first
+// pass reads the pairs and the last one writes them (a single pass does both); between two passes the keys and
+// values sit split in the scratch. Per pair: 8 B histogram read + 16 B per pass.
+template <int BITS, int KPT, int WAVES>
+int run_sort_pairs(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t begin_bit,
+                   uint32_t end_bit, char* scratch, const Layout& L)
+{
+    constexpr int RADIX = 1 << BITS;
+    constexpr int TILE = WAVES * 64 * KPT;
+    const uint32_t passes = (end_bit - begin_bit + BITS - 1) / BITS;
+    if (passes > 2) return vren_fail(ctx, VREN_EINVAL, "radix_sort: interleaved pairs take at most %d bits", 2 * BITS);
+    const uint64_t tiles = ceil_div_u64(n, TILE);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + L.hist);
+    uint32_t* counters = reinterpret_cast<uint32_t*>(scratch + L.counters);
+    uint32_t* st[2] = {reinterpret_cast<uint32_t*>(scratch + L.status_a),
+                       reinterpret_cast<uint32_t*>(scratch + L.status_b)};
+    uint32_t* tk = reinterpret_cast<uint32_t*>(scratch + L.tmp_keys);
+    uint32_t* tv = reinterpret_cast<uint32_t*>(scratch + L.tmp_vals);
+
+    rs_init_kernel<<<1, 256, 0, s>>>(hist, passes * RADIX, counters, passes);
+    VREN_LAUNCH_CHECK(ctx);
+    uint32_t hb = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus, std::max<uint64_t>(1, ceil_div_u64(n, 16384)));
+    rs_hist_kernel<BITS, true><<<hb, kHistThreads, 0, s>>>(in, n, begin_bit, end_bit, passes, hist, st[0], tiles * RADIX);
+    VREN_LAUNCH_CHECK(ctx);
+    rs_hexscan_kernel<BITS><<<1, 256, 0, s>>>(hist, passes);
+    VREN_LAUNCH_CHECK(ctx);
+    if (passes == 1) {
+        rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+            in, nullptr, out, nullptr, n, begin_bit, end_bit - begin_bit, hist, st[0], counters, nullptr);
+        VREN_LAUNCH_CHECK(ctx);
+        return VREN_OK;
+    }
+    const uint32_t shift1 = begin_bit + BITS;
+    rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, false><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+        in, nullptr, tk, tv, n, begin_bit, BITS, hist, st[0], counters, st[1]);
+    VREN_LAUNCH_CHECK(ctx);
+    rs_onesweep_kernel<BITS, KPT, WAVES, true, false, false, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+        tk, tv, out, nullptr, n, shift1, end_bit - shift1, hist + RADIX, st[1], counters + 1, nullptr);
+    VREN_LAUNCH_CHECK(ctx);
     return VREN_OK;
 }
 
@@ -489,6 +574,22 @@ int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t*
         return vren_fail(ctx, VREN_EINVAL, "radix_sort: scratch of %zu bytes, %zu required", scratch_bytes, L.total);
     if (begin_bit >= end_bit) return VREN_OK;
     return kConfigs[cfg].fn(ctx, s, keys, vals, n, begin_bit, end_bit, static_cast<char*>(scratch), L);
+}
+
+int vren_radix_sort_pairs_interleaved(vren_ctx* ctx, hipStream_t s, const uint32_t* in_pairs, uint32_t* out_pairs,
+                                      uint64_t n, uint32_t begin_bit, uint32_t end_bit, void* scratch,
+                                      size_t scratch_bytes)
+{
+    Layout L = layout_for(n, true, kMinTile, 256);
+    if (scratch_bytes < L.total)
+        return vren_fail(ctx, VREN_EINVAL, "radix_sort: scratch of %zu bytes, %zu required", scratch_bytes, L.total);
+    if (n == 0) return VREN_OK;
+    if (begin_bit >= end_bit) {
+        VREN_HIP(ctx, hipMemcpyAsync(out_pairs, in_pairs, n * 8, hipMemcpyDeviceToDevice, s));
+        return VREN_OK;
+    }
+    // the default configuration's tile (8-bit digits, 23 keys per thread, 16 waves)
+    return run_sort_pairs<8, 23, 16>(ctx, s, in_pairs, out_pairs, n, begin_bit, end_bit, static_cast<char*>(scratch), L);
 }
 
 extern "C" {
