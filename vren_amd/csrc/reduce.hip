@@ -64,13 +64,17 @@ static_assert(kRows * kTreeWaves == VREN_WAVE, "one run per lane of the finishin
 template <typename T>
 constexpr uint64_t tree_tile() { return (uint64_t)kTreeThreads * (16 / sizeof(T)) * kRows; }
 
-// One tile of the view: elements e = tile * TILE + i, stored at base[offset + e * stride] (e < count), identity
-// beyond `valid`. Writes back every element with index < count.
+// One tile of the view: elements e = tile * TILE + i, read from in[in_offset + e * in_stride] (identity at and
+// beyond `valid`) and written to out[offset + e * stride] (e < count). With `totals`, the tile's total (the node
+// at its last element) is also written to totals[segment * gridDim.x + tile], a compact array that the next pass
+// reads instead of the totals' strided places in the tree.
 template <typename T, typename Op>
 __global__ __launch_bounds__(kTreeThreads) void upsweep_tile_kernel(const T* in, T* out,
                                                                     uint64_t in_seg_stride, uint64_t out_seg_stride,
-                                                                    uint64_t valid, uint64_t count, uint64_t offset,
-                                                                    uint64_t stride)
+                                                                    uint64_t valid, uint64_t count,
+                                                                    uint64_t in_offset, uint64_t in_stride,
+                                                                    uint64_t offset, uint64_t stride,
+                                                                    T* __restrict__ totals)
 {
     constexpr int E = 16 / sizeof(T);
     constexpr uint64_t TILE = tree_tile<T>();
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(kTreeThreads) void upsweep_tile_kernel(const T* in,
     const T* src = in + seg * in_seg_stride;
     T* dst = out + seg * out_seg_stride;
     const uint64_t tile_start = (uint64_t)blockIdx.x * TILE;
-    const bool vin = stride == 1 && offset == 0 && tile_start + TILE <= valid &&
+    const bool vin = in_stride == 1 && in_offset == 0 && tile_start + TILE <= valid &&
                      (reinterpret_cast<uintptr_t>(src + tile_start) & 15) == 0;
     const bool vout = stride == 1 && offset == 0 && tile_start + TILE <= count &&
                       (reinterpret_cast<uintptr_t>(dst + tile_start) & 15) == 0;
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(kTreeThreads) void upsweep_tile_kernel(const T* in,
 #pragma unroll
             for (int c = 0; c < E; c++) {
                 const uint64_t e = first + c;
-                v[j][c] = e < valid ? src[offset + e * stride] : Op::identity();
+                v[j][c] = e < valid ? src[in_offset + e * in_stride] : Op::identity();
             }
         }
     }
@@ -132,6 +136,7 @@ __global__ __launch_bounds__(kTreeThreads) void upsweep_tile_kernel(const T* in,
         s_run[lane] = x;
     }
     __syncthreads();
+    if (totals && tid == 0) totals[seg * gridDim.x + blockIdx.x] = s_run[kRows * kTreeWaves - 1];
     if (lane == VREN_WAVE - 1) {
 #pragma unroll
         for (int j = 0; j < kRows; j++) v[j][E - 1] = s_run[j * kTreeWaves + wave];
@@ -158,16 +163,26 @@ int reduce_tree(vren_ctx* ctx, hipStream_t s, const T* in, uint32_t n, T* out, u
 {
     constexpr uint64_t TILE = tree_tile<T>();
     const uint64_t p = next_pow2_u32(n);
-    // pass 0: in (n valid, segment stride n) -> out (p elements, segment stride p)
+    // compact tile totals of each pass but the last (p / TILE, then p / TILE^2, ... per segment), double-buffered
+    const uint64_t t1 = p > TILE ? p / TILE : 0, t2 = t1 > TILE ? t1 / TILE : 0;
+    const size_t a_bytes = (size_t)((t1 * blocks * sizeof(T) + 255) & ~(uint64_t)255);
+    if (t1) {
+        int st = vren_ensure_scratch(ctx, a_bytes + t2 * blocks * sizeof(T));
+        if (st) return st;
+    }
+    T* buf[2] = {static_cast<T*>(ctx->scratch), reinterpret_cast<T*>(static_cast<char*>(ctx->scratch) + a_bytes)};
+    // pass 0: in (n valid, segment stride n) -> out (p elements, segment stride p); pass k > 0: the compact
+    // totals of pass k - 1 -> their places in the tree (the last element of each tile of the previous view)
     uint64_t count = p, offset = 0, stride = 1;
-    bool first = true;
-    while (true) {
-        uint64_t tiles = ceil_div_u64(count, TILE);
-        dim3 grid((uint32_t)tiles, blocks);
-        if (first) {
-            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(in, out, n, p, n, count, 0, 1);
+    for (int pass = 0;; pass++) {
+        const uint64_t tiles = ceil_div_u64(count, TILE);
+        const dim3 grid((uint32_t)tiles, blocks);
+        T* totals = count > TILE ? buf[pass & 1] : nullptr;
+        if (pass == 0) {
+            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(in, out, n, p, n, count, 0, 1, 0, 1, totals);
         } else {
-            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(out, out, p, p, count, count, offset, stride);
+            upsweep_tile_kernel<T, Op><<<grid, kTreeThreads, 0, s>>>(buf[(pass - 1) & 1], out, count, p, count, count,
+                                                                     0, 1, offset, stride, totals);
         }
         VREN_LAUNCH_CHECK(ctx);
         if (count <= TILE) break;
@@ -175,7 +190,6 @@ int reduce_tree(vren_ctx* ctx, hipStream_t s, const T* in, uint32_t n, T* out, u
         offset = offset + (TILE - 1) * stride;
         stride = stride * TILE;
         count = count / TILE;
-        first = false;
     }
     return VREN_OK;
 }
