@@ -53,19 +53,15 @@ __global__ __launch_bounds__(kHistThreads) void rs_hist_kernel(const uint32_t* _
          i += (uint64_t)gridDim.x * kHistThreads)
         clear[i] = 0u;
     __syncthreads();
-    auto count = [&](uint32_t k) {
+    // the pass loop runs outside the keys held in registers, so its shift, width and bounds (wave-uniform) are
+    // computed once per batch of keys rather than once per key
+    auto count_keys = [&](const uint32_t* k, int m) {
         for (uint32_t p = 0; p < passes; p++) {
-            uint32_t sh = begin_bit + p * BITS;
-            uint32_t w = min((uint32_t)BITS, end_bit - sh);
-            uint32_t d = (k >> sh) & ((1u << w) - 1u);
-            atomicAdd(&s_h[p * RADIX + d], 1u);
+            const uint32_t sh = begin_bit + p * BITS;
+            const uint32_t w = min((uint32_t)BITS, end_bit - sh);
+            uint32_t* h = s_h + p * RADIX;
+            for (int u = 0; u < m; u++) atomicAdd(&h[__builtin_amdgcn_ubfe(k[u], sh, w)], 1u);
         }
-    };
-    auto count_vec = [&](uint4 q) {
-        count(q.x);
-        if (!PAIRS) count(q.y);
-        count(q.z);
-        if (!PAIRS) count(q.w);
     };
     const bool aligned = (reinterpret_cast<uintptr_t>(keys) & 15) == 0;
     const uint64_t n4 = aligned ? n / KPV : 0;
@@ -78,12 +74,36 @@ __global__ __launch_bounds__(kHistThreads) void rs_hist_kernel(const uint32_t* _
         uint4 q[U];
 #pragma unroll
         for (int u = 0; u < U; u++) q[u] = ntload4(k4 + i + u * stride);
+        uint32_t k[U * KPV];
 #pragma unroll
-        for (int u = 0; u < U; u++) count_vec(q[u]);
+        for (int u = 0; u < U; u++) {
+            if (PAIRS) {
+                k[2 * u] = q[u].x;
+                k[2 * u + 1] = q[u].z;
+            } else {
+                k[4 * u] = q[u].x;
+                k[4 * u + 1] = q[u].y;
+                k[4 * u + 2] = q[u].z;
+                k[4 * u + 3] = q[u].w;
+            }
+        }
+        for (uint32_t p = 0; p < passes; p++) {
+            const uint32_t sh = begin_bit + p * BITS;
+            const uint32_t w = min((uint32_t)BITS, end_bit - sh);
+            uint32_t* h = s_h + p * RADIX;
+#pragma unroll
+            for (int u = 0; u < U * (int)KPV; u++) atomicAdd(&h[__builtin_amdgcn_ubfe(k[u], sh, w)], 1u);
+        }
     }
-    for (; i < n4; i += stride) count_vec(k4[i]);
-    for (uint64_t j = n4 * KPV + (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; j < n; j += stride)
-        count(keys[PAIRS ? 2 * j : j]);
+    for (; i < n4; i += stride) {
+        const uint4 q = k4[i];
+        const uint32_t k[4] = {q.x, PAIRS ? q.z : q.y, q.z, q.w};
+        count_keys(k, (int)KPV);
+    }
+    for (uint64_t j = n4 * KPV + (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; j < n; j += stride) {
+        const uint32_t k = keys[PAIRS ? 2 * j : j];
+        count_keys(&k, 1);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < passes * RADIX; i += kHistThreads) {
         uint32_t c = s_h[i];
