@@ -134,6 +134,13 @@ def test_primitive_argument_validation(lib, hctx):
     for nb in (0, 3, 65537, 1 << 17):
         assert lib.vren_bucket_sort(hctx, None, buf, 10, P(8192), buf, nb, 0) == EINVAL
     assert lib.vren_bucket_sort(hctx, None, buf, 10, buf, buf, 64, 0) == EINVAL  # in place
+    # the radix passes load and store 8-byte pairs
+    assert lib.vren_bucket_sort(hctx, None, P(4100), 10, P(8192), buf, 64, 0) == EINVAL
+    assert b"8-byte aligned" in lib.vren_last_error(hctx)
+    assert lib.vren_bucket_sort(hctx, None, buf, 10, P(8196), buf, 64, 0) == EINVAL
+    assert b"8-byte aligned" in lib.vren_last_error(hctx)
+    assert lib.vren_bucket_sort(hctx, None, buf, 10, P(8192), buf, 64, 0) == EINVAL
+    assert b"no device" in lib.vren_last_error(hctx)
     # build_lbvh: at least one leaf, at most 2^20, non-null buffers
     assert lib.vren_build_lbvh(hctx, None, buf, 0, buf, buf, buf, buf) == EINVAL
     assert lib.vren_build_lbvh(hctx, None, buf, (1 << 20) + 1, buf, buf, buf, buf) == EINVAL
