@@ -232,8 +232,8 @@ def _hbm(ms, nbytes, **extra):
 
 def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     """The path's other rows at roofline sizes: uint32 add-reduce (the up-sweep tree and the scalar total) and
-    exclusive scan of 2^26 keys, bucket sort of 2^24 pairs into 65536 buckets, and the shade forward and adjoint on
-    one 1920x1080 frame with 4096 lights (bench_shade_frame)."""
+    exclusive scan of 2^26 keys, bucket sort of 2^24 pairs into 65536 buckets, the point-light animation of 2^24
+    lights, and the shade forward and adjoint on one 1920x1080 frame with 4096 lights (bench_shade_frame)."""
     import numpy as np
     import torch
     n = 1 << log2n
@@ -257,6 +257,17 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     # counting-sort unit: count read 8 B + scatter read 8 B + write 8 B per pair (pairs in and out alone: 16 B)
     res["bucket_sort_65536"] = _hbm(ms, 24 * nb, n=nb, algorithmic_bytes_per_pair=24)
     del keys, pairs_in, pairs_out, offs
+    # point-light animation of 2^24 lights in place, one 60 Hz frame at speed 10 per call (position and direction
+    # read and written: 64 B/light)
+    nl = 1 << 24
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    lpos = torch.rand(nl, 4, device="cuda", generator=gen) * 120.0 - 60.0
+    ldir = torch.randn(nl, 4, device="cuda", generator=gen)
+    ldir[:, 3] = 0.0
+    ldir = (ldir / ldir.norm(dim=1, keepdim=True)).contiguous()
+    ms = _median_ms(lambda: ctx.bounce_point_lights(lpos, ldir, nl, (-50.0,) * 3, (50.0,) * 3, 10.0, 1.0 / 60.0), reps)
+    res["bounce_point_lights_2p24"] = _hbm(ms, 64 * nl, n=nl, algorithmic_bytes_per_light=64)
+    del lpos, ldir
     res.update(bench_shade_frame(ctx, vren_amd, reps))
     return res
 

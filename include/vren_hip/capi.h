@@ -157,6 +157,14 @@ VREN_EXPORT int vren_bucket_sort(vren_ctx* ctx, vren_stream_t stream, const uint
                                  uint32_t* out_pairs, uint32_t* bucket_offsets, uint32_t n_buckets,
                                  uint32_t key_shift);
 
+/* vren_demo::point_light_bouncer::bounce (point_light_bouncer.cpp:17-57) and bounce_point_lights.comp:33-73: moves
+ * n point lights in place by speed * dt inside [aabb_min, aabb_max] (each position first clamped into it),
+ * reflecting the direction components of the walls reached, at most 32 segments. positions and directions are
+ * vec4[n] (16-byte aligned); w is written as 1 and 0. The per-frame light update ahead of the light BVH build. */
+VREN_EXPORT int vren_bounce_point_lights(vren_ctx* ctx, vren_stream_t stream, float* positions, float* directions,
+                                         uint32_t n, const float* aabb_min, const float* aabb_max, float speed,
+                                         float dt);
+
 /* calc_bvh_* (build_bvh.hpp:52-56, build_bvh.cpp:101-136), host only */
 VREN_EXPORT uint32_t vren_calc_bvh_padded_leaf_count(uint32_t leaf_count);
 VREN_EXPORT uint32_t vren_calc_bvh_buffer_length(uint32_t leaf_count);

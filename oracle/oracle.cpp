@@ -30,6 +30,9 @@
  *   o_shade_f64          closed forms on hand-built single-light scenes; agreement with the float
  *                        restatement; finite differences of the smooth-attenuation forward
  *   o_shade_backward_f64 FP64 central differences of o_shade_loss_f64
+ *   o_bounce_point_lights  closed forms: unobstructed motion p + speed dt d, clamping of outside positions, and
+ *                        the unfolded billiard path (1-D and 2-D, several reflections) to within the 1e-5 wall
+ *                        offsets; |d| per component preserved, final positions inside the AABB
  */
 #include <algorithm>
 #include <cmath>
@@ -1491,4 +1494,48 @@ OEXPORT void o_shade_backward_f64(uint32_t W, uint32_t H, const vren_camera* cam
                 g_light[4 * li + 3] += g_att * datt_dx * (-pr.x / pr.r);
             }
         }
+}
+
+/* bounce_point_lights.comp:33-73 (dispatched by point_light_bouncer.cpp:17-57), one light at a time: clamp the
+ * position into [lo, hi] with GLSL min(max(p, lo), hi), then up to 32 segments: the six wall times
+ * (wall - p) / d with non-positive ones replaced by 1e35, min_t = their GLSL min in the shader's nesting,
+ * step = min(min_t - 1e-5, remaining), p += step * d, and when min_t < remaining the components whose wall time
+ * equals min_t flip sign. Writes w = 1 (positions) and w = 0 (directions). */
+static inline float o_glsl_min(float x, float y) { return y < x ? y : x; }
+static inline float o_glsl_max(float x, float y) { return x < y ? y : x; }
+
+OEXPORT void o_bounce_point_lights(float* pos, float* dir, uint32_t n, const float* lo, const float* hi, float speed,
+                                   float dt)
+{
+    const float kInf = 1e35f, kEps = 1e-5f;
+    for (uint32_t i = 0; i < n; i++) {
+        float p[3], d[3];
+        for (int c = 0; c < 3; c++) {
+            p[c] = o_glsl_min(o_glsl_max(pos[4 * i + c], lo[c]), hi[c]);
+            d[c] = dir[4 * i + c];
+        }
+        float rem = speed * dt;
+        for (int j = 0; j < 32 && rem > 0.0f; j++) {
+            float t1[3], t2[3];
+            for (int c = 0; c < 3; c++) {
+                t1[c] = (lo[c] - p[c]) / d[c];
+                t1[c] = t1[c] <= 0.0f ? kInf : t1[c];
+                t2[c] = (hi[c] - p[c]) / d[c];
+                t2[c] = t2[c] <= 0.0f ? kInf : t2[c];
+            }
+            const float min_t =
+                o_glsl_min(t1[0], o_glsl_min(t2[0], o_glsl_min(t1[1], o_glsl_min(t2[1], o_glsl_min(t1[2], t2[2])))));
+            const float step = o_glsl_min(min_t - kEps, rem);
+            for (int c = 0; c < 3; c++) p[c] = p[c] + step * d[c];
+            if (min_t < rem)
+                for (int c = 0; c < 3; c++) d[c] = d[c] * ((t1[c] == min_t || t2[c] == min_t) ? -1.0f : 1.0f);
+            rem = rem - step;
+        }
+        for (int c = 0; c < 3; c++) {
+            pos[4 * i + c] = p[c];
+            dir[4 * i + c] = d[c];
+        }
+        pos[4 * i + 3] = 1.0f;
+        dir[4 * i + 3] = 0.0f;
+    }
 }

@@ -111,6 +111,13 @@ class Oracle:
         self.lib.o_bucket_sort(_p(pairs), c_u32(pairs.shape[0]), _p(out), _p(offs), c_u32(n_buckets), c_u32(shift))
         return out, offs
 
+    def bounce_point_lights(self, pos, dirs, lo, hi, speed, dt):
+        pos = np.array(pos, np.float32, copy=True).reshape(-1, 4)
+        dirs = np.array(dirs, np.float32, copy=True).reshape(-1, 4)
+        lo3, hi3 = (ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi)
+        self.lib.o_bounce_point_lights(_p(pos), _p(dirs), c_u32(pos.shape[0]), lo3, hi3, c_f(speed), c_f(dt))
+        return pos, dirs
+
     def calc(self, name, n):
         return getattr(self.lib, "o_calc_bvh_" + name)(c_u32(n))
 

@@ -141,6 +141,14 @@ def test_primitive_argument_validation(lib, hctx):
     assert b"8-byte aligned" in lib.vren_last_error(hctx)
     assert lib.vren_bucket_sort(hctx, None, buf, 10, P(8192), buf, 64, 0) == EINVAL
     assert b"no device" in lib.vren_last_error(hctx)
+    # bounce_point_lights: non-null AABB and (for n > 0) buffers, 16-byte aligned vec4 buffers
+    box = (ctypes.c_float * 3)(0.0, 0.0, 0.0)
+    assert lib.vren_bounce_point_lights(hctx, None, buf, buf, 10, None, box, 1.0, 1.0) == EINVAL
+    assert lib.vren_bounce_point_lights(hctx, None, None, buf, 10, box, box, 1.0, 1.0) == EINVAL
+    assert lib.vren_bounce_point_lights(hctx, None, P(4104), buf, 10, box, box, 1.0, 1.0) == EINVAL
+    assert b"16-byte aligned" in lib.vren_last_error(hctx)
+    assert lib.vren_bounce_point_lights(hctx, None, buf, buf, 10, box, box, 1.0, 1.0) == EINVAL
+    assert b"no device" in lib.vren_last_error(hctx)
     # build_lbvh: at least one leaf, at most 2^20, non-null buffers
     assert lib.vren_build_lbvh(hctx, None, buf, 0, buf, buf, buf, buf) == EINVAL
     assert lib.vren_build_lbvh(hctx, None, buf, (1 << 20) + 1, buf, buf, buf, buf) == EINVAL

@@ -271,3 +271,46 @@ def test_build_bvh_levels_and_point_queries(oracle, n):
             if nd["next"] not in (0xFFFFFFFF, 0xFFFFFFFE):
                 stack.extend(range(int(nd["next"]), int(nd["next"]) + 32))
         assert sorted(got) == want
+
+
+def test_bounce_point_lights_closed_forms(oracle):
+    # bounce_point_lights.comp:33-73 pinned by the motion it describes, not by its own formula
+    lo, hi = (-50.0, -50.0, -50.0), (50.0, 50.0, 50.0)
+    # no wall within reach: one segment of length speed * dt = 2 along d, direction kept, w = 1 and 0
+    p0 = np.array([[1, 2, 3, 7]], np.float32)
+    d0 = np.array([[0.5, -0.25, 0.125, 9]], np.float32)
+    p, d = oracle.bounce_point_lights(p0, d0, lo, hi, 4.0, 0.5)
+    assert np.array_equal(p[0], [2.0, 1.5, 3.25, 1.0]) and np.array_equal(d[0], [0.5, -0.25, 0.125, 0.0])
+    # a position outside the box is clamped onto it before moving (x to -50, y to 50)
+    p, d = oracle.bounce_point_lights([[-100, 60, 5, 1]], [[0.5, -0.5, 0, 0]], lo, hi, 1.0, 1.0)
+    assert np.array_equal(p[0, :3], [-49.5, 49.5, 5.0])
+    # billiard in [0, 10]^3: the unfolded path folds back with period 20 per axis; each reflection stops 1e-5
+    # short of its wall, hence the tolerance
+    box = ((0.0, 0.0, 0.0), (10.0, 10.0, 10.0))
+    p, d = oracle.bounce_point_lights([[5, 5, 5, 1]], [[1, 0, 0, 0]], *box, 4.0, 5.0)  # x: 5 + 20 -> 5, moving +
+    assert abs(p[0, 0] - 5.0) < 1e-3 and p[0, 1] == 5.0 and p[0, 2] == 5.0 and d[0, 0] == 1.0
+
+    def fold(u):
+        m = u % 20.0
+        return (m, 1.0) if m <= 10.0 else (20.0 - m, -1.0)
+
+    for dx, dy, length in ((0.6, 0.8, 37.0), (-0.8, 0.6, 51.0), (0.28, -0.96, 23.5)):
+        p, d = oracle.bounce_point_lights([[5, 5, 5, 1]], [[dx, dy, 0, 0]], *box, length, 1.0)
+        (wx, sx), (wy, sy) = fold(5.0 + dx * length), fold(5.0 + dy * length)
+        assert abs(p[0, 0] - wx) < 1e-3 and abs(p[0, 1] - wy) < 1e-3, (dx, dy, p[0])
+        assert np.sign(d[0, 0]) == sx * np.sign(dx) and np.sign(d[0, 1]) == sy * np.sign(dy)
+    # the shader's quirk: a light clamped onto a wall and moving outward has both wall times of that axis <= 0
+    # (replaced by 1e35), so it is never reflected and leaves the box
+    p, d = oracle.bounce_point_lights([[0, -60, 0, 1]], [[0, -1, 0, 0]], lo, hi, 2.0, 1.0)
+    assert np.array_equal(p[0, :3], [0.0, -52.0, 0.0]) and d[0, 1] == -1.0
+    # invariants on random lights that start inside: direction components keep their magnitudes, and over a
+    # frame-sized travel (30 units) positions stay in the box. (Over hundreds of units a few lights do leave it in
+    # the shader's arithmetic too: a segment stops 1e-5 short in time, which can round onto the wall at |p| ~ 50.)
+    rng = np.random.default_rng(5)
+    pos = np.concatenate([rng.uniform(-49, 49, (2000, 3)), np.ones((2000, 1))], 1).astype(np.float32)
+    dirs = rng.normal(size=(2000, 3))
+    dirs = np.concatenate([dirs / np.linalg.norm(dirs, axis=1, keepdims=True), np.zeros((2000, 1))], 1).astype(np.float32)
+    p, d = oracle.bounce_point_lights(pos, dirs, lo, hi, 300.0, 1.0)
+    assert np.array_equal(np.abs(d[:, :3]), np.abs(dirs[:, :3]))
+    p, d = oracle.bounce_point_lights(pos, dirs, lo, hi, 30.0, 1.0)
+    assert (p[:, :3] >= -50.001).all() and (p[:, :3] <= 50.001).all()

@@ -119,6 +119,12 @@ class Context:
         return self.check(self.lib.vren_bucket_sort(self.h, self.stream(), _ptr(in_pairs), n, _ptr(out_pairs),
                                                     _ptr(offsets), n_buckets, key_shift))
 
+    def bounce_point_lights(self, positions, directions, n, aabb_min, aabb_max, speed, dt):
+        """Moves n lights (float32 [n, 4] position and direction tensors, in place) by speed * dt inside the AABB."""
+        lo, hi = (ctypes.c_float * 3)(*aabb_min), (ctypes.c_float * 3)(*aabb_max)
+        return self.check(self.lib.vren_bounce_point_lights(self.h, self.stream(), _ptr(positions), _ptr(directions),
+                                                            n, lo, hi, speed, dt))
+
     def build_bvh(self, nodes, padded_leaf_count):
         return self.check(self.lib.vren_build_bvh(self.h, self.stream(), _ptr(nodes), padded_leaf_count))
 

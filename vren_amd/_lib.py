@@ -71,6 +71,8 @@ PROTOTYPES = {
     "vren_autotune_save": (c_int, [c_void_p, c_char_p]),
     "vren_autotune_load": (c_int, [c_void_p, c_char_p]),
     "vren_bucket_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_uint32, c_uint32]),
+    "vren_bounce_point_lights": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, P(c_float), P(c_float),
+                                         c_float, c_float]),
     "vren_calc_bvh_padded_leaf_count": (c_uint32, [c_uint32]),
     "vren_calc_bvh_buffer_length": (c_uint32, [c_uint32]),
     "vren_calc_bvh_buffer_size": (c_size_t, [c_uint32]),
