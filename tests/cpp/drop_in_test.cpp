@@ -16,6 +16,7 @@
 #include "primitives/build_bvh.hpp"
 #include "primitives/radix_sort.hpp"
 #include "primitives/reduce.hpp"
+#include "point_light_bouncer.hpp"
 #include "pipeline/clustered_shading.hpp"
 #include "../../include/vren_hip/synth.h"
 
@@ -227,6 +228,21 @@ static void primitives(vren::context& ctx)
         EXPECT(ok, "bucket_sort order");
         uint32_t key0 = r[idx[0]] & 0xFFFF, key_last = r[idx[n - 1]] & 0xFFFF;
         EXPECT(off[key0] == 0 && off[key_last] <= n - 1 && off[65535] <= n, "bucket offsets");
+    }
+    // point_light_bouncer: a light with no wall within reach moves by exactly speed * dt * d (w = 1 and 0)
+    {
+        std::vector<float> pos = {1.0f, 2.0f, 3.0f, 7.0f}, dir = {0.5f, -0.25f, 0.125f, 9.0f};
+        auto dp = upload(pos);
+        auto dd = upload(dir);
+        vren_demo::point_light_bouncer bouncer(ctx);
+        bouncer.bounce(0, s, rc, dp, dd, 1, vren::vec3{-50.0f, -50.0f, -50.0f}, vren::vec3{50.0f, 50.0f, 50.0f}, 4.0f,
+                       0.5f);
+        ctx.sync(s);
+        auto gp = download<float>(dp, 4);
+        auto gd = download<float>(dd, 4);
+        EXPECT(gp == std::vector<float>({2.0f, 1.5f, 3.25f, 1.0f}) &&
+                   gd == std::vector<float>({0.5f, -0.25f, 0.125f, 0.0f}),
+               "point_light_bouncer");
     }
     // build_bvh over 1024 leaves: every parent = min/max over its valid children (build_bvh.cpp:175-210)
     {
