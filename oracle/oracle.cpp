@@ -1309,6 +1309,27 @@ OEXPORT void o_shade_f64(uint32_t W, uint32_t H, const vren_camera* cam, const v
                         indices, index_capacity, pos4, light4, dir_lights, n_dir, out, shaded_mask);
 }
 
+/* FP64 shade of selected pixels only (a sample of a large frame): out3[3 i ..] for pixels[i], shaded[i] = 0 for
+ * background pixels (left unwritten) */
+OEXPORT void o_shade_f64_pixels(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
+                                const float* depth, const uint16_t* normals, const float* albedo_rough,
+                                const float* metallic, const uint32_t* refs, uint32_t key_capacity,
+                                const uint32_t* counts, const uint32_t* offsets, const uint32_t* indices,
+                                uint64_t index_capacity, const float* pos4, const float* light4, uint32_t L,
+                                const uint64_t* pixels, uint64_t n_pixels, double* out3, uint8_t* shaded)
+{
+    frame_consts c = make_consts(W, H, cam);
+    std::vector<double> p4(pos4, pos4 + 4 * (size_t)L), l4(light4, light4 + 4 * (size_t)L);
+    for (uint64_t i = 0; i < n_pixels; i++) {
+        uint64_t p = pixels[i];
+        shaded[i] = shade_pixel_f64(c, cam, cfg, (uint32_t)(p % W), (uint32_t)(p / W), depth, normals, albedo_rough,
+                                    metallic, refs, key_capacity, counts, offsets, indices, index_capacity,
+                                    p4.data(), l4.data(), out3 + 3 * i, nullptr, nullptr)
+                        ? 1
+                        : 0;
+    }
+}
+
 OEXPORT void o_shade_f32(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
                          const float* depth, const uint16_t* normals, const float* albedo_rough, const float* metallic,
                          const uint32_t* refs, uint32_t key_capacity, const uint32_t* counts, const uint32_t* offsets,

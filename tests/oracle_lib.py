@@ -271,6 +271,20 @@ class ClusteredOracle:
            _p(dirl) if dirl is not None else None, c_u32(nd), _p(out), _p(mask))
         return out, mask.astype(bool)
 
+    def shade_pixels(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos, lig,
+                     pixels):
+        """FP64 radiance of the listed pixels only: (rgb[n, 3], shaded mask[n])."""
+        pixels = np.ascontiguousarray(pixels, np.uint64)
+        out = np.full((pixels.size, 3), np.nan, np.float64)
+        sh = np.zeros(pixels.size, np.uint8)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        lig = np.ascontiguousarray(lig, np.float32).reshape(-1, 4)
+        self.lib.o_shade_f64_pixels(
+            c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(ar), _p(metal), _p(refs),
+            c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(pos), _p(lig), c_u32(pos.shape[0]),
+            _p(pixels), c_u64(pixels.size), _p(out), _p(sh))
+        return out, sh.astype(bool)
+
     def shade_loss(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos_d, lig_d,
                    target, pixels, pixel_ar=None):
         self.lib.o_shade_loss_f64.restype = ctypes.c_double

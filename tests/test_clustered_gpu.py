@@ -230,18 +230,26 @@ def run_assign(vctx, co, W, H, cam, cf, depth, normals, pos, lig, key_cap=1 << 2
                 bvh=from_dev(bvh, ol.BVH_DTYPE, (nb,)))
 
 
+C3_ENCODERS = {"reference": dict(), "sliced32": dict(key_mode=1, depth_slices=32)}
+
+
+@pytest.mark.parametrize("encoder", ["reference", "sliced32"])
 @pytest.mark.parametrize("aabb_mode", [0, 1])
-def test_assign_lights_c3(vctx, co, c3, aabb_mode):
+def test_assign_lights_c3(vctx, co, c3, aabb_mode, encoder):
+    """1920x1080 with 4096 lights, with the reference key encoder and with the configuration's own (32x32 tiles x 32
+    log-depth slices over [near, far] x 6 cube faces): counts, offsets and lists bit for bit, both AABB modes."""
     cam = c3["cam"]
-    cf = ol.cfg(cluster_aabb_mode=aabb_mode)
+    cf = ol.cfg(cluster_aabb_mode=aabb_mode, **C3_ENCODERS[encoder])
     g = run_assign(vctx, co, W3, H3, cam, cf, c3["depth"], c3["normals"], c3["pos"], c3["lig"])
     ovp, ocodes, obvh = co.light_bvh(c3["pos"], c3["lig"], cam, cf)
     okeys, oU, orefs = co.unique_clusters(W3, H3, cam, cf, c3["depth"], c3["normals"])
     oc, oo, oi, ot = co.assign(W3, H3, cam, cf, okeys, oU, obvh, ocodes, ovp)
+    assert g["U"] == oU and np.array_equal(g["keys"], okeys) and np.array_equal(g["refs"], orefs)
     assert g["total"] == ot
     assert np.array_equal(g["counts"], oc)
     assert np.array_equal(g["offsets"], oo)
     assert np.array_equal(g["idx"], oi)
+    print("\n[c3 %s aabb=%d] %d clusters, %d assignments" % (encoder, aabb_mode, oU, ot))
 
 
 def test_assign_lights_many_lights(vctx, co):
@@ -265,11 +273,12 @@ def shade_error_report(got, ref, mask):
     return err
 
 
-@pytest.mark.parametrize("atten", [0, 1])
-def test_shade_c3_vs_fp64(vctx, co, c3, atten):
+@pytest.mark.parametrize("atten,encoder", [(0, "reference"), (1, "reference"), (0, "sliced32"), (1, "sliced32")])
+def test_shade_c3_vs_fp64(vctx, co, c3, atten, encoder):
     cam = c3["cam"]
-    cf = ol.cfg(attenuation=atten)
-    g = run_assign(vctx, co, W3, H3, cam, ol.cfg(), c3["depth"], c3["normals"], c3["pos"], c3["lig"])
+    cf = ol.cfg(attenuation=atten, **C3_ENCODERS[encoder])
+    g = run_assign(vctx, co, W3, H3, cam, ol.cfg(**C3_ENCODERS[encoder]), c3["depth"], c3["normals"], c3["pos"],
+                   c3["lig"])
     out = empty_dev(16 * W3 * H3, fill=0xFF)  # NaN sentinel: background pixels must stay untouched
     key_cap = 1 << 20
     vctx.shade(W3, H3, cam, cf, to_dev(c3["depth"]), to_dev(c3["normals"]), to_dev(c3["ar"]), to_dev(c3["me"]),
@@ -285,8 +294,8 @@ def test_shade_c3_vs_fp64(vctx, co, c3, atten):
     assert np.all(np.isnan(got[~mask]))               # background untouched (shade.comp:86)
     assert np.all(got[mask, 3] == 1.0)
     err = shade_error_report(got, ref, mask)
-    print("\nshade atten=%d: %d shaded outputs; rel err max %.3e p99.9 %.3e p50 %.3e per channel max %s"
-          % (atten, err.size, err.max(), np.quantile(err, 0.999), np.median(err),
+    print("\nshade %s atten=%d: %d shaded outputs; rel err max %.3e p99.9 %.3e p50 %.3e per channel max %s"
+          % (encoder, atten, err.size, err.max(), np.quantile(err, 0.999), np.median(err),
              ["%.2e" % x for x in err.max(axis=0)]))
     assert err.max() <= SHADE_RTOL, err.max()
 
@@ -391,12 +400,82 @@ def test_batch_modes_identical_and_match_oracle(vctx, co):
     b.close()
 
 
+def _batch_host_frame(b, keep, i):
+    k = keep[i]
+    W, H, L = k["W"], k["H"], k["L"]
+    o = b.outputs(i)
+    U = int(_read_ptr(o.cluster_count, 4, np.uint32)[0])
+    tot = int(_read_ptr(o.assigned_total, 4, np.uint32)[0])
+    return dict(W=W, H=H, L=L, cam=k["cam"], depth=from_dev(k["d"], np.float32, (W * H,)),
+                normals=from_dev(k["n"], np.uint16, (W * H, 4)), pos=from_dev(k["p"], np.float32, (L, 4)),
+                lig=from_dev(k["l"], np.float32, (L, 4)), ar=from_dev(k["a"], np.float32, (W * H, 4)),
+                me=from_dev(k["m"], np.float32, (W * H,)), out=from_dev(k["o"], np.float32, (W * H, 4)),
+                U=U, keys=_read_ptr(o.cluster_keys, 4 * U, np.uint32).copy(),
+                refs=_read_ptr(o.cluster_refs, 4 * W * H, np.uint32).copy(),
+                counts=_read_ptr(o.light_counts, 4 * U, np.uint32).copy(),
+                offsets=_read_ptr(o.light_offsets, 4 * U, np.uint32).copy(), total=tot,
+                idx=_read_ptr(o.light_indices, 4 * tot, np.uint32).copy(), key_capacity=o.key_capacity)
+
+
+def _batch_check_frame(co, cf, i, f, n_shade=2048):
+    """One frame of a batch against the oracle: keys, references, counts, offsets and lists bit for bit; the
+    radiance of n_shade sampled pixels within 1e-5 of FP64 (background pixels untouched)."""
+    W, H, L, cam = f["W"], f["H"], f["L"], f["cam"]
+    okeys, oU, orefs = co.unique_clusters(W, H, cam, cf, f["depth"], f["normals"])
+    assert f["U"] == oU and np.array_equal(f["keys"], okeys) and np.array_equal(f["refs"], orefs), i
+    ovp, ocodes, obvh = co.light_bvh(f["pos"], f["lig"], cam, cf)
+    oc, oo, oi, ot = co.assign(W, H, cam, cf, okeys, oU, obvh, ocodes, ovp)
+    assert f["total"] == ot, i
+    assert np.array_equal(f["counts"], oc) and np.array_equal(f["offsets"], oo), i
+    assert np.array_equal(f["idx"], oi), i
+    cc = np.zeros(f["key_capacity"], np.uint32)
+    cco = np.zeros(f["key_capacity"], np.uint32)
+    cc[:oU], cco[:oU] = oc, oo
+    pix = np.random.default_rng(i).choice(W * H, min(n_shade, W * H), replace=False).astype(np.uint64)
+    ref, sh = co.shade_pixels(W, H, cam, cf, f["depth"], f["normals"], f["ar"], f["me"], orefs, f["key_capacity"],
+                              cc, cco, oi, f["pos"], f["lig"], pix)
+    got = f["out"][pix.astype(np.int64)]
+    assert np.all(np.isnan(got[~sh])), i
+    err = shade_error_report(got[sh], ref[sh], np.ones(int(sh.sum()), bool))
+    return oU, int(ot), int(sh.sum()), float(err.max()) if err.size else 0.0
+
+
+def _batch_check_all(b, keep, co, cf, tag):
+    import concurrent.futures
+    results = []
+    with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
+        for c0 in range(0, len(keep), 8):   # 8 frames on the host at a time
+            frames = [(i, _batch_host_frame(b, keep, i)) for i in range(c0, min(c0 + 8, len(keep)))]
+            results += list(ex.map(lambda a: _batch_check_frame(co, cf, *a), frames))
+            del frames
+    worst = max(r[3] for r in results)
+    print("[%s] all %d frames bit-exact: %d clusters, %d assignments; shade sample %d px, rel err max %.2e"
+          % (tag, len(keep), sum(r[0] for r in results), sum(r[1] for r in results), sum(r[2] for r in results),
+             worst), flush=True)
+    assert worst <= SHADE_RTOL, worst
+
+
+def _batch_all_modes_identical(b, keep):
+    import torch
+    first = None
+    for mode in (b.PER_FRAME, b.BATCHED, b.GRAPH, b.GRAPH):
+        for k in keep:
+            k["o"].fill_(0xFF)
+        b.run(mode)
+        torch.cuda.synchronize()
+        if first is None:
+            first = [k["o"].clone() for k in keep]
+        else:
+            for i, (a, k) in enumerate(zip(first, keep)):
+                assert torch.equal(a, k["o"]), (mode, i)
+
+
 def test_batch_config4_64_heterogeneous_frames(vctx, co):
     """The batched configuration at full size: 64 frames, resolutions drawn from {640x360, 1280x720, 1920x1080,
     2560x1440, 3840x2160}, light counts log-uniform in [256, 65536]. Per-frame, batched and graph launches give the
-    same bytes for every frame (compared on the device); the frame with the most lights is checked against the
-    oracle (keys, references, light lists)."""
-    import torch
+    same bytes for every frame (compared on the device). Every frame is checked against the oracle: keys,
+    references, light counts, offsets and lists bit for bit, and the radiance of 2048 sampled pixels per frame
+    within 1e-5 of FP64."""
     import vren_amd
     rng = np.random.default_rng(4444)
     res = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
@@ -410,37 +489,25 @@ def test_batch_config4_64_heterogeneous_frames(vctx, co):
           flush=True)
     descs, keep = make_frames_batch(vctx, co, specs)
     b = vren_amd.Batch(vctx, descs, ol.cfg())
-    first = None
-    for mode in (b.PER_FRAME, b.BATCHED, b.GRAPH, b.GRAPH):
-        for k in keep:
-            k["o"].fill_(0xFF)
-        b.run(mode)
-        torch.cuda.synchronize()
-        if first is None:
-            first = [k["o"].clone() for k in keep]
-        else:
-            for i, (a, k) in enumerate(zip(first, keep)):
-                assert torch.equal(a, k["o"]), (mode, i)
+    _batch_all_modes_identical(b, keep)
     vctx.sync_status()
-    del first
-    i = max(range(len(specs)), key=lambda j: (specs[j][2], -specs[j][0]))
-    k = keep[i]
-    W, H, L = k["W"], k["H"], k["L"]
-    depth = from_dev(k["d"], np.float32, (W * H,))
-    normals = from_dev(k["n"], np.uint16, (W * H, 4))
-    pos = from_dev(k["p"], np.float32, (L, 4))
-    lig = from_dev(k["l"], np.float32, (L, 4))
-    okeys, oU, orefs = co.unique_clusters(W, H, k["cam"], ol.cfg(), depth, normals)
-    ovp, ocodes, obvh = co.light_bvh(pos, lig, k["cam"], ol.cfg())
-    oc, oo, oi, ot = co.assign(W, H, k["cam"], ol.cfg(), okeys, oU, obvh, ocodes, ovp)
-    o = b.outputs(i)
-    assert int(_read_ptr(o.cluster_count, 4, np.uint32)[0]) == oU
-    assert np.array_equal(_read_ptr(o.cluster_keys, 4 * oU, np.uint32), okeys)
-    assert np.array_equal(_read_ptr(o.cluster_refs, 4 * W * H, np.uint32), orefs)
-    assert np.array_equal(_read_ptr(o.light_counts, 4 * oU, np.uint32), oc)
-    assert np.array_equal(_read_ptr(o.light_offsets, 4 * oU, np.uint32), oo)
-    assert int(_read_ptr(o.assigned_total, 4, np.uint32)[0]) == int(ot)
-    assert np.array_equal(_read_ptr(o.light_indices, 4 * int(ot), np.uint32), oi)
+    _batch_check_all(b, keep, co, ol.cfg(), "batch64")
+    b.close()
+
+
+@pytest.mark.parametrize("aabb_mode", [0, 1])
+def test_batch_sliced_encoder(vctx, co, aabb_mode):
+    """The configuration's own key encoder (32 log-depth slices x 6 cube faces) through the batched chain: 1080p
+    with 4096 lights and three other frames; every launch mode gives the same bytes and every frame matches the
+    oracle (keys, references, lists; shade samples within 1e-5 of FP64)."""
+    import vren_amd
+    specs = [(1920, 1080, 4096, 8100), (640, 360, 700, 8101), (1280, 720, 20000, 8102), (333, 200, 3, 8103)]
+    cf = ol.cfg(key_mode=1, depth_slices=32, cluster_aabb_mode=aabb_mode)
+    descs, keep = make_frames_batch(vctx, co, specs)
+    b = vren_amd.Batch(vctx, descs, cf)
+    _batch_all_modes_identical(b, keep)
+    vctx.sync_status()
+    _batch_check_all(b, keep, co, cf, "batch sliced aabb=%d" % aabb_mode)
     b.close()
 
 

@@ -415,3 +415,21 @@ def test_lbvh_sphere_queries_match_brute_force(co):
         d = np.maximum(leaf_lo - c, 0.0) + np.maximum(c - leaf_hi, 0.0)
         brute = sorted(int(x) for x in nodes["right"][n - 1:][np.einsum("ij,ij->i", d, d) <= rad * rad])
         assert sorted(found) == brute
+
+
+def test_shade_pixels_equals_whole_frame_shade(co):
+    """The per-pixel FP64 shade (used to sample large frames) gives the whole-frame FP64 shade's values."""
+    W, H, L = 96, 64, 300
+    cam = ol.camera(W, H)
+    cf = ol.cfg()
+    pos, lig = co.synth_lights(61, L)
+    depth, normals = co.synth_frame(62, W, H, cam)
+    ar, me = co.synth_materials(63, W * H)
+    vp, codes, bvh = co.light_bvh(pos, lig, cam, cf)
+    keys, U, refs = co.unique_clusters(W, H, cam, cf, depth, normals)
+    counts, offsets, idx, tot = co.assign(W, H, cam, cf, keys, U, bvh, codes, vp)
+    full, mask = co.shade(W, H, cam, cf, depth, normals, ar, me, refs, U, counts, offsets, idx, pos, lig)
+    pix = np.arange(0, W * H, 7, dtype=np.uint64)
+    got, sh = co.shade_pixels(W, H, cam, cf, depth, normals, ar, me, refs, U, counts, offsets, idx, pos, lig, pix)
+    assert np.array_equal(sh, mask[pix.astype(np.int64)]) and sh.sum() > 100
+    assert np.array_equal(got[sh], full[pix.astype(np.int64)][sh, :3])
