@@ -126,6 +126,14 @@ VREN_EXPORT int vren_reduce_total_u32(vren_ctx* ctx, vren_stream_t stream, const
  * Any n >= 1 (the reference requires a power of 2). */
 VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in, uint32_t* out,
                                         uint64_t n, uint32_t blocks);
+/* No counterpart in the reference: vren_exclusive_scan_u32 for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid hipMalloc). The same kernels and results, with
+ * the look-back workspace in a fixed device array (one copy per GPU, 512 KiB + 256 B, resolved by vren_create), so
+ * the call makes no hipMalloc, hipFree or other allocation. At most 2^16 tiles of 16384 keys over all segments:
+ * ceil(n / 16384) * blocks <= 65536 (n * blocks <= 2^30), else VREN_EINVAL. Calls that overlap on one GPU (other
+ * streams or contexts) share the array and race; calls in sequence on one stream do not. */
+VREN_EXPORT int vren_exclusive_scan_u32_noalloc(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in,
+                                                uint32_t* out, uint64_t n, uint32_t blocks);
 
 /* radix_sort (radix_sort.hpp:9-53, radix_sort.cpp:149-338): stable LSD sort of uint32 keys (and, if vals is
  * not NULL, uint32 values carried with them) over key bits [begin_bit, end_bit). The result is left in

@@ -120,6 +120,11 @@ def test_primitive_argument_validation(lib, hctx):
     assert lib.vren_reduce_vec4(hctx, None, None, 10, buf, 0, 1) == EINVAL
     assert lib.vren_reduce_total_u32(hctx, None, buf, 0, buf, 0) == EINVAL
     assert lib.vren_exclusive_scan_u32(hctx, None, buf, buf, 0, 1) == EINVAL
+    # allocation-free scan: beyond its fixed workspace of 2^16 tiles of 16384 keys, then no device
+    assert lib.vren_exclusive_scan_u32_noalloc(hctx, None, buf, buf, 16, 65537) == EINVAL
+    assert b"fixed workspace" in lib.vren_last_error(hctx)
+    assert lib.vren_exclusive_scan_u32_noalloc(hctx, None, buf, buf, 16, 65536) == EINVAL
+    assert b"no device" in lib.vren_last_error(hctx)
     # radix sort: n == 0, n >= 2^30, bad bit range, too little scratch (radix_sort.cpp:158-161 throws on bad length)
     sb = lib.vren_radix_sort_scratch_bytes(1000, 1)
     assert lib.vren_radix_sort(hctx, None, buf, buf, 0, 0, 32, buf, sb) == EINVAL

@@ -160,7 +160,7 @@ struct FrameDev; // clustered.hpp
 struct vren_ctx {
     int device = 0;
     int num_cus = 256;
-    std::string last_error;
+    char last_error[512] = {0};        // fixed buffer: reporting an error allocates nothing
     uint32_t* d_flags = nullptr;       // device capacity flags
     // grow-only device scratch for primitives that do not take a caller scratch buffer
     void* scratch = nullptr;
@@ -168,6 +168,8 @@ struct vren_ctx {
     // ring of device frame-descriptor slots for the single-frame stage entry points
     FrameDev* frame_slots = nullptr;
     uint32_t frame_slot_next = 0;
+    // this device's copy of the fixed workspace of vren_exclusive_scan_u32_noalloc (scan.hip)
+    void* noalloc_scan_work = nullptr;
     // radix-sort autotuner: forced configuration (-1 = none) and tuned choices keyed by problem bucket
     int radix_forced = -1;
     std::map<uint64_t, int> radix_tuned;
@@ -178,6 +180,7 @@ struct vren_ctx {
 int vren_fail(vren_ctx* ctx, int status, const char* fmt, ...);
 int vren_hip_fail(vren_ctx* ctx, hipError_t err, const char* what);
 int vren_ensure_scratch(vren_ctx* ctx, size_t bytes);
+int vren_noalloc_scan_setup(vren_ctx* ctx);
 
 #define VREN_HIP(ctx, expr)                                                                                            \
     do {                                                                                                               \

@@ -9,12 +9,10 @@
 int vren_fail(vren_ctx* ctx, int status, const char* fmt, ...)
 {
     if (ctx) {
-        char buf[512];
         va_list ap;
         va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
+        vsnprintf(ctx->last_error, sizeof(ctx->last_error), fmt, ap);
         va_end(ap);
-        ctx->last_error = buf;
     }
     return status;
 }
@@ -65,6 +63,7 @@ VREN_EXPORT int vren_create(int hip_device, vren_ctx** out)
     if (e == hipSuccess) e = hipMalloc(&ctx->d_flags, 64);
     if (e == hipSuccess) e = hipMemset(ctx->d_flags, 0, 64);
     if (e == hipSuccess) e = hipMalloc(&ctx->frame_slots, sizeof(FrameDev) * VREN_FRAME_SLOTS);
+    if (e == hipSuccess && vren_noalloc_scan_setup(ctx) != VREN_OK) e = hipErrorInvalidSymbol;
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         vren_destroy(ctx);
@@ -101,7 +100,7 @@ VREN_EXPORT const char* vren_status_string(int status)
     }
 }
 
-VREN_EXPORT const char* vren_last_error(vren_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+VREN_EXPORT const char* vren_last_error(vren_ctx* ctx) { return ctx ? ctx->last_error : ""; }
 
 VREN_EXPORT int vren_sync_status(vren_ctx* ctx, vren_stream_t stream)
 {

@@ -136,3 +136,42 @@ extern "C" VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t 
     if (st) return st;
     return vren_scan_with_work(ctx, (hipStream_t)stream, in, out, n, blocks, ctx->scratch);
 }
+
+// ---- allocation-free variant -------------------------------------------------------------------------------------
+// Callers that cannot allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+// hipMalloc) get the same scan over a fixed workspace: a file-scope device array, of which every GPU holds its own
+// copy at its own address until the process ends. Its first 256 bytes are the tile ticket counter, the rest one
+// 64-bit look-back status word per tile, for at most kNoAllocTiles tiles over all segments (2^30 keys). The host
+// never passes the array's name to a kernel (on the host it is only a stand-in for the symbol): vren_create, with
+// its device current, resolves this device's address once into ctx->noalloc_scan_work, and the entry point passes
+// that. Every call on one GPU shares the array; scan_init_kernel clears the counter and the status words at the
+// start of each call, so calls in sequence on one stream are independent, and calls that overlap (other streams,
+// other contexts on the same GPU) race.
+constexpr uint64_t kNoAllocTiles = 1ull << 16;
+constexpr size_t kNoAllocBytes = 256 + kNoAllocTiles * sizeof(uint64_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+int vren_noalloc_scan_setup(vren_ctx* ctx)
+{
+    void* p = nullptr;
+    VREN_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ctx->noalloc_scan_work = p;
+    return VREN_OK;
+}
+
+extern "C" VREN_EXPORT int vren_exclusive_scan_u32_noalloc(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in,
+                                                           uint32_t* out, uint64_t n, uint32_t blocks)
+{
+    if (!ctx) return VREN_EINVAL;
+    if (!in || !out || n == 0 || blocks == 0)
+        return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32_noalloc: null buffer, n == 0 or blocks == 0");
+    if (n > kNoAllocTiles * kTile || ceil_div_u64(n, kTile) * blocks > kNoAllocTiles)
+        return vren_fail(ctx, VREN_EINVAL,
+                         "exclusive_scan_u32_noalloc: %llu x %u keys need more than the fixed workspace's %llu tiles "
+                         "of %d keys",
+                         (unsigned long long)n, blocks, (unsigned long long)kNoAllocTiles, kTile);
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32_noalloc: context has no device");
+    if (!ctx->noalloc_scan_work)
+        return vren_fail(ctx, VREN_EHIP, "exclusive_scan_u32_noalloc: workspace address not resolved");
+    return vren_scan_with_work(ctx, (hipStream_t)stream, in, out, n, blocks, ctx->noalloc_scan_work);
+}
