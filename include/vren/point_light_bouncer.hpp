@@ -22,6 +22,10 @@ namespace vren_demo
         {
             (void) frame_idx;
             (void) resource_container;
+            const size_t need = (size_t) point_light_count * 16;
+            if (point_light_position_buffer.m_size < need || point_light_direction_buffer.m_size < need)
+                throw std::invalid_argument("vren_demo::point_light_bouncer: position and direction buffers need "
+                                            "point_light_count * 16 bytes");
             vren_ctx* ctx = m_context->handle();
             const float lo[3] = {aabb_min.x, aabb_min.y, aabb_min.z};
             const float hi[3] = {aabb_max.x, aabb_max.y, aabb_max.z};

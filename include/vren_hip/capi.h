@@ -8,7 +8,10 @@
  *   - calls are asynchronous on `stream` (a hipStream_t; NULL = the legacy default stream);
  *   - return 0 (VREN_OK) or a negative vren_status; arguments are validated on the host before any HIP call,
  *     so invalid lengths / capacities are reported without touching the device;
- *   - a vren_ctx is not thread-safe (one per host thread), mirroring single-threaded command recording;
+ *   - a vren_ctx is not thread-safe (one per host thread), mirroring single-threaded command recording, and is
+ *     used from one stream at a time: the primitives that take no caller scratch share the context's scratch
+ *     buffer. Growing it synchronises the device, so a call that would grow it during a stream capture fails
+ *     with VREN_EINVAL (make the same call once outside the capture first);
  *   - capacity overflows that only the device can see (unique clusters, assigned lights) never write out of
  *     bounds: they raise a device-side flag that vren_sync_status() reports as VREN_ECAPACITY
  *     (the reference writes past its buffers silently, config.hpp:23-24).
@@ -126,6 +129,14 @@ VREN_EXPORT int vren_reduce_total_u32(vren_ctx* ctx, vren_stream_t stream, const
  * Any n >= 1 (the reference requires a power of 2). */
 VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t stream, const uint32_t* in, uint32_t* out,
                                         uint64_t n, uint32_t blocks);
+/* blelloch_scan::downsweep (blelloch_scan.hpp:31-39, blelloch_scan.cpp:57-139, blelloch_scan_downsweep.comp):
+ * in place on `blocks` segments of `length` (a power of 2) elements. Every element i becomes
+ * P + sum over the set bits l of i of d[((i >> (l+1)) << (l+1)) + 2^l - 1] (the reference's level-by-level swaps
+ * and adds, unrolled), where P = 0 if clear_last or length < 1024 (the reference zero-pads shorter segments in its
+ * 1024-wide LDS pass), else the segment's last element. On an up-sweep tree (vren_reduce_u32 add) with clear_last
+ * this is the exclusive prefix sum of the leaves. */
+VREN_EXPORT int vren_blelloch_downsweep_u32(vren_ctx* ctx, vren_stream_t stream, uint32_t* data, uint32_t length,
+                                            uint32_t blocks, int clear_last);
 /* No counterpart in the reference: vren_exclusive_scan_u32 for callers that cannot allocate device memory (code
  * under stream capture, tight serving loops, harnesses that forbid hipMalloc). The same kernels and results, with
  * the look-back workspace in a fixed device array (one copy per GPU, 512 KiB + 256 B, resolved by vren_create), so

@@ -295,6 +295,54 @@ OEXPORT void o_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n,
     }
 }
 
+/* blelloch_scan::downsweep (blelloch_scan.cpp:57-139, blelloch_scan_downsweep.comp:33-125), restated level by
+ * level as the reference dispatches it, in place on `blocks` segments of `length` (a power of 2) elements:
+ * global levels log2(length) - 1 down to 10 (the first one clears the segment's last element when clear_last),
+ * then the 1024-thread LDS pass over each 1024-element chunk, whose shared array is zero-padded past the segment
+ * and whose last element is not loaded (reads 0) when clear_last is still set. */
+OEXPORT void o_blelloch_downsweep_u32(uint32_t* data, uint32_t length, uint32_t blocks, int clear_last_in)
+{
+    const uint32_t wg = 1024;
+    for (uint32_t b = 0; b < blocks; b++) {
+        uint32_t* d = data + (uint64_t)b * length;
+        int clear_last = clear_last_in;
+        int log_len = (int)ilog2(length);
+        for (int level = log_len - 1; level > 9; level--) {
+            uint32_t stride = 1u << level, offset = stride - 1;
+            for (uint64_t x = 0;; x++) {
+                uint64_t a = offset + x * stride * 2ull, bi = a + stride;
+                if (bi >= length) break;
+                uint32_t bv = (clear_last && bi == length - 1) ? 0u : d[bi];
+                d[bi] = d[a] + bv;
+                d[a] = bv;
+            }
+            clear_last = 0;
+        }
+        uint32_t groups = (length + wg - 1) / wg;
+        std::vector<uint32_t> sh(wg);
+        for (uint32_t g = 0; g < groups; g++) {
+            for (uint32_t t = 0; t < wg; t++) {
+                uint64_t idx = (uint64_t)g * wg + t;
+                sh[t] = idx < (uint64_t)length - (clear_last ? 1u : 0u) ? d[idx] : 0u;
+            }
+            for (int level = 9; level >= 0; level--) {
+                uint32_t mask = (1u << (level + 1)) - 1;
+                for (uint32_t t = 0; t < wg; t++) {
+                    if ((t & mask) != mask) continue;
+                    uint32_t bi = t, a = bi - (1u << level);
+                    uint32_t tmp = sh[bi];
+                    sh[bi] = sh[a] + tmp;
+                    sh[a] = tmp;
+                }
+            }
+            for (uint32_t t = 0; t < wg; t++) {
+                uint64_t idx = (uint64_t)g * wg + t;
+                if (idx < length) d[idx] = sh[t];
+            }
+        }
+    }
+}
+
 /* std::stable_sort by key bits [begin_bit, end_bit) (radix_sort.cpp:88 uses std::sort on keys only) */
 OEXPORT void o_stable_sort_kv(uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t begin_bit, uint32_t end_bit)
 {

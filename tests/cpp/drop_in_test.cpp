@@ -94,6 +94,17 @@ static void host_only()
     EXPECT(throws_invalid_argument([&] { bvh({}, rc, fake, 33); }), "bvh leaf count 33");
     vren::reduce<uint32_t, vren::ReduceOperationAdd> red(ctx);
     EXPECT(throws_invalid_argument([&] { red({}, rc, fake, 0, 0, 1); }), "reduce n = 0");
+    vren::blelloch_scan scan(ctx);
+    // blelloch_scan.cpp:68: the down-sweep's length must be a power of 2
+    EXPECT(throws_invalid_argument([&] { scan.downsweep({}, rc, fake, 1000, 0, 1, true); }), "downsweep length 1000");
+    vren_demo::point_light_bouncer bouncer(ctx);
+    vren::device_buffer small = fake;
+    small.m_size = 16 * 10;
+    EXPECT(throws_invalid_argument([&] {
+               bouncer.bounce(0, {}, rc, small, fake, 11, vren::vec3{-1.0f, -1.0f, -1.0f}, vren::vec3{1.0f, 1.0f, 1.0f},
+                              1.0f, 0.1f);
+           }),
+           "bouncer buffers smaller than 16 * count bytes");
     vren::clustered_shading::find_unique_cluster_list fuc(ctx);
     vren::camera cam;
     cam.m_near_plane = 0.0f;
@@ -228,6 +239,22 @@ static void primitives(vren::context& ctx)
         EXPECT(ok, "bucket_sort order");
         uint32_t key0 = r[idx[0]] & 0xFFFF, key_last = r[idx[n - 1]] & 0xFFFF;
         EXPECT(off[key0] == 0 && off[key_last] <= n - 1 && off[65535] <= n, "bucket offsets");
+    }
+    // blelloch_scan::downsweep after reduce, the radix sort's use (radix_sort.cpp:230-289): 16 segments of 2^14
+    {
+        const uint32_t len = 1u << 14, blocks = 16;
+        auto keys = pcg_keys(49, 54, len * blocks);
+        auto buf = upload(keys);
+        vren::reduce<uint32_t, vren::ReduceOperationAdd> red(ctx);
+        vren::blelloch_scan scan(ctx);
+        red(s, rc, buf, len, 0, blocks);
+        scan.downsweep(s, rc, buf, len, 0, blocks, true);
+        ctx.sync(s);
+        auto got = download<uint32_t>(buf, len * blocks);
+        std::vector<uint32_t> want(len * blocks);
+        for (uint32_t b = 0; b < blocks; b++)
+            std::exclusive_scan(keys.begin() + b * len, keys.begin() + (b + 1) * len, want.begin() + b * len, 0u);
+        EXPECT(got == want, "reduce + downsweep == std::exclusive_scan");
     }
     // point_light_bouncer: a light with no wall within reach moves by exactly speed * dt * d (w = 1 and 0)
     {

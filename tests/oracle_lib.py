@@ -98,6 +98,11 @@ class Oracle:
         self.lib.o_exclusive_scan_u32(_p(x), _p(out), c_u64(x.size // blocks), c_u32(blocks))
         return out
 
+    def blelloch_downsweep_u32(self, x, length, blocks=1, clear_last=True):
+        d = np.array(x, np.uint32, copy=True)
+        self.lib.o_blelloch_downsweep_u32(_p(d), c_u32(length), c_u32(blocks), c_int(1 if clear_last else 0))
+        return d
+
     def stable_sort_kv(self, keys, vals=None, begin_bit=0, end_bit=32):
         k = np.array(keys, np.uint32, copy=True)
         v = None if vals is None else np.array(vals, np.uint32, copy=True)

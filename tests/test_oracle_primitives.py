@@ -179,6 +179,63 @@ def test_exclusive_scan_vs_numpy(oracle):
     assert np.array_equal(oracle.exclusive_scan_u32(ones), np.arange(1 << 12, dtype=np.uint32))
 
 
+def _numpy_upsweep(x):
+    """Up-sweep tree of x (a power-of-2 length), built by pairwise sums level by level in numpy."""
+    t = x.astype(np.uint64).copy()
+    n = t.size
+    step = 1
+    while step < n:
+        t[2 * step - 1::2 * step] += t[step - 1::2 * step]
+        step *= 2
+    return (t & 0xFFFFFFFF).astype(np.uint32)
+
+
+def _exclusive(x):
+    c = np.concatenate([np.zeros(1, np.uint64), np.cumsum(x.astype(np.uint64))[:-1]])
+    return (c & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("length", [1, 2, 8, 16, 512, 1024, 2048, 1 << 14])
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_blelloch_downsweep_of_upsweep_trees(oracle, length, blocks):
+    """downsweep(clear_last = true) of an up-sweep tree is the exclusive scan of the leaves (the use in
+    radix_sort.cpp:230-289 and blelloch_scan.cpp:141-166); with clear_last = false the root's value becomes the
+    prefix of the whole block, except in blocks shorter than 1024, where the reference's 1024-wide LDS pass
+    overwrites the root with its zero padding's prefix."""
+    rng = np.random.default_rng(length + blocks)
+    x = rng.integers(0, 2 ** 32, (blocks, length), dtype=np.uint64).astype(np.uint32)
+    trees = np.concatenate([_numpy_upsweep(r) for r in x])
+    want = np.concatenate([_exclusive(r) for r in x])
+    assert np.array_equal(oracle.blelloch_downsweep_u32(trees, length, blocks, True), want)
+    kept = oracle.blelloch_downsweep_u32(trees, length, blocks, False)
+    for b in range(blocks):
+        total = np.uint32(np.sum(x[b], dtype=np.uint64) & 0xFFFFFFFF)
+        root = total if length >= 1024 else np.uint32(0)
+        assert np.array_equal(kept[b * length:(b + 1) * length], want[b * length:(b + 1) * length] + root), b
+
+
+@pytest.mark.parametrize("length", [1, 4, 32, 1024, 8192])
+def test_blelloch_downsweep_closed_form_on_arbitrary_input(oracle, length):
+    """On any input (not only an up-sweep tree) each output is P plus, for every set bit l of its index i, the input
+    at the last element of i's left sibling at that level, ((i >> (l + 1)) << (l + 1)) + 2^l - 1 (no level above l
+    writes that element); P = 0 when clear_last or length < 1024, else the last input element."""
+    rng = np.random.default_rng(length)
+    d = rng.integers(0, 2 ** 32, 2 * length, dtype=np.uint64).astype(np.uint32)
+    for clear in (True, False):
+        got = oracle.blelloch_downsweep_u32(d, length, 2, clear)
+        for b in range(2):
+            seg = d[b * length:(b + 1) * length].astype(np.uint64)
+            P = 0 if (clear or length < 1024) else int(seg[-1])
+            i = np.arange(length)
+            want = np.full(length, P, np.uint64)
+            lvl = 0
+            while (1 << lvl) < length:
+                sib = ((i >> (lvl + 1)) << (lvl + 1)) + (1 << lvl) - 1
+                want += np.where((i >> lvl) & 1, seg[sib], 0)
+                lvl += 1
+            assert np.array_equal(got[b * length:(b + 1) * length], (want & 0xFFFFFFFF).astype(np.uint32)), (clear, b)
+
+
 def test_stable_sort_vs_numpy(oracle):
     rng = np.random.default_rng(4)
     k = rng.integers(0, 64, 5000, dtype=np.uint32)  # many ties

@@ -90,6 +90,33 @@ def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, c1_keys):
         assert np.array_equal(from_dev(o, np.uint32, (n,)), np.arange(n, dtype=np.uint32))
 
 
+@pytest.mark.parametrize("blocks", [1, 16])
+@pytest.mark.parametrize("length", [16, 1024, 1 << 12, 1 << 17, 1 << 20])
+def test_reduce_then_downsweep_is_exclusive_scan(vctx, oracle, c1_keys, length, blocks):
+    """The reference's radix-sort use (radix_sort.cpp:230-289): reduce (add, in place) over `blocks` segments, then
+    downsweep(clear_last = true): every segment holds std::exclusive_scan of its keys."""
+    n = length * blocks
+    x = np.resize(c1_keys, n)
+    d = to_dev(x)
+    vctx.reduce_u32(d, length, d, 0, blocks)
+    vctx.blelloch_downsweep_u32(d, length, blocks, True)
+    assert np.array_equal(from_dev(d, np.uint32, (n,)), oracle.exclusive_scan_u32(x, blocks=blocks))
+
+
+@pytest.mark.parametrize("length", [1, 2, 4, 8, 16, 32, 512, 1024, 2048, 4096, 8192, 1 << 16, 1 << 22])
+def test_downsweep_arbitrary_input_vs_oracle(vctx, oracle, c1_keys, length):
+    """On arbitrary input (not an up-sweep tree), both clear_last settings, 1 and 3 segments (the second and third
+    not 16-byte aligned when length < 4): bit for bit against the oracle's level-by-level restatement."""
+    for blocks in (1, 3):
+        n = length * blocks
+        x = np.resize(c1_keys[::-1], n)
+        for clear in (True, False):
+            d = to_dev(x)
+            vctx.blelloch_downsweep_u32(d, length, blocks, clear)
+            want = oracle.blelloch_downsweep_u32(x, length, blocks, clear)
+            assert np.array_equal(from_dev(d, np.uint32, (n,)), want), (blocks, clear)
+
+
 def _sort_gpu(vctx, keys, vals, begin=0, end=32):
     n = keys.size
     dk = to_dev(keys)

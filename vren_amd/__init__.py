@@ -88,6 +88,10 @@ class Context:
     def exclusive_scan_u32(self, inp, out, n, blocks=1):
         return self.check(self.lib.vren_exclusive_scan_u32(self.h, self.stream(), _ptr(inp), _ptr(out), n, blocks))
 
+    def blelloch_downsweep_u32(self, data, length, blocks=1, clear_last=True):
+        return self.check(self.lib.vren_blelloch_downsweep_u32(self.h, self.stream(), _ptr(data), length, blocks,
+                                                               1 if clear_last else 0))
+
     def radix_sort_scratch_bytes(self, n, with_values):
         return self.lib.vren_radix_sort_scratch_bytes(n, 1 if with_values else 0)
 

@@ -61,6 +61,7 @@ PROTOTYPES = {
     "vren_reduce_vec4": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_int, c_uint32]),
     "vren_reduce_total_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_int]),
     "vren_exclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32]),
+    "vren_blelloch_downsweep_u32": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_int]),
     "vren_exclusive_scan_u32_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32]),
     "vren_radix_sort_scratch_bytes": (c_size_t, [c_uint64, c_int]),
     "vren_radix_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, c_void_p,

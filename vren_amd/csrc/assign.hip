@@ -586,7 +586,7 @@ extern "C" VREN_EXPORT int vren_assign_lights(vren_ctx* ctx, vren_stream_t strea
     ChainWork w;
     memset(&w, 0, sizeof(w));
     const uint32_t offs_tiles = (key_capacity + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
-    st = vren_ensure_scratch(ctx, chain_work_bytes(0, offs_tiles, 0));
+    st = vren_ensure_scratch(ctx, chain_work_bytes(0, offs_tiles, 0), s);
     if (st) return st;
     chain_work_bind(&w, ctx->scratch, 0, offs_tiles, 0);
     w.max_L = light_count;

@@ -234,15 +234,24 @@ VREN_EXPORT int vren_batch_run(vren_batch* b, vren_stream_t stream, int mode)
     if (mode == VREN_RUN_BATCHED) return run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots);
     if (mode == VREN_RUN_GRAPH) {
         if (!b->exec) {
-            VREN_HIP(ctx, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+            // the capture stream lives as long as the batch (created once, destroyed by vren_batch_destroy); a
+            // failed capture or instantiation leaves no graph behind, so the next call starts over
+            if (!b->capture) VREN_HIP(ctx, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
             VREN_HIP(ctx, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
             int st = run_chain(b, b->capture, b->d_table, b->F, b->w, b->total_live_slots);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->capture, &g);
-            if (st) return st;
-            if (e != hipSuccess) return vren_hip_fail(ctx, e, "hipStreamEndCapture");
+            if (st || e != hipSuccess) {
+                if (g) vren_teardown(hipGraphDestroy(g));
+                return st ? st : vren_hip_fail(ctx, e, "hipStreamEndCapture");
+            }
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                b->exec = nullptr;
+                vren_teardown(hipGraphDestroy(g));
+                return vren_hip_fail(ctx, e, "hipGraphInstantiate");
+            }
             b->graph = g;
-            VREN_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         }
         VREN_HIP(ctx, hipGraphLaunch(b->exec, s));
         return VREN_OK;

@@ -60,7 +60,7 @@ extern "C" VREN_EXPORT int vren_bucket_sort(vren_ctx* ctx, vren_stream_t stream,
     // bits of the key at or above 32 are zero in the bucket number and need no sorting
     const uint32_t end_bit = std::min<uint32_t>(32u, key_shift + bits);
     const size_t rs_bytes = vren_radix_sort_scratch_bytes(n, 1);
-    int st = vren_ensure_scratch(ctx, rs_bytes);
+    int st = vren_ensure_scratch(ctx, rs_bytes, s);
     if (st) return st;
     st = vren_radix_sort_pairs_interleaved(ctx, s, in_pairs, out_pairs, n, key_shift, end_bit, ctx->scratch,
                                            rs_bytes);

@@ -475,7 +475,7 @@ extern "C" VREN_EXPORT int vren_find_unique_cluster_list(vren_ctx* ctx, vren_str
     ChainWork w;
     memset(&w, 0, sizeof(w));
     uint32_t tiles = f.Sx * f.Sy;
-    st = vren_ensure_scratch(ctx, chain_work_bytes(tiles, 0, 1));
+    st = vren_ensure_scratch(ctx, chain_work_bytes(tiles, 0, 1), s);
     if (st) return st;
     chain_work_bind(&w, ctx->scratch, tiles, 0, 1);
     FrameDev* slot = vren_next_frame_slot(ctx);

@@ -179,7 +179,7 @@ struct vren_ctx {
 
 int vren_fail(vren_ctx* ctx, int status, const char* fmt, ...);
 int vren_hip_fail(vren_ctx* ctx, hipError_t err, const char* what);
-int vren_ensure_scratch(vren_ctx* ctx, size_t bytes);
+int vren_ensure_scratch(vren_ctx* ctx, size_t bytes, hipStream_t s);
 int vren_noalloc_scan_setup(vren_ctx* ctx);
 
 #define VREN_HIP(ctx, expr)                                                                                            \

@@ -23,9 +23,21 @@ int vren_hip_fail(vren_ctx* ctx, hipError_t err, const char* what)
                      hipGetErrorString(err));
 }
 
-int vren_ensure_scratch(vren_ctx* ctx, size_t bytes)
+// The context's scratch serves one stream at a time (a context is used from one stream, like the reference's
+// single command buffer per recording thread). Growing it synchronises the device and frees the old buffer, which
+// must not happen while `s` is being captured: the capture would break, and a graph captured earlier would keep a
+// pointer into freed memory. A capture therefore needs the scratch already grown by an earlier call of the same
+// size outside the capture.
+int vren_ensure_scratch(vren_ctx* ctx, size_t bytes, hipStream_t s)
 {
     if (bytes <= ctx->scratch_bytes) return VREN_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    VREN_HIP(ctx, hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return vren_fail(ctx, VREN_EINVAL,
+                         "context scratch must grow to %zu bytes during a stream capture; make the same call once "
+                         "outside the capture first",
+                         bytes);
     if (ctx->scratch) {
         // a grown scratch replaces the old one only after the device is done with it
         VREN_HIP(ctx, hipDeviceSynchronize());

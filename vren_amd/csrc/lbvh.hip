@@ -100,7 +100,7 @@ extern "C" VREN_EXPORT int vren_build_lbvh(vren_ctx* ctx, vren_stream_t stream, 
     if (n > (1u << 20)) return vren_fail(ctx, VREN_EINVAL, "build_lbvh: at most 2^20 leaves");
     if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "build_lbvh: context has no device");
     hipStream_t s = (hipStream_t)stream;
-    int st = vren_ensure_scratch(ctx, (size_t)n * 4);
+    int st = vren_ensure_scratch(ctx, (size_t)n * 4, s);
     if (st) return st;
     uint32_t* counters = static_cast<uint32_t*>(ctx->scratch);
     const uint2* codes = reinterpret_cast<const uint2*>(light_codes);

@@ -400,7 +400,7 @@ VREN_EXPORT int vren_construct_point_light_bvh(vren_ctx* ctx, vren_stream_t stre
     f.bvh_levels = vren_calc_bvh_level_count(light_count);
     f.bvh_length = vren_calc_bvh_buffer_length(light_count);
     if (light_count > kLdsCap) {
-        st = vren_ensure_scratch(ctx, (size_t)light_count * 8);
+        st = vren_ensure_scratch(ctx, (size_t)light_count * 8, s);
         if (st) return st;
         f.codes_tmp = static_cast<uint32_t*>(ctx->scratch);
     }

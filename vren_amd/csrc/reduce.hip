@@ -167,7 +167,7 @@ int reduce_tree(vren_ctx* ctx, hipStream_t s, const T* in, uint32_t n, T* out, u
     const uint64_t t1 = p > TILE ? p / TILE : 0, t2 = t1 > TILE ? t1 / TILE : 0;
     const size_t a_bytes = (size_t)((t1 * blocks * sizeof(T) + 255) & ~(uint64_t)255);
     if (t1) {
-        int st = vren_ensure_scratch(ctx, a_bytes + t2 * blocks * sizeof(T));
+        int st = vren_ensure_scratch(ctx, a_bytes + t2 * blocks * sizeof(T), s);
         if (st) return st;
     }
     T* buf[2] = {static_cast<T*>(ctx->scratch), reinterpret_cast<T*>(static_cast<char*>(ctx->scratch) + a_bytes)};
@@ -247,7 +247,7 @@ template <typename Op>
 int reduce_total(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint64_t n, uint32_t* out)
 {
     uint32_t blocks = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 8, std::max<uint64_t>(1, ceil_div_u64(n, 4096)));
-    int st = vren_ensure_scratch(ctx, blocks * sizeof(uint32_t));
+    int st = vren_ensure_scratch(ctx, blocks * sizeof(uint32_t), s);
     if (st) return st;
     uint32_t* partials = static_cast<uint32_t*>(ctx->scratch);
     total_partial_kernel<Op><<<blocks, kThreads, 0, s>>>(in, n, partials);

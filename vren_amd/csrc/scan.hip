@@ -132,7 +132,7 @@ extern "C" VREN_EXPORT int vren_exclusive_scan_u32(vren_ctx* ctx, vren_stream_t 
     if (ceil_div_u64(n, kTile) * blocks >= (1ull << 31))
         return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: input too large");
     if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "exclusive_scan_u32: context has no device");
-    int st = vren_ensure_scratch(ctx, vren_scan_work_bytes(n, blocks));
+    int st = vren_ensure_scratch(ctx, vren_scan_work_bytes(n, blocks), (hipStream_t)stream);
     if (st) return st;
     return vren_scan_with_work(ctx, (hipStream_t)stream, in, out, n, blocks, ctx->scratch);
 }

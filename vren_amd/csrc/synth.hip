@@ -134,7 +134,7 @@ VREN_EXPORT int vren_synth_frame(vren_ctx* ctx, vren_stream_t stream, uint64_t i
     pp.W = width;
     pp.H = height;
     pp.n_boxes = n_boxes;
-    st = vren_ensure_scratch(ctx, sizeof(double) * 6 * (n_boxes + 1));
+    st = vren_ensure_scratch(ctx, sizeof(double) * 6 * (n_boxes + 1), s);
     if (st) return st;
     double* boxes = static_cast<double*>(ctx->scratch);
     if (n_boxes) {
