@@ -21,6 +21,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "../vren_hip/capi.h"
 
@@ -96,9 +97,58 @@ namespace vren
         }
     };
 
+    // texture_manager.hpp:29-47: the bindless texture array shade samples. Each texture is an RGBA8 UNORM device
+    // image (tinygltf_parser.cpp:157) with its sampler state; rewrite_descriptor_set() uploads the descriptor
+    // table after textures are added, as the reference rewrites its descriptor set.
+    struct texture
+    {
+        vren::device_buffer m_texels; // height rows of width RGBA8 texels, 4-byte aligned
+        uint32_t m_width = 0, m_height = 0;
+        uint32_t m_filter = VREN_FILTER_LINEAR;
+        uint32_t m_address_u = VREN_ADDRESS_REPEAT, m_address_v = VREN_ADDRESS_REPEAT;
+    };
+
+    class texture_manager
+    {
+    public:
+        static constexpr uint32_t k_max_texture_count = 65536;
+
+        std::vector<vren::texture> m_textures;
+
+        void rewrite_descriptor_set()
+        {
+            if (m_textures.size() > k_max_texture_count)
+                throw std::invalid_argument("texture_manager: more than 65536 textures");
+            std::vector<vren_texture> d(m_textures.size());
+            for (size_t i = 0; i < m_textures.size(); i++) {
+                vren::texture const& t = m_textures[i];
+                if (t.m_texels.m_size < (size_t) t.m_width * t.m_height * 4)
+                    throw std::invalid_argument("texture_manager: texel buffer smaller than width * height * 4");
+                d[i] = vren_texture{t.m_texels.at<uint8_t>(), t.m_width, t.m_height, t.m_filter, t.m_address_u,
+                                    t.m_address_v, 0u};
+            }
+            if (m_table.m_size < d.size() * sizeof(vren_texture)) m_table = owned_device_buffer(d.size() * sizeof(vren_texture));
+            if (!d.empty()) {
+                hipError_t e = hipMemcpy(m_table.m_ptr, d.data(), d.size() * sizeof(vren_texture), hipMemcpyHostToDevice);
+                if (e != hipSuccess) throw std::runtime_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+            }
+            m_count = (uint32_t) d.size();
+        }
+
+        vren_texture const* device_table() const { return m_count ? m_table.at<vren_texture>() : nullptr; }
+        uint32_t count() const { return m_count; }
+
+    private:
+        owned_device_buffer m_table;
+        uint32_t m_count = 0;
+    };
+
     class context
     {
     public:
+        // the reference keeps its texture manager in the context's toolbox (m_context->m_toolbox->m_texture_manager)
+        mutable vren::texture_manager m_texture_manager;
+
         explicit context(int hip_device = 0)
         {
             check_status(nullptr, vren_create(hip_device, &m_handle), "vren_create");

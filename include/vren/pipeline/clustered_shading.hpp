@@ -3,13 +3,15 @@
 // The four stages keep the reference's class names and operator() argument order. Differences at the boundary:
 //   - images become device buffers: the D32 depth (f32[W*H]), the RGBA16F normals (f16x4[W*H]), the R32UI cluster
 //     reference image (u32[W*H]) and the RGBA32F output (f32x4[W*H]), all row-major;
-//   - the G-buffer carries resolved per-pixel materials (albedo + roughness f32x4, metallic f32) instead of the
-//     texcoord / material-index pair that shade.comp:89-96 resolves through bindless textures;
+//   - shade and cluster_and_shade take the reference's material path (texcoord + material index G-buffer planes,
+//     a vren::material_buffer and the context's texture manager, resolved as shade.comp:89-96 does) and, as a
+//     second overload, per-pixel resolved materials (albedo + roughness f32x4, metallic f32);
 //   - capacities are the sizes of the buffers passed in (the reference assumes its config.hpp maxima), and an
 //     overflow is reported by vren::context::sync() instead of writing out of bounds;
 //   - each stage takes the cluster configuration (vren_cluster_cfg) in m_config, defaulting to the reference's.
 #pragma once
 
+#include "../material.hpp"
 #include "../primitives/build_bvh.hpp"
 
 namespace vren
@@ -70,9 +72,13 @@ namespace vren
         uint32_t m_directional_light_count = 0;
     };
 
+    // gbuffer.hpp / gbuffer.cpp:7-27: normals (RGBA16F), texcoords (RG32F) and material indices (R16_UINT); the
+    // resolved-material planes serve the overloads that take no material buffer
     struct gbuffer
     {
         vren::device_buffer m_normal_buffer;           // f16x4[W*H]
+        vren::device_buffer m_texcoord_buffer;         // f32x2[W*H]
+        vren::device_buffer m_material_index_buffer;   // u16[W*H]
         vren::device_buffer m_albedo_roughness_buffer; // f32x4[W*H]
         vren::device_buffer m_metallic_buffer;         // f32[W*H]
     };
@@ -270,6 +276,52 @@ namespace vren
                     "vren::clustered_shading::shade");
             }
 
+            // the reference's signature (clustered_shading.hpp:129-140): materials resolved from the G-buffer's
+            // texcoords and material indices, the material buffer and the context's textures (shade.comp:89-96)
+            void operator()(uint32_t frame_idx, vren::stream stream, vren::resource_container& resource_container,
+                            vren::uvec2 const& screen, vren::camera const& camera, vren::gbuffer const& gbuffer,
+                            vren::depth_buffer_t const& depth_buffer,
+                            vren::device_buffer const& cluster_reference_buffer,
+                            vren::device_buffer const& assigned_light_indices_buffer,
+                            vren::device_buffer const& assigned_light_counts_buffer,
+                            vren::device_buffer const& assigned_light_offsets_buffer,
+                            vren::light_array const& light_array, vren::material_buffer const& material_buffer,
+                            vren::device_buffer const& output) const
+            {
+                (void) frame_idx;
+                (void) resource_container;
+                const size_t px = (size_t) screen.x * screen.y;
+                if (output.m_size < px * 16 || gbuffer.m_texcoord_buffer.m_size < px * 8 ||
+                    gbuffer.m_material_index_buffer.m_size < px * 2)
+                    throw std::invalid_argument("shade: output, texcoord or material-index buffer too small");
+                size_t key_cap = assigned_light_counts_buffer.m_size < assigned_light_offsets_buffer.m_size
+                                     ? assigned_light_counts_buffer.m_size
+                                     : assigned_light_offsets_buffer.m_size;
+                vren_material_set ms;
+                ms.texcoords = gbuffer.m_texcoord_buffer.at<float>();
+                ms.material_indices = gbuffer.m_material_index_buffer.at<uint16_t>();
+                ms.materials = material_buffer.m_material_count ? material_buffer.m_buffer.at<vren_material>() : nullptr;
+                ms.material_count = material_buffer.m_material_count;
+                ms.texture_count = m_context->m_texture_manager.count();
+                ms.textures = m_context->m_texture_manager.device_table();
+                vren_ctx* ctx = m_context->handle();
+                vren_camera c = camera.to_c();
+                check_status(
+                    ctx,
+                    vren_shade_materials(
+                        ctx, stream.get(), screen.x, screen.y, &c, &m_config, depth_buffer.m_buffer.at<float>(),
+                        gbuffer.m_normal_buffer.at<uint16_t>(), &ms, cluster_reference_buffer.at<uint32_t>(),
+                        (uint32_t) (key_cap / 4), assigned_light_counts_buffer.at<uint32_t>(),
+                        assigned_light_offsets_buffer.at<uint32_t>(), assigned_light_indices_buffer.at<uint32_t>(),
+                        (uint32_t) (assigned_light_indices_buffer.m_size / 4),
+                        light_array.m_point_light_position_buffer.at<float>(),
+                        light_array.m_point_light_buffer.at<float>(), light_array.m_point_light_count,
+                        light_array.m_directional_light_count ? light_array.m_directional_light_buffer.at<float>()
+                                                              : nullptr,
+                        light_array.m_directional_light_count, output.at<float>()),
+                    "vren::clustered_shading::shade");
+            }
+
         private:
             vren::context const* m_context;
         };
@@ -329,9 +381,29 @@ namespace vren
             m_shade.m_config = cfg;
         }
 
+        // the reference's signature (clustered_shading.hpp:173-181): shade resolves materials from the material
+        // buffer and the context's textures
+        void operator()(vren::stream stream, vren::uvec2 const& screen, vren::camera const& camera,
+                        vren::gbuffer const& gbuffer, vren::depth_buffer_t const& depth_buffer,
+                        vren::light_array const& light_array, vren::material_buffer const& material_buffer,
+                        vren::device_buffer const& output)
+        {
+            run(stream, screen, camera, gbuffer, depth_buffer, light_array, &material_buffer, output);
+        }
+
+        // per-pixel resolved materials (gbuffer.m_albedo_roughness_buffer, gbuffer.m_metallic_buffer)
         void operator()(vren::stream stream, vren::uvec2 const& screen, vren::camera const& camera,
                         vren::gbuffer const& gbuffer, vren::depth_buffer_t const& depth_buffer,
                         vren::light_array const& light_array, vren::device_buffer const& output)
+        {
+            run(stream, screen, camera, gbuffer, depth_buffer, light_array, nullptr, output);
+        }
+
+    private:
+        void run(vren::stream stream, vren::uvec2 const& screen, vren::camera const& camera,
+                 vren::gbuffer const& gbuffer, vren::depth_buffer_t const& depth_buffer,
+                 vren::light_array const& light_array, vren::material_buffer const* material_buffer,
+                 vren::device_buffer const& output)
         {
             if (screen.x > m_limits.m_max_screen_width || screen.y > m_limits.m_max_screen_height ||
                 light_array.m_point_light_count > m_limits.m_max_point_light_count)
@@ -348,11 +420,15 @@ namespace vren
             m_assign_lights(0, stream, rc, screen, camera, m_cluster_key_buffer, m_cluster_key_dispatch_params_buffer,
                             bvh, L ? vren::calc_bvh_root_index(L) : 0, L, codes, m_assigned_light_indices_buffer,
                             m_assigned_light_counts_buffer, m_assigned_light_offsets_buffer, view_pos);
-            m_shade(0, stream, rc, screen, camera, gbuffer, depth_buffer, refs, m_assigned_light_indices_buffer,
-                    m_assigned_light_counts_buffer, m_assigned_light_offsets_buffer, light_array, output);
+            if (material_buffer)
+                m_shade(0, stream, rc, screen, camera, gbuffer, depth_buffer, refs, m_assigned_light_indices_buffer,
+                        m_assigned_light_counts_buffer, m_assigned_light_offsets_buffer, light_array, *material_buffer,
+                        output);
+            else
+                m_shade(0, stream, rc, screen, camera, gbuffer, depth_buffer, refs, m_assigned_light_indices_buffer,
+                        m_assigned_light_counts_buffer, m_assigned_light_offsets_buffer, light_array, output);
         }
 
-    private:
         static vren::device_buffer sized(vren::device_buffer const& b, size_t size)
         {
             vren::device_buffer r;

@@ -265,6 +265,63 @@ VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint32_t width, 
                            const float* light_positions, const float* point_lights, uint32_t light_count,
                            const float* directional_lights, uint32_t directional_light_count, float* output);
 
+/* ---- material resolve (shade.comp:89-96) ---------------------------------------------------------------------
+ * The reference's G-buffer carries a texcoord (RG32F) and a material index (R16_UINT) per pixel (gbuffer.cpp:14-27);
+ * shade looks the material up in the material buffer (material.hpp:19-58) and samples its base-colour and
+ * metallic-roughness textures from the bindless texture array (texture_manager.hpp:32):
+ *   albedo = base_colour_texture(uv).rgb * base_color_factor.rgb
+ *   metallic = metallic_roughness_texture(uv).b * metallic_factor
+ *   roughness = metallic_roughness_texture(uv).g * roughness_factor
+ * Textures are RGBA8 UNORM with one mip level (tinygltf_parser.cpp:157); texel selection and filtering are those
+ * of include/vren_hip/sampler.h. A material index at or past material_count, or a texture index at or past
+ * texture_count, reads zeros (robust buffer access), as does a texture with a zero size or no texels. */
+enum {
+    VREN_FILTER_NEAREST = 0,
+    VREN_FILTER_LINEAR = 1,
+};
+enum {
+    VREN_ADDRESS_REPEAT = 0,
+    VREN_ADDRESS_MIRRORED_REPEAT = 1,
+    VREN_ADDRESS_CLAMP_TO_EDGE = 2,
+};
+
+typedef struct vren_material {   /* vren::material / GLSL Material (common.glsl:61-68): 32 bytes */
+    uint32_t base_color_texture_idx;
+    uint32_t metallic_roughness_texture_idx;
+    float metallic_factor;
+    float roughness_factor;
+    float base_color_factor[4];
+} vren_material;
+
+typedef struct vren_texture {    /* one vk_utils::texture and its sampler: 32 bytes */
+    const uint8_t* texels;       /* device, 4-byte aligned: height rows of width RGBA8 texels */
+    uint32_t width, height;
+    uint32_t filter;             /* VREN_FILTER_* (the magnification filter: compute shaders sample at lambda = 0) */
+    uint32_t address_u, address_v; /* VREN_ADDRESS_* */
+    uint32_t pad;
+} vren_texture;
+
+typedef struct vren_material_set {
+    const float* texcoords;           /* f32x2[W*H] (RG32F G-buffer texcoords) */
+    const uint16_t* material_indices; /* u16[W*H] (R16_UINT G-buffer material indices) */
+    const vren_material* materials;   /* device, material_count records (vren::material_buffer::m_buffer) */
+    uint32_t material_count;
+    uint32_t texture_count;
+    const vren_texture* textures;     /* device, texture_count descriptors (texture_manager::m_textures) */
+} vren_material_set;
+
+/* shade (clustered_shading.hpp:114-140) with the reference's material path: as vren_shade, with the per-pixel
+ * albedo, roughness and metallic resolved from `materials` (shade.comp:89-96) instead of read from buffers. */
+VREN_EXPORT int vren_shade_materials(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                     const vren_camera* camera, const vren_cluster_cfg* cfg, const float* depth,
+                                     const uint16_t* normals, const vren_material_set* materials,
+                                     const uint32_t* cluster_refs, uint32_t key_capacity,
+                                     const uint32_t* light_counts, const uint32_t* light_offsets,
+                                     const uint32_t* light_indices, uint32_t index_capacity,
+                                     const float* light_positions, const float* point_lights, uint32_t light_count,
+                                     const float* directional_lights, uint32_t directional_light_count,
+                                     float* output);
+
 /* shade_backward: adjoint of vren_shade with the light lists held fixed (no reference counterpart; the reference
  * renders forward only). Given dL/d(output) (RGBA32F, alpha ignored) it ADDS into
  *   grad_light_positions[L] (float4: d/dx, d/dy, d/dz, 0), grad_point_lights[L] (float4: d/d colour*intensity rgb,
@@ -303,6 +360,8 @@ typedef struct vren_frame_desc {
     float* output;                   /* f32x4[W*H] RGBA32F */
     uint32_t key_capacity;           /* 0: min(tiles * 1024, 2^20) */
     uint32_t index_capacity;         /* 0: min(key_capacity * L, 2^24) */
+    const vren_material_set* materials; /* NULL: albedo_roughness + metallic; else the material path (then those
+                                           two may be NULL). Read when the batch is created. */
 } vren_frame_desc;
 
 typedef struct vren_frame_outputs {
@@ -338,6 +397,41 @@ VREN_EXPORT int vren_batch_run(vren_batch* batch, vren_stream_t stream, int mode
 VREN_EXPORT int vren_batch_run_profiled(vren_batch* batch, vren_stream_t stream, float* kernel_ms);
 VREN_EXPORT int vren_batch_frame_outputs(vren_batch* batch, uint32_t frame, vren_frame_outputs* out);
 VREN_EXPORT void vren_batch_destroy(vren_batch* batch);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Depth-buffer pyramid (depth_buffer_pyramid.hpp:19-80, depth_buffer_pyramid.cpp:14-23, 177-299)
+ * The Hi-Z max pyramid of a D32 depth buffer: levels 0 .. n-1, n = floor(log2(max(W, H))) + 1, level l of
+ * max(W >> l, 1) x max(H >> l, 1) texels, packed one after another (row-major) per image. Level 0 is the depth
+ * buffer (depth_buffer_copy.comp); level l + 1 texel (x, y) is max(0, the level-l texels (2x + i, 2y + j) inside
+ * level l) with GLSL max (depth_buffer_reduce.comp: NaN and negative depths never win; an odd last row or column
+ * of a level is not read by the next). `images` depth buffers of W * H floats each, stacked, give `images`
+ * pyramids of vren_depth_pyramid_length(W, H) floats each.
+ * ------------------------------------------------------------------------------------------------------- */
+VREN_EXPORT uint32_t vren_depth_pyramid_level_count(uint32_t width, uint32_t height);
+VREN_EXPORT uint64_t vren_depth_pyramid_level_offset(uint32_t width, uint32_t height, uint32_t level);
+VREN_EXPORT uint64_t vren_depth_pyramid_length(uint32_t width, uint32_t height);
+VREN_EXPORT int vren_depth_pyramid(vren_ctx* ctx, vren_stream_t stream, const float* depth, uint32_t width,
+                                   uint32_t height, uint32_t images, float* pyramid);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Debug consumers of the demo (vren_demo): image and line dumps of the clustered-shading outputs
+ * ------------------------------------------------------------------------------------------------------- */
+/* show_clusters (clustered_shading_debug.cpp:52-100, show_clusters.comp:27-126): RGBA32F image of W*H pixels,
+ * pixel p shows the cluster cluster_refs[p] (reference key layout): mode 0 hash(key), 1 (i / tiles_x, j / tiles_y,
+ * k / 1024), 2 (face / 6, u / 3, v / 3) of the normal code, 3 min(count / 64, 1), 4 hash(offset), 5 hash(xor of
+ * the cluster's light indices), any other mode (1, 0, 0); alpha 1. hash(x) = float(triple32(x)) / float(2^32 - 1).
+ * A reference at or past key_capacity shows (1, 0, 0). */
+VREN_EXPORT int vren_show_clusters(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height, int32_t mode,
+                                   const uint32_t* cluster_refs, const uint32_t* cluster_keys, uint32_t key_capacity,
+                                   const uint32_t* light_counts, const uint32_t* light_offsets,
+                                   const uint32_t* light_indices, uint32_t index_capacity, float* output);
+/* visualize_bvh (visualize_bvh.cpp:21-90, show_bvh.comp): for every node of a 32-ary BVH of level_count + 1
+ * levels (leaves first, as build_bvh lays them out), 12 lines = 24 vertices {f32 x, y, z; u32 colour} (16 bytes,
+ * debug_renderer.hpp:25-30) at vertices[24 * node]: the node's AABB edges in show_bvh.comp's order, coloured by
+ * level (leaves 0x00ff00, then 0x0000ff, 0x00ffff, 0xff00ff, 0xffffff); invalid nodes give zero vertices of
+ * colour 0. level_count = vren_calc_bvh_level_count(lights) <= 5. */
+VREN_EXPORT int vren_visualize_bvh(vren_ctx* ctx, vren_stream_t stream, const vren_bvh_node* bvh, uint32_t level_count,
+                                   float* vertices);
 
 /* ---------------------------------------------------------------------------------------------------------
  * Synthetic inputs on the device (include/vren_hip/synth.h; no reference counterpart: the reference draws its

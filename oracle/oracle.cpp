@@ -33,15 +33,28 @@
  *   o_bounce_point_lights  closed forms: unobstructed motion p + speed dt d, clamping of outside positions, and
  *                        the unfolded billiard path (1-D and 2-D, several reflections) to within the 1e-5 wall
  *                        offsets; |d| per component preserved, final positions inside the AABB
+ *   o_blelloch_downsweep_u32  of an up-sweep tree: numpy exclusive cumsum (root kept when clear_last is false);
+ *                        on arbitrary input: the closed form P + sum over the index's set bits of the left
+ *                        sibling's last element
+ *   o_depth_pyramid      numpy 2x2 max over the in-bounds texels of each level, brute force on odd sizes
+ *   o_resolve_materials_f64, o_shade_f64_materials  hand-made textures with known texels for each filter and
+ *                        address mode; the material path against o_shade_f64 on the resolved values
+ *   o_show_clusters, o_visualize_bvh  closed forms of each view mode; the 24 vertices of a node are its AABB's
+ *                        12 edges
+ *   o_cpu_*              timing helpers of bench.py's CPU baseline (std::stable_sort / std::exclusive_scan /
+ *                        std::accumulate on 1 or N std::threads); their results are checked against the serial ones
  */
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <chrono>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "../include/vren_hip/capi.h"
+#include "../include/vren_hip/sampler.h"
 #include "../include/vren_hip/synth.h"
 #include "../include/vren_hip/vmath.h"
 
@@ -1306,7 +1319,8 @@ bool shade_pixel_f64(const frame_consts& c, const vren_camera* cam, const vren_c
                      uint32_t gy, const float* depth, const uint16_t* normals, const float* albedo_rough,
                      const float* metallic, const uint32_t* refs, uint32_t key_capacity, const uint32_t* counts,
                      const uint32_t* offsets, const uint32_t* indices, uint64_t index_capacity, const double* pos4,
-                     const double* light4, double out[3], double* albedo_override, double* rough_override)
+                     const double* light4, double out[3], double* albedo_override, double* rough_override,
+                     const double* metal_override = nullptr)
 {
     typedef double T;
     uint64_t p = (uint64_t)gy * c.W + gx;
@@ -1314,11 +1328,10 @@ bool shade_pixel_f64(const frame_consts& c, const vren_camera* cam, const vren_c
     if (!reconstruct<T>(c, gx, gy, depth[p], P)) return false;
     v3<T> N = mk((T)vren_half_to_float(normals[4 * p]), (T)vren_half_to_float(normals[4 * p + 1]),
                  (T)vren_half_to_float(normals[4 * p + 2]));
-    v3<T> albedo = mk((T)albedo_rough[4 * p], (T)albedo_rough[4 * p + 1], (T)albedo_rough[4 * p + 2]);
-    T rough = (T)albedo_rough[4 * p + 3];
-    if (albedo_override) albedo = mk(albedo_override[0], albedo_override[1], albedo_override[2]);
-    if (rough_override) rough = *rough_override;
-    T metal = (T)metallic[p];
+    v3<T> albedo = albedo_override ? mk(albedo_override[0], albedo_override[1], albedo_override[2])
+                                   : mk((T)albedo_rough[4 * p], (T)albedo_rough[4 * p + 1], (T)albedo_rough[4 * p + 2]);
+    T rough = rough_override ? *rough_override : (T)albedo_rough[4 * p + 3];
+    T metal = metal_override ? *metal_override : (T)metallic[p];
     v3<T> eye = mk((T)cam->position[0], (T)cam->position[1], (T)cam->position[2]);
     v3<T> V = normalize(eye - P);
     v3<T> Lo = mk(T(0), T(0), T(0));
@@ -1375,6 +1388,224 @@ OEXPORT void o_shade_f64_pixels(uint32_t W, uint32_t H, const vren_camera* cam, 
                                     p4.data(), l4.data(), out3 + 3 * i, nullptr, nullptr)
                         ? 1
                         : 0;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Depth-buffer pyramid (depth_buffer_pyramid.cpp:14-23, 177-299; depth_buffer_copy.comp;
+ * depth_buffer_reduce.comp), level by level as the reference dispatches it: level 0 = the depth buffer; level l + 1
+ * (max(W >> (l+1), 1) x max(H >> (l+1), 1)) texel (x, y) = GLSL max over the in-bounds level-l texels
+ * (2x + i, 2y + j), i outer, j inner, starting from 0.0. Levels are packed one after another per image.
+ * ---------------------------------------------------------------------------------------------------------- */
+
+OEXPORT uint32_t o_depth_pyramid_level_count(uint32_t W, uint32_t H)
+{
+    return ilog2(std::max(W, H)) + 1;
+}
+
+OEXPORT void o_depth_pyramid(const float* depth, uint32_t W, uint32_t H, float* out)
+{
+    const uint32_t levels = o_depth_pyramid_level_count(W, H);
+    std::memcpy(out, depth, sizeof(float) * (size_t)W * H);
+    float* prev = out;
+    uint32_t pw = W, ph = H;
+    for (uint32_t l = 1; l < levels; l++) {
+        const uint32_t w = std::max(W >> l, 1u), h = std::max(H >> l, 1u);
+        float* cur = prev + (size_t)pw * ph;
+        for (uint32_t y = 0; y < h; y++)
+            for (uint32_t x = 0; x < w; x++) {
+                float m = 0.0f;
+                for (uint32_t i = 0; i < 2; i++)
+                    for (uint32_t j = 0; j < 2; j++) {
+                        uint32_t fx = 2 * x + i, fy = 2 * y + j;
+                        if (fx < pw && fy < ph) m = glsl_max(m, prev[(size_t)fy * pw + fx]);
+                    }
+                cur[(size_t)y * w + x] = m;
+            }
+        prev = cur;
+        pw = w;
+        ph = h;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Debug consumers: show_clusters.comp:27-126 (per-pixel cluster views) and show_bvh.comp (node boxes as lines)
+ * ---------------------------------------------------------------------------------------------------------- */
+
+static uint32_t triple32(uint32_t x)
+{
+    x ^= x >> 17;
+    x *= 0xed5ad4bbu;
+    x ^= x >> 11;
+    x *= 0xac4c1b51u;
+    x ^= x >> 15;
+    x *= 0x31848babu;
+    x ^= x >> 14;
+    return x;
+}
+
+static float hash01(uint32_t x) { return (float)triple32(x) / (float)0xffffffffu; }
+
+OEXPORT uint32_t o_triple32(uint32_t x) { return triple32(x); }
+
+OEXPORT void o_show_clusters(uint32_t W, uint32_t H, int32_t mode, const uint32_t* refs, const uint32_t* keys,
+                             uint32_t key_cap, const uint32_t* counts, const uint32_t* offsets,
+                             const uint32_t* indices, uint64_t index_cap, float* out)
+{
+    const uint32_t tx = (W + 31) / 32, ty = (H + 31) / 32;
+    for (uint64_t p = 0; p < (uint64_t)W * H; p++) {
+        float c[3] = {1.0f, 0.0f, 0.0f};
+        const uint32_t ci = refs[p];
+        if (ci < key_cap) {
+            const uint32_t key = keys[ci];
+            const uint32_t i = key & 0xFFu, j = (key >> 8) & 0xFFu, k = (key >> 16) & 0x3FFu, n = key >> 26;
+            if (mode == 0) {
+                c[0] = c[1] = c[2] = hash01(key);
+            } else if (mode == 1) {
+                c[0] = (float)i / (float)tx;
+                c[1] = (float)j / (float)ty;
+                c[2] = (float)k / 1024.0f;
+            } else if (mode == 2) {
+                c[0] = (float)(n / 9) / 6.0f;
+                c[1] = (float)((n / 3) % 3) / 3.0f;
+                c[2] = (float)(n % 3) / 3.0f;
+            } else if (mode == 3) {
+                c[0] = c[1] = c[2] = glsl_min((float)counts[ci] / 64.0f, 1.0f);
+            } else if (mode == 4) {
+                c[0] = c[1] = c[2] = hash01(offsets[ci]);
+            } else if (mode == 5) {
+                uint32_t h = 0;
+                for (uint32_t q = 0; q < counts[ci] && (uint64_t)offsets[ci] + q < index_cap; q++)
+                    h ^= indices[offsets[ci] + q];
+                c[0] = c[1] = c[2] = hash01(h);
+            }
+        }
+        out[4 * p] = c[0];
+        out[4 * p + 1] = c[1];
+        out[4 * p + 2] = c[2];
+        out[4 * p + 3] = 1.0f;
+    }
+}
+
+/* visualize_bvh.cpp:54-86 + show_bvh.comp: levels level_count (leaves) .. 0, 24 vertices {x, y, z, colour} per node */
+OEXPORT void o_visualize_bvh(const vren_bvh_node* bvh, uint32_t level_count, uint32_t* vertices)
+{
+    static const uint32_t colors[7] = {0xff0000u, 0xffff00u, 0x00ff00u, 0x0000ffu, 0x00ffffu, 0xff00ffu, 0xffffffu};
+    uint64_t node = 0;
+    for (int32_t level = (int32_t)level_count; level >= 0; level--) {
+        const uint64_t count = 1ull << (5 * level);
+        const uint32_t ci = level_count - (uint32_t)level + 2;
+        const uint32_t color = ci < 7 ? colors[ci] : 0xffffffu;
+        for (uint64_t q = 0; q < count; q++, node++) {
+            const vren_bvh_node& n = bvh[node];
+            float m[3] = {n.min[0], n.min[1], n.min[2]}, M[3] = {n.max[0], n.max[1], n.max[2]};
+            uint32_t col = color;
+            if (n.next == VREN_BVH_INVALID_NODE) {
+                m[0] = m[1] = m[2] = M[0] = M[1] = M[2] = 0.0f;
+                col = 0u;
+            }
+            /* VREN_WRITE_DEBUG_DRAW_BUFFER_AABB: bottom face loop, top face loop, then the four verticals */
+            const float v[24][3] = {
+                {m[0], m[1], m[2]}, {M[0], m[1], m[2]}, {M[0], m[1], m[2]}, {M[0], m[1], M[2]},
+                {M[0], m[1], M[2]}, {m[0], m[1], M[2]}, {m[0], m[1], M[2]}, {m[0], m[1], m[2]},
+                {m[0], M[1], m[2]}, {M[0], M[1], m[2]}, {M[0], M[1], m[2]}, {M[0], M[1], M[2]},
+                {M[0], M[1], M[2]}, {m[0], M[1], M[2]}, {m[0], M[1], M[2]}, {m[0], M[1], m[2]},
+                {m[0], m[1], m[2]}, {m[0], M[1], m[2]}, {M[0], m[1], m[2]}, {M[0], M[1], m[2]},
+                {M[0], m[1], M[2]}, {M[0], M[1], M[2]}, {m[0], m[1], M[2]}, {m[0], M[1], M[2]}};
+            for (int k = 0; k < 24; k++) {
+                uint32_t* o = vertices + 4 * (24 * node + k);
+                std::memcpy(o, v[k], 12);
+                o[3] = col;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------------
+ * Material resolve (shade.comp:89-96; material.hpp:19-26; tinygltf_parser.cpp:157 RGBA8 UNORM textures) in FP64:
+ * the texels are chosen by include/vren_hip/sampler.h (the float coordinate transformation, as the kernels do),
+ * their values c / 255 and the bilinear weights (1 - a)(1 - b), a (1 - b), (1 - a) b, a b are evaluated in double.
+ * Textures here hold host texel pointers. A material index past the table, a texture index past the table and an
+ * empty texture read zeros.
+ * ---------------------------------------------------------------------------------------------------------- */
+
+static void texel_f64(const vren_texture& t, uint32_t i, uint32_t j, double o[4])
+{
+    const uint8_t* c = t.texels + 4 * ((uint64_t)j * t.width + i);
+    for (int k = 0; k < 4; k++) o[k] = (double)c[k] / 255.0;
+}
+
+static void sample_texture_f64(const vren_texture* texs, uint32_t n_tex, uint32_t ti, float s, float t, double o[4])
+{
+    for (int k = 0; k < 4; k++) o[k] = 0.0;
+    if (ti >= n_tex) return;
+    const vren_texture& tx = texs[ti];
+    if (!tx.texels || tx.width == 0 || tx.height == 0) return;
+    uint32_t i0, i1, j0, j1;
+    float a, b;
+    vren_texel_axis(s, tx.width, tx.filter, tx.address_u, &i0, &i1, &a);
+    vren_texel_axis(t, tx.height, tx.filter, tx.address_v, &j0, &j1, &b);
+    double c00[4];
+    texel_f64(tx, i0, j0, c00);
+    if (tx.filter != VREN_FILTER_LINEAR) {
+        for (int k = 0; k < 4; k++) o[k] = c00[k];
+        return;
+    }
+    double c10[4], c01[4], c11[4];
+    texel_f64(tx, i1, j0, c10);
+    texel_f64(tx, i0, j1, c01);
+    texel_f64(tx, i1, j1, c11);
+    const double A = a, B = b;
+    for (int k = 0; k < 4; k++)
+        o[k] = (1.0 - A) * (1.0 - B) * c00[k] + A * (1.0 - B) * c10[k] + (1.0 - A) * B * c01[k] + A * B * c11[k];
+}
+
+/* albedo_rough4[4 n] (albedo rgb, roughness) and metal[n] of n pixels */
+OEXPORT void o_resolve_materials_f64(uint64_t n, const float* texcoords, const uint16_t* mat_idx,
+                                     const vren_material* mats, uint32_t n_mat, const vren_texture* texs,
+                                     uint32_t n_tex, double* albedo_rough4, double* metal)
+{
+    for (uint64_t p = 0; p < n; p++) {
+        double* ar = albedo_rough4 + 4 * p;
+        ar[0] = ar[1] = ar[2] = ar[3] = 0.0;
+        metal[p] = 0.0;
+        const uint32_t mi = mat_idx[p];
+        if (mi >= n_mat) continue;
+        const vren_material& m = mats[mi];
+        const float s = texcoords[2 * p], t = texcoords[2 * p + 1];
+        double base[4], mr[4];
+        sample_texture_f64(texs, n_tex, m.base_color_texture_idx, s, t, base);
+        sample_texture_f64(texs, n_tex, m.metallic_roughness_texture_idx, s, t, mr);
+        for (int k = 0; k < 3; k++) ar[k] = base[k] * (double)m.base_color_factor[k];
+        ar[3] = mr[1] * (double)m.roughness_factor;
+        metal[p] = mr[2] * (double)m.metallic_factor;
+    }
+}
+
+/* shade (FP64) with the material path: every pixel's albedo, roughness and metallic resolved in FP64 first */
+OEXPORT void o_shade_f64_materials(uint32_t W, uint32_t H, const vren_camera* cam, const vren_cluster_cfg* cfg,
+                                   const float* depth, const uint16_t* normals, const float* texcoords,
+                                   const uint16_t* mat_idx, const vren_material* mats, uint32_t n_mat,
+                                   const vren_texture* texs, uint32_t n_tex, const uint32_t* refs,
+                                   uint32_t key_capacity, const uint32_t* counts, const uint32_t* offsets,
+                                   const uint32_t* indices, uint64_t index_capacity, const float* pos4,
+                                   const float* light4, uint32_t L, double* out, uint8_t* shaded_mask)
+{
+    frame_consts c = make_consts(W, H, cam);
+    const uint64_t n = (uint64_t)W * H;
+    std::vector<double> ar(4 * n), me(n);
+    o_resolve_materials_f64(n, texcoords, mat_idx, mats, n_mat, texs, n_tex, ar.data(), me.data());
+    std::vector<double> p4(pos4, pos4 + 4 * (size_t)L), l4(light4, light4 + 4 * (size_t)L);
+    for (uint64_t p = 0; p < n; p++) {
+        double rgb[3];
+        double rough = ar[4 * p + 3];
+        /* no per-pixel material arrays: every value comes from the overrides */
+        bool sh = shade_pixel_f64(c, cam, cfg, (uint32_t)(p % W), (uint32_t)(p / W), depth, normals, nullptr, nullptr,
+                                  refs, key_capacity, counts, offsets, indices, index_capacity, p4.data(), l4.data(),
+                                  rgb, &ar[4 * p], &rough, &me[p]);
+        shaded_mask[p] = sh ? 1 : 0;
+        for (int k = 0; k < 3; k++) out[4 * p + k] = sh ? rgb[k] : NAN;
+        out[4 * p + 3] = sh ? 1.0 : NAN;
     }
 }
 
@@ -1607,4 +1838,85 @@ OEXPORT void o_bounce_point_lights(float* pos, float* dir, uint32_t n, const flo
         pos[4 * i + 3] = 1.0f;
         dir[4 * i + 3] = 0.0f;
     }
+}
+
+/* ------------------------------------------------------------------------------------------------------------
+ * CPU baseline helpers (bench.py): the reference tests' CPU operations (radix_sort.cpp:88 std::sort,
+ * blelloch_scan.cpp:100-116 std::exclusive_scan, reduce.cpp std::accumulate) on 1 or `threads` std::threads
+ * (no TBB): chunks in parallel, then a combine (pairwise std::inplace_merge levels for the sort, a prefix over
+ * chunk totals for the scan). Each returns its wall time in seconds (std::chrono::steady_clock), inputs in place.
+ * ---------------------------------------------------------------------------------------------------------- */
+
+namespace {
+
+template <typename F>
+void parallel_for(uint32_t threads, F f)
+{
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; t++) pool.emplace_back(f, t);
+    f(0u);
+    for (auto& th : pool) th.join();
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+} // namespace
+
+/* stable sort of (key, value) pairs by key: std::stable_sort per chunk, then log2(threads) levels of
+ * std::inplace_merge (stable) */
+OEXPORT double o_cpu_stable_sort_pairs(uint64_t* pairs, uint64_t n, uint32_t threads)
+{
+    auto by_key = [](uint64_t a, uint64_t b) { return (uint32_t)a < (uint32_t)b; };
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads <= 1) {
+        std::stable_sort(pairs, pairs + n, by_key);
+        return seconds_since(t0);
+    }
+    std::vector<uint64_t> bounds(threads + 1);
+    for (uint32_t t = 0; t <= threads; t++) bounds[t] = n * t / threads;
+    parallel_for(threads, [&](uint32_t t) { std::stable_sort(pairs + bounds[t], pairs + bounds[t + 1], by_key); });
+    for (uint32_t w = 1; w < threads; w *= 2) {
+        uint32_t merges = (threads + 2 * w - 1) / (2 * w);
+        parallel_for(merges, [&](uint32_t m) {
+            uint32_t a = 2 * w * m, b = std::min(a + w, threads), c = std::min(a + 2 * w, threads);
+            if (b < c) std::inplace_merge(pairs + bounds[a], pairs + bounds[b], pairs + bounds[c], by_key);
+        });
+    }
+    return seconds_since(t0);
+}
+
+/* exclusive scan mod 2^32: chunk totals, their prefix, then std::exclusive_scan per chunk from its prefix */
+OEXPORT double o_cpu_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t threads)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads <= 1) {
+        std::exclusive_scan(in, in + n, out, 0u);
+        return seconds_since(t0);
+    }
+    std::vector<uint64_t> bounds(threads + 1);
+    for (uint32_t t = 0; t <= threads; t++) bounds[t] = n * t / threads;
+    std::vector<uint32_t> tot(threads);
+    parallel_for(threads, [&](uint32_t t) { tot[t] = std::accumulate(in + bounds[t], in + bounds[t + 1], 0u); });
+    std::vector<uint32_t> pre(threads);
+    std::exclusive_scan(tot.begin(), tot.end(), pre.begin(), 0u);
+    parallel_for(threads, [&](uint32_t t) {
+        std::exclusive_scan(in + bounds[t], in + bounds[t + 1], out + bounds[t], pre[t]);
+    });
+    return seconds_since(t0);
+}
+
+/* sum mod 2^32 (std::accumulate per chunk, then over the chunk sums) */
+OEXPORT double o_cpu_reduce_add(const uint32_t* in, uint64_t n, uint32_t threads, uint32_t* result)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    threads = std::max(threads, 1u);
+    std::vector<uint64_t> bounds(threads + 1);
+    for (uint32_t t = 0; t <= threads; t++) bounds[t] = n * t / threads;
+    std::vector<uint32_t> tot(threads);
+    parallel_for(threads, [&](uint32_t t) { tot[t] = std::accumulate(in + bounds[t], in + bounds[t + 1], 0u); });
+    *result = std::accumulate(tot.begin(), tot.end(), 0u);
+    return seconds_since(t0);
 }

@@ -17,7 +17,9 @@
 #include "primitives/radix_sort.hpp"
 #include "primitives/reduce.hpp"
 #include "point_light_bouncer.hpp"
+#include "debug_consumers.hpp"
 #include "pipeline/clustered_shading.hpp"
+#include "pipeline/depth_buffer_pyramid.hpp"
 #include "../../include/vren_hip/synth.h"
 
 static int g_failures = 0;
@@ -410,6 +412,80 @@ static void clustered(vren::context& ctx)
     for (size_t i = 0; i < a.size(); i += 4) lit += a[i] != 0;
     EXPECT(lit > (size_t) W * H / 4, "shaded pixels %zu", lit);
     vren_batch_destroy(batch);
+
+    // the reference's material path (cluster_and_shade with a material_buffer, shade.comp:89-96): one material over
+    // 1x1 textures of 255s gives albedo, roughness and metallic equal to its factors exactly (255 / 255 = 1), so
+    // the output must equal the resolved-material path's over G-buffer planes filled with those factors
+    {
+        const float fac[4] = {0.25f, 0.5f, 0.75f, 0.6f}, metal_f = 0.3f;
+        std::vector<float> ar((size_t) W * H * 4), me((size_t) W * H, metal_f);
+        for (size_t i = 0; i < (size_t) W * H; i++)
+            for (int c = 0; c < 4; c++) ar[4 * i + c] = fac[c];
+        auto d_ar = upload(ar);
+        auto d_me = upload(me);
+        std::vector<uint8_t> white(4, 255);
+        vren::texture t;
+        auto d_white = upload(white);
+        t.m_texels = d_white;
+        t.m_width = t.m_height = 1;
+        ctx.m_texture_manager.m_textures = {t, t};
+        ctx.m_texture_manager.rewrite_descriptor_set();
+        vren::material m{0u, 1u, metal_f, fac[3], vren::vec4{fac[0], fac[1], fac[2], 1.0f}};
+        vren::material_buffer mb(ctx, 4);
+        mb.write(&m, 1);
+        std::vector<float> tc((size_t) W * H * 2, 0.5f);
+        std::vector<uint16_t> mi((size_t) W * H, 0);
+        auto d_tc = upload(tc);
+        auto d_mi = upload(mi);
+        vren::gbuffer gm;
+        gm.m_normal_buffer = normals;
+        gm.m_texcoord_buffer = d_tc;
+        gm.m_material_index_buffer = d_mi;
+        vren::gbuffer gr;
+        gr.m_normal_buffer = normals;
+        gr.m_albedo_roughness_buffer = d_ar;
+        gr.m_metallic_buffer = d_me;
+        cas(s, {W, H}, cam, gm, db, la, mb, out_a);
+        cas(s, {W, H}, cam, gr, db, la, out_b);
+        ctx.sync(s);
+        EXPECT(download<uint32_t>(out_a, (size_t) W * H * 4) == download<uint32_t>(out_b, (size_t) W * H * 4),
+               "material path == resolved path for constant materials");
+    }
+
+    // depth_buffer_reductor: level 0 is the depth copy; the last level is the max over the image's covered texels
+    {
+        vren::depth_buffer_pyramid pyr(ctx, W, H);
+        vren::depth_buffer_reductor red(ctx);
+        red.copy_and_reduce(s, db, pyr);
+        ctx.sync(s);
+        auto d = download<float>(depth, (size_t) W * H);
+        auto p = download<float>(pyr.buffer(), (size_t) (pyr.buffer().m_size / 4));
+        EXPECT(std::equal(d.begin(), d.end(), p.begin()), "pyramid level 0 == depth");
+        const uint32_t last = pyr.get_level_count() - 1;
+        EXPECT(pyr.get_image_width(last) == 1 && pyr.get_image_height(last) == 1, "pyramid top is 1x1");
+        // level 1 texel (1, 2) = max(0, depth texels (2..3, 4..5)) (depth_buffer_reduce.comp)
+        float m = 0.0f;
+        for (uint32_t i = 0; i < 2; i++)
+            for (uint32_t j = 0; j < 2; j++) {
+                const float v = d[(size_t) (4 + j) * W + 2 + i];
+                m = m < v ? v : m;
+            }
+        EXPECT(p[pyr.level_offset(1) + 2 * pyr.get_image_width(1) + 1] == m, "pyramid level 1 texel");
+    }
+
+    // show_clusters / visualize_bvh over the cluster_and_shade outputs
+    {
+        vren_demo::show_clusters sc(ctx);
+        sc(s, {W, H}, vren_demo::show_clusters::k_light_assignment_counts, cas, out_a);
+        vren_demo::visualize_bvh vb(ctx);
+        const uint32_t levels = vren::calc_bvh_level_count(L);
+        vren::owned_device_buffer lines(vren_demo::visualize_bvh::get_required_draw_buffer_size(levels));
+        vren::resource_container rc2;
+        vb.write(s, rc2, cas.m_point_light_bvh_buffer, levels, lines);
+        ctx.sync(s);
+        auto v = download<float>(lines, 8);
+        EXPECT(v[0] <= v[4] && v[1] == v[5] && v[2] == v[6], "first edge of the first leaf runs along x");
+    }
 
     // a camera with near >= far is rejected like the reference's assert
     vren::camera bad = cam;

@@ -45,6 +45,11 @@ class Oracle:
         L.o_find_unique_clusters.restype = c_u32
         L.o_assign_lights.restype = c_u64
         L.o_assign_lights_brute.restype = c_u64
+        L.o_depth_pyramid_level_count.restype = c_u32
+        L.o_triple32.restype = c_u32
+        L.o_triple32.argtypes = [c_u32]
+        for n in ("o_cpu_stable_sort_pairs", "o_cpu_exclusive_scan", "o_cpu_reduce_add"):
+            getattr(L, n).restype = ctypes.c_double
 
     # rng ---------------------------------------------------------------------------------------------------
     def pcg32(self, initstate, initseq, n, start=0, jump=False):
@@ -122,6 +127,24 @@ class Oracle:
         lo3, hi3 = (ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi)
         self.lib.o_bounce_point_lights(_p(pos), _p(dirs), c_u32(pos.shape[0]), lo3, hi3, c_f(speed), c_f(dt))
         return pos, dirs
+
+    # CPU baseline helpers (wall seconds; results in place)
+    def cpu_stable_sort_pairs(self, keys, vals, threads):
+        pairs = (np.asarray(vals, np.uint64) << np.uint64(32)) | np.asarray(keys, np.uint64)
+        t = self.lib.o_cpu_stable_sort_pairs(_p(pairs), c_u64(pairs.size), c_u32(threads))
+        return t, (pairs & np.uint64(0xFFFFFFFF)).astype(np.uint32), (pairs >> np.uint64(32)).astype(np.uint32)
+
+    def cpu_exclusive_scan(self, x, threads):
+        x = np.ascontiguousarray(x, np.uint32)
+        out = np.zeros_like(x)
+        t = self.lib.o_cpu_exclusive_scan(_p(x), _p(out), c_u64(x.size), c_u32(threads))
+        return t, out
+
+    def cpu_reduce_add(self, x, threads):
+        x = np.ascontiguousarray(x, np.uint32)
+        r = ctypes.c_uint32()
+        t = self.lib.o_cpu_reduce_add(_p(x), c_u64(x.size), c_u32(threads), ctypes.byref(r))
+        return t, r.value
 
     def calc(self, name, n):
         return getattr(self.lib, "o_calc_bvh_" + name)(c_u32(n))
@@ -289,6 +312,70 @@ class ClusteredOracle:
             c_u32(key_cap), _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(pos), _p(lig), c_u32(pos.shape[0]),
             _p(pixels), c_u64(pixels.size), _p(out), _p(sh))
         return out, sh.astype(bool)
+
+    # material path: `materials` is a ctypes array of vren_amd.Material, `textures` a list of (rgba8 uint8 array of
+    # shape (h, w, 4), filter, address_u, address_v)
+    @staticmethod
+    def texture_table(textures):
+        from vren_amd._lib import Texture
+        keep = []
+        arr = (Texture * max(len(textures), 1))()
+        for i, (texels, filt, au, av) in enumerate(textures):
+            t = np.ascontiguousarray(texels, np.uint8)
+            keep.append(t)
+            arr[i].texels = t.ctypes.data if t.size else None
+            arr[i].height, arr[i].width = (t.shape[0], t.shape[1]) if t.size else (0, 0)
+            arr[i].filter, arr[i].address_u, arr[i].address_v = filt, au, av
+        return arr, keep
+
+    def resolve_materials(self, texcoords, mat_idx, materials, textures):
+        texcoords = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+        mat_idx = np.ascontiguousarray(mat_idx, np.uint16)
+        n = mat_idx.size
+        tab, keep = self.texture_table(textures)
+        ar = np.zeros((n, 4), np.float64)
+        me = np.zeros(n, np.float64)
+        self.lib.o_resolve_materials_f64(c_u64(n), _p(texcoords), _p(mat_idx), ctypes.byref(materials),
+                                         c_u32(len(materials)), tab, c_u32(len(textures)), _p(ar), _p(me))
+        return ar, me
+
+    def shade_materials(self, W, H, cam, cf, depth, normals, texcoords, mat_idx, materials, textures, refs, key_cap,
+                        counts, offsets, idx, pos, lig):
+        texcoords = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+        mat_idx = np.ascontiguousarray(mat_idx, np.uint16)
+        tab, keep = self.texture_table(textures)
+        out = np.full((W * H, 4), np.nan, np.float64)
+        mask = np.zeros(W * H, np.uint8)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        lig = np.ascontiguousarray(lig, np.float32).reshape(-1, 4)
+        self.lib.o_shade_f64_materials(
+            c_u32(W), c_u32(H), _bref(cam), _bref(cf), _p(depth), _p(normals), _p(texcoords), _p(mat_idx),
+            ctypes.byref(materials), c_u32(len(materials)), tab, c_u32(len(textures)), _p(refs), c_u32(key_cap),
+            _p(counts), _p(offsets), _p(idx), c_u64(idx.size), _p(pos), _p(lig), c_u32(pos.shape[0]), _p(out),
+            _p(mask))
+        return out, mask.astype(bool)
+
+    def depth_pyramid(self, depth, W, H):
+        n = self.lib.o_depth_pyramid_level_count(c_u32(W), c_u32(H))
+        total = sum(max(W >> l, 1) * max(H >> l, 1) for l in range(n))
+        out = np.zeros(total, np.float32)
+        self.lib.o_depth_pyramid(_p(np.ascontiguousarray(depth, np.float32)), c_u32(W), c_u32(H), _p(out))
+        return out, n
+
+    def show_clusters(self, W, H, mode, refs, keys, counts, offsets, idx):
+        out = np.zeros((W * H, 4), np.float32)
+        keys = np.ascontiguousarray(keys, np.uint32)
+        self.lib.o_show_clusters(c_u32(W), c_u32(H), ctypes.c_int32(mode), _p(np.ascontiguousarray(refs, np.uint32)),
+                                 _p(keys), c_u32(keys.size), _p(np.ascontiguousarray(counts, np.uint32)),
+                                 _p(np.ascontiguousarray(offsets, np.uint32)), _p(np.ascontiguousarray(idx, np.uint32)),
+                                 c_u64(idx.size), _p(out))
+        return out
+
+    def visualize_bvh(self, bvh, level_count):
+        n = sum(32 ** l for l in range(level_count + 1))
+        out = np.zeros((n * 24, 4), np.uint32)
+        self.lib.o_visualize_bvh(_p(np.ascontiguousarray(bvh)), c_u32(level_count), _p(out))
+        return out
 
     def shade_loss(self, W, H, cam, cf, depth, normals, ar, metal, refs, key_cap, counts, offsets, idx, pos_d, lig_d,
                    target, pixels, pixel_ar=None):

@@ -8,14 +8,17 @@ streams only). Every call goes to the library; there is no other implementation 
 import ctypes
 
 from . import _lib
-from ._lib import (ATTEN_REFERENCE, ATTEN_SMOOTH, BVH_INVALID_NODE, BVH_LEAF_NODE, CLUSTER_AABB_REFERENCE,
-                   CLUSTER_AABB_TIGHT, KEYS_REFERENCE, KEYS_SLICED, MORTON_GIVEN_AABB, MORTON_LIGHTS_AABB,
-                   REDUCE_ADD, REDUCE_MAX, REDUCE_MIN, Camera, ClusterCfg, FrameDesc, FrameOutputs, VrenError)
+from ._lib import (ADDRESS_CLAMP_TO_EDGE, ADDRESS_MIRRORED_REPEAT, ADDRESS_REPEAT, ATTEN_REFERENCE, ATTEN_SMOOTH,
+                   BVH_INVALID_NODE, BVH_LEAF_NODE, CLUSTER_AABB_REFERENCE, CLUSTER_AABB_TIGHT, FILTER_LINEAR,
+                   FILTER_NEAREST, KEYS_REFERENCE, KEYS_SLICED, MORTON_GIVEN_AABB, MORTON_LIGHTS_AABB, REDUCE_ADD,
+                   REDUCE_MAX, REDUCE_MIN, Camera, ClusterCfg, FrameDesc, FrameOutputs, Material, MaterialSet,
+                   Texture, VrenError)
 
 __all__ = ["Context", "Camera", "ClusterCfg", "FrameDesc", "FrameOutputs", "VrenError", "lib", "default_cfg",
            "REDUCE_ADD", "REDUCE_MIN", "REDUCE_MAX", "KEYS_REFERENCE", "KEYS_SLICED", "CLUSTER_AABB_REFERENCE",
            "CLUSTER_AABB_TIGHT", "MORTON_LIGHTS_AABB", "MORTON_GIVEN_AABB", "ATTEN_REFERENCE", "ATTEN_SMOOTH",
-           "BVH_LEAF_NODE", "BVH_INVALID_NODE"]
+           "BVH_LEAF_NODE", "BVH_INVALID_NODE", "Material", "MaterialSet", "Texture", "FILTER_NEAREST",
+           "FILTER_LINEAR", "ADDRESS_REPEAT", "ADDRESS_MIRRORED_REPEAT", "ADDRESS_CLAMP_TO_EDGE"]
 
 
 def lib():
@@ -177,6 +180,28 @@ class Context:
             index_cap, _ptr(light_pos), _ptr(lights), L, _ptr(grad_out), _ptr(g_pos), _ptr(g_lights),
             _ptr(g_albedo_rough)))
 
+    def shade_materials(self, W, H, cam, cfg, depth, normals, material_set, refs, key_cap, counts, offsets, indices,
+                        index_cap, light_pos, lights, L, out, dir_lights=None, n_dir=0):
+        """`material_set` is a MaterialSet whose pointers are device addresses (texcoords, material indices, the
+        material table and the texture descriptor table)."""
+        return self.check(self.lib.vren_shade_materials(
+            self.h, self.stream(), W, H, ctypes.byref(cam), ctypes.byref(cfg), _ptr(depth), _ptr(normals),
+            ctypes.byref(material_set), _ptr(refs), key_cap, _ptr(counts), _ptr(offsets), _ptr(indices), index_cap,
+            _ptr(light_pos), _ptr(lights), L, _ptr(dir_lights), n_dir, _ptr(out)))
+
+    # ---- depth pyramid and debug views ---------------------------------------------------------------------
+    def depth_pyramid(self, depth, W, H, images, pyramid):
+        return self.check(self.lib.vren_depth_pyramid(self.h, self.stream(), _ptr(depth), W, H, images,
+                                                      _ptr(pyramid)))
+
+    def show_clusters(self, W, H, mode, refs, keys, key_cap, counts, offsets, indices, index_cap, out):
+        return self.check(self.lib.vren_show_clusters(self.h, self.stream(), W, H, mode, _ptr(refs), _ptr(keys),
+                                                      key_cap, _ptr(counts), _ptr(offsets), _ptr(indices), index_cap,
+                                                      _ptr(out)))
+
+    def visualize_bvh(self, bvh, level_count, vertices):
+        return self.check(self.lib.vren_visualize_bvh(self.h, self.stream(), _ptr(bvh), level_count, _ptr(vertices)))
+
     # ---- synthetic inputs ----------------------------------------------------------------------------------
     def synth_keys(self, initstate, initseq, n, out):
         return self.check(self.lib.vren_synth_keys(self.h, self.stream(), initstate, initseq, n, _ptr(out)))
@@ -200,6 +225,8 @@ class Batch:
     PER_FRAME, BATCHED, GRAPH = 0, 1, 2
 
     def __init__(self, ctx, frames, cfg):
+        """A frame on the material path points its `materials` field at a MaterialSet, which must stay alive until
+        the batch has been created."""
         self.ctx = ctx
         arr = (FrameDesc * len(frames))(*frames)
         self._frames = arr

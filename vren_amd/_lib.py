@@ -33,12 +33,31 @@ class ClusterCfg(ctypes.Structure):
                 ("morton_aabb_max", c_float * 3), ("attenuation", c_uint32)]
 
 
+FILTER_NEAREST, FILTER_LINEAR = 0, 1
+ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP_TO_EDGE = 0, 1, 2
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("base_color_texture_idx", c_uint32), ("metallic_roughness_texture_idx", c_uint32),
+                ("metallic_factor", c_float), ("roughness_factor", c_float), ("base_color_factor", c_float * 4)]
+
+
+class Texture(ctypes.Structure):
+    _fields_ = [("texels", c_void_p), ("width", c_uint32), ("height", c_uint32), ("filter", c_uint32),
+                ("address_u", c_uint32), ("address_v", c_uint32), ("pad", c_uint32)]
+
+
+class MaterialSet(ctypes.Structure):
+    _fields_ = [("texcoords", c_void_p), ("material_indices", c_void_p), ("materials", c_void_p),
+                ("material_count", c_uint32), ("texture_count", c_uint32), ("textures", c_void_p)]
+
+
 class FrameDesc(ctypes.Structure):
     _fields_ = [("width", c_uint32), ("height", c_uint32), ("camera", Camera),
                 ("depth", c_void_p), ("normals", c_void_p), ("albedo_roughness", c_void_p), ("metallic", c_void_p),
                 ("light_positions", c_void_p), ("point_lights", c_void_p), ("light_count", c_uint32),
                 ("directional_lights", c_void_p), ("directional_light_count", c_uint32), ("output", c_void_p),
-                ("key_capacity", c_uint32), ("index_capacity", c_uint32)]
+                ("key_capacity", c_uint32), ("index_capacity", c_uint32), ("materials", P(MaterialSet))]
 
 
 class FrameOutputs(ctypes.Structure):
@@ -93,6 +112,16 @@ PROTOTYPES = {
     "vren_shade": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, P(Camera), P(ClusterCfg), c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p,
                            c_void_p, c_uint32, c_void_p, c_uint32, c_void_p]),
+    "vren_shade_materials": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, P(Camera), P(ClusterCfg), c_void_p,
+                                     c_void_p, P(MaterialSet), c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,
+                                     c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p]),
+    "vren_depth_pyramid_level_count": (c_uint32, [c_uint32, c_uint32]),
+    "vren_depth_pyramid_level_offset": (c_uint64, [c_uint32, c_uint32, c_uint32]),
+    "vren_depth_pyramid_length": (c_uint64, [c_uint32, c_uint32]),
+    "vren_depth_pyramid": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p]),
+    "vren_show_clusters": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, ctypes.c_int32, c_void_p, c_void_p,
+                                   c_uint32, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
+    "vren_visualize_bvh": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p]),
     "vren_shade_backward": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, P(Camera), P(ClusterCfg), c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,
                                     c_uint32, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,

@@ -32,7 +32,9 @@ def sources():
 
 def _compile(src, verbose):
     obj = os.path.join(OBJ, src[:-4] + ".o")
-    fp = ["-ffp-contract=fast"] if src in CONTRACT_OK else ["-ffp-contract=off"]
+    # fast-honor-pragmas: fused multiply-adds everywhere except in the `#pragma clang fp contract(off)` regions
+    # (plain -ffp-contract=fast lets the backend fuse across those regions as well)
+    fp = ["-ffp-contract=fast-honor-pragmas"] if src in CONTRACT_OK else ["-ffp-contract=off"]
     cmd = [HIPCC] + COMMON + fp + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)

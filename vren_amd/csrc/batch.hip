@@ -71,8 +71,12 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         if (st) return st;
         if (d.width == 0 || d.height == 0 || d.width > 8192 || d.height > 8192)
             return vren_fail(ctx, VREN_EINVAL, "batch: frame %u screen must be in [1, 8192]^2", i);
-        if (!d.depth || !d.normals || !d.albedo_roughness || !d.metallic || !d.output)
+        if (!d.depth || !d.normals || (!d.materials && (!d.albedo_roughness || !d.metallic)) || !d.output)
             return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has a null G-buffer or output", i);
+        if (d.materials) {
+            st = vren_validate_material_set(ctx, d.materials, "batch");
+            if (st) return st;
+        }
         if (d.light_count > 0 && (!d.light_positions || !d.point_lights))
             return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has null light buffers", i);
         if (d.light_count > (1u << 20)) return vren_fail(ctx, VREN_EINVAL, "batch: frame %u has > 2^20 lights", i);
@@ -87,7 +91,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->leaf_slots.resize(n_frames);
     b->live_slots.resize(n_frames);
     // sizes and arena layout
-    uint32_t tile_base = 0, leaf_base = 0, live_base = 0, offs_base = 0, max_L = 0;
+    uint32_t tile_base = 0, leaf_base = 0, live_base = 0, offs_base = 0, max_L = 0, any_materials = 0;
     std::vector<size_t> off(n_frames * 14);
     size_t arena = 0;
     for (uint32_t i = 0; i < n_frames; i++) {
@@ -114,6 +118,10 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         f.normals = d.normals;
         f.albedo_rough = d.albedo_roughness;
         f.metallic = d.metallic;
+        if (d.materials) {
+            vren_frame_set_materials(&f, d.materials);
+            any_materials = 1;
+        }
         f.out = d.output;
         f.key_cap = key_cap;
         f.index_cap = index_cap;
@@ -190,6 +198,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->w.total_leaf_slots = leaf_base;
     b->total_live_slots = live_base;
     b->w.max_L = max_L;
+    b->w.any_materials = any_materials;
     // per-frame tables: same buffers, bases 0, the workspace viewed from its start
     std::vector<FrameDev> single(b->host);
     b->w_frame.resize(n_frames);
@@ -203,6 +212,7 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
                         1);
         wf.total_leaf_slots = b->leaf_slots[i];
         wf.max_L = single[i].L;
+        wf.any_materials = single[i].texcoords != nullptr;
     }
     hipStream_t s = nullptr;
     st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);

@@ -7,6 +7,8 @@
 int vren_validate_cfg(vren_ctx* ctx, const vren_cluster_cfg* cfg);
 int vren_validate_camera(vren_ctx* ctx, const vren_camera* cam);
 int vren_upload_frames(vren_ctx* ctx, hipStream_t s, FrameDev* dst, const FrameDev* src, uint32_t n);
+int vren_validate_material_set(vren_ctx* ctx, const vren_material_set* m, const char* what);
+void vren_frame_set_materials(FrameDev* f, const vren_material_set* m);
 FrameDev* vren_next_frame_slot(vren_ctx* ctx);
 
 #define VREN_LEAF_ALIGN 256u
@@ -28,6 +30,7 @@ struct ChainWork {
     uint32_t total_offs_tiles;
     uint32_t total_leaf_slots;
     uint32_t max_L;
+    uint32_t any_materials; // some frame of the table takes the material path (shade picks its kernel by it)
     // optional per-kernel timing: an event is recorded after every kernel of the chain
     hipEvent_t* events = nullptr;
     int* n_events = nullptr;

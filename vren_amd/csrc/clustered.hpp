@@ -51,6 +51,12 @@ struct FrameDev {
     const uint16_t* normals;
     const float* albedo_rough;
     const float* metallic;
+    // material path (shade.comp:89-96): texcoords != nullptr selects it over albedo_rough / metallic
+    const float* texcoords;          // float2[W*H]
+    const uint16_t* mat_idx;         // u16[W*H]
+    const vren_material* materials;
+    const vren_texture* textures;
+    uint32_t n_materials, n_textures;
     // clusters
     uint32_t* keys;
     uint32_t* cluster_count;  // device u32

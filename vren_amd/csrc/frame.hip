@@ -111,6 +111,29 @@ int vren_validate_camera(vren_ctx* ctx, const vren_camera* cam)
     return VREN_OK;
 }
 
+int vren_validate_material_set(vren_ctx* ctx, const vren_material_set* m, const char* what)
+{
+    if (!m->texcoords || !m->material_indices)
+        return vren_fail(ctx, VREN_EINVAL, "%s: the material path needs texcoords and material indices", what);
+    if ((m->material_count > 0 && !m->materials) || (m->texture_count > 0 && !m->textures))
+        return vren_fail(ctx, VREN_EINVAL, "%s: null material or texture table with a non-zero count", what);
+    if ((reinterpret_cast<uintptr_t>(m->texcoords) & 7) || (reinterpret_cast<uintptr_t>(m->material_indices) & 1) ||
+        (reinterpret_cast<uintptr_t>(m->materials) & 15) || (reinterpret_cast<uintptr_t>(m->textures) & 15))
+        return vren_fail(ctx, VREN_EINVAL, "%s: texcoords must be 8-byte, material indices 2-byte, material and "
+                                           "texture tables 16-byte aligned", what);
+    return VREN_OK;
+}
+
+void vren_frame_set_materials(FrameDev* f, const vren_material_set* m)
+{
+    f->texcoords = m->texcoords;
+    f->mat_idx = m->material_indices;
+    f->materials = m->materials;
+    f->n_materials = m->materials ? m->material_count : 0;
+    f->textures = m->textures;
+    f->n_textures = m->textures ? m->texture_count : 0;
+}
+
 int vren_upload_frames(vren_ctx* ctx, hipStream_t s, FrameDev* dst, const FrameDev* src, uint32_t n)
 {
     for (uint32_t i = 0; i < n; i++) {

@@ -14,6 +14,7 @@
 //   - contributions that are exactly zero in the reference formula (attenuation 0 or n.l <= 0) are skipped.
 // The background test (view depth >= far - 0.001) uses the double reconstruction and is exactly the oracle's.
 #include "chain.hpp"
+#include "../../include/vren_hip/sampler.h"
 
 namespace {
 
@@ -41,7 +42,6 @@ struct PixelIn {
 
 __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, uint32_t gy, float depth)
 {
-#pragma clang fp contract(off)
     PixelIn o;
     // view depth and the background test exactly as the oracle evaluates them (shade.comp:75-86): the projection's
     // inverse has zero x/y coefficients in its z and w rows, so r[2] and r[3] do not depend on the pixel's x/y
@@ -53,18 +53,24 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
     for (int i = 0; i < 2; i++)
         r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) + fd.inv_proj_d[2 * 4 + i] * v[2]) +
                fd.inv_proj_d[3 * 4 + i] * v[3];
-    if (fd.proj_zw_xy_zero) {
-        // ((0 x + 0 y) + m2 depth) + m3 is m2 depth + m3 exactly (up to the sign of a zero)
-        r[2] = fd.inv_proj_d[2 * 4 + 2] * v[2] + fd.inv_proj_d[3 * 4 + 2];
-        r[3] = fd.inv_proj_d[2 * 4 + 3] * v[2] + fd.inv_proj_d[3 * 4 + 3];
-    } else {
-        for (int i = 2; i < 4; i++)
-            r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
-                    fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
+    double w, z;
+    {
+        // the view depth and the background test round exactly as the oracle's (no fused multiply-add here; this
+        // file is built with -ffp-contract=fast-honor-pragmas)
+#pragma clang fp contract(off)
+        if (fd.proj_zw_xy_zero) {
+            // ((0 x + 0 y) + m2 depth) + m3 is m2 depth + m3 exactly (up to the sign of a zero)
+            r[2] = fd.inv_proj_d[2 * 4 + 2] * v[2] + fd.inv_proj_d[3 * 4 + 2];
+            r[3] = fd.inv_proj_d[2 * 4 + 3] * v[2] + fd.inv_proj_d[3 * 4 + 3];
+        } else {
+            for (int i = 2; i < 4; i++)
+                r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
+                        fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
+        }
+        w = r[3];
+        z = r[2] / w;
+        o.shaded = z < (double)fd.bg_threshold;
     }
-    const double w = r[3];
-    const double z = r[2] / w;
-    o.shaded = z < (double)fd.bg_threshold;
     // 1 / w for the x and y coordinates: hardware reciprocal and two Newton steps (~1 ulp, double)
     double iw = __builtin_amdgcn_rcp(w);
     iw = iw + iw * (1.0 - w * iw);
@@ -243,6 +249,59 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     *((__attribute__((address_space(1))) vf4*)fd.out + p) = ov;
 }
 
+// ---- material resolve (shade.comp:89-96) -------------------------------------------------------------------------
+
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) vf2 gvf2;
+typedef const __attribute__((address_space(1))) uint16_t gu16;
+
+// one RGBA8 UNORM texel (a 4-byte load) as floats c / 255
+__device__ __forceinline__ float4 texel_rgba8(const vren_texture& t, uint32_t i, uint32_t j)
+{
+    const uint32_t v = *((guint*)(const uint32_t*)t.texels + ((uint64_t)j * t.width + i));
+    constexpr float k = 1.0f / 255.0f;
+    return make_float4((float)(v & 255u) * k, (float)((v >> 8) & 255u) * k, (float)((v >> 16) & 255u) * k,
+                       (float)(v >> 24) * k);
+}
+
+// texture(textures[ti], (s, t)) at the base level: the texture's filter, address modes and texel selection of
+// include/vren_hip/sampler.h; zeros for an index past the table or an empty texture
+__device__ __forceinline__ float4 sample_texture(const FrameDev& fd, uint32_t ti, float s, float t)
+{
+    if (ti >= fd.n_textures) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const vren_texture tx = fd.textures[ti];
+    if (!tx.texels || tx.width == 0 || tx.height == 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t i0, i1, j0, j1;
+    float a, b;
+    vren_texel_axis(s, tx.width, tx.filter, tx.address_u, &i0, &i1, &a);
+    vren_texel_axis(t, tx.height, tx.filter, tx.address_v, &j0, &j1, &b);
+    const float4 c00 = texel_rgba8(tx, i0, j0);
+    if (tx.filter != VREN_FILTER_LINEAR) return c00;
+    const float4 c10 = texel_rgba8(tx, i1, j0), c01 = texel_rgba8(tx, i0, j1), c11 = texel_rgba8(tx, i1, j1);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return make_float4(c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11,
+                       c00.y * w00 + c10.y * w10 + c01.y * w01 + c11.y * w11,
+                       c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11,
+                       c00.w * w00 + c10.w * w10 + c01.w * w01 + c11.w * w11);
+}
+
+// albedo (rgb) + roughness (w) and metallic of a pixel from its texcoord and material index; a material index past
+// the table reads a zero material
+__device__ __forceinline__ void resolve_material(const FrameDev& fd, vf2 tc, uint32_t mi, float4& ar, float& metal)
+{
+    ar = make_float4(0.f, 0.f, 0.f, 0.f);
+    metal = 0.0f;
+    if (mi >= fd.n_materials) return;
+    const vren_material m = fd.materials[mi];
+    const float4 base = sample_texture(fd, m.base_color_texture_idx, tc.x, tc.y);
+    const float4 mr = m.metallic_roughness_texture_idx == m.base_color_texture_idx
+                          ? base
+                          : sample_texture(fd, m.metallic_roughness_texture_idx, tc.x, tc.y);
+    ar = make_float4(base.x * m.base_color_factor[0], base.y * m.base_color_factor[1],
+                     base.z * m.base_color_factor[2], mr.y * m.roughness_factor);
+    metal = mr.z * m.metallic_factor;
+}
+
 // [begin, end) of the index-buffer entries of a tile's light lists: the tile's clusters are the consecutive indices
 // [c0, c0 + u) (clusters kernel) and their lists are consecutive in the index buffer, so the range runs from the
 // first cluster's offset to the end of the last list (both clamped to the capacities like the per-pixel lists).
@@ -273,7 +332,8 @@ __global__ __launch_bounds__(256) void tile_range_kernel(const FrameDev* __restr
 // clusters kernel's), the run is known from one load, so the staged indices are requested beside the pixel data and
 // the light records beside the pixels' list bounds: two dependent rounds of loads. Without it (references from the
 // caller), the workgroup first reduces the run from its pixels' lists.
-template <int PPT>
+// MAT: the table may hold frames on the material path (texcoords + material index + material and texture tables)
+template <int PPT, bool MAT>
 // at most 64 VGPRs: eight waves per SIMD (the split light loops would otherwise take 66 and seven)
 __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restrict__ frames,
                                                     const uint32_t* __restrict__ tile_frame,
@@ -310,6 +370,9 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
     uint32_t cl_raw[PPT];
     uint64_t p[PPT];
     bool in[PPT];
+    vf2 tc[PPT];
+    uint32_t mi[PPT];
+    const bool mat = MAT && fd.texcoords != nullptr;
     // every per-pixel load is issued before any of them is used
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
@@ -322,16 +385,28 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
         ar[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         metal[q] = 0.0f;
         cl_raw[q] = 0xFFFFFFFFu;
+        tc[q] = vf2{0.0f, 0.0f};
+        mi[q] = 0xFFFFFFFFu;
         if (in[q]) {
             const float depth = *((gfloat*)fd.depth + p[q]);
             const vu2 nv = *((gvu2*)fd.normals + p[q]);
-            const vf4 av = *((gvf4*)fd.albedo_rough + p[q]);
             nb[q] = make_uint2(nv.x, nv.y);
-            ar[q] = make_float4(av.x, av.y, av.z, av.w);
-            metal[q] = *((gfloat*)fd.metallic + p[q]);
+            if (mat) {
+                tc[q] = *((gvf2*)fd.texcoords + p[q]);
+                mi[q] = *((gu16*)fd.mat_idx + p[q]);
+            } else {
+                const vf4 av = *((gvf4*)fd.albedo_rough + p[q]);
+                ar[q] = make_float4(av.x, av.y, av.z, av.w);
+                metal[q] = *((gfloat*)fd.metallic + p[q]);
+            }
             cl_raw[q] = *((guint*)fd.refs + p[q]);
             px[q] = reconstruct(fd, gx, gy, depth);
         }
+    }
+    if (mat) {
+#pragma unroll
+        for (int q = 0; q < PPT; q++)
+            if (px[q].shaded) resolve_material(fd, tc[q], mi[q], ar[q], metal[q]);
     }
     // each pixel's light list
     bool has_list[PPT];
@@ -778,8 +853,11 @@ int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_
                                                                     w.total_tiles, w.tile_cl, w.tile_range);
         VREN_LAUNCH_CHECK(ctx);
     }
-    shade_kernel<kShadePPT><<<w.total_tiles * (4 / kShadePPT), 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr,
-                                                                           w.tile_range);
+    const uint32_t grid = w.total_tiles * (4 / kShadePPT);
+    if (w.any_materials)
+        shade_kernel<kShadePPT, true><<<grid, 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.tile_range);
+    else
+        shade_kernel<kShadePPT, false><<<grid, 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.tile_range);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
@@ -833,6 +911,61 @@ extern "C" VREN_EXPORT int vren_shade(vren_ctx* ctx, vren_stream_t stream, uint3
     ChainWork w;
     memset(&w, 0, sizeof(w));
     w.total_tiles = f.Sx * f.Sy;
+    if (!st) st = launch_shade(ctx, s, slot, 1, w);
+    return st;
+}
+
+extern "C" VREN_EXPORT int vren_shade_materials(vren_ctx* ctx, vren_stream_t stream, uint32_t width, uint32_t height,
+                                                const vren_camera* camera, const vren_cluster_cfg* cfg,
+                                                const float* depth, const uint16_t* normals,
+                                                const vren_material_set* materials, const uint32_t* cluster_refs,
+                                                uint32_t key_capacity, const uint32_t* light_counts,
+                                                const uint32_t* light_offsets, const uint32_t* light_indices,
+                                                uint32_t index_capacity, const float* light_positions,
+                                                const float* point_lights, uint32_t light_count,
+                                                const float* directional_lights, uint32_t directional_light_count,
+                                                float* output)
+{
+    if (!ctx) return VREN_EINVAL;
+    int st = vren_validate_camera(ctx, camera);
+    if (!st) st = vren_validate_cfg(ctx, cfg);
+    if (st) return st;
+    if (width == 0 || height == 0 || width > 8192 || height > 8192)
+        return vren_fail(ctx, VREN_EINVAL, "shade_materials: screen must be in [1, 8192]^2");
+    if (!depth || !normals || !materials || !cluster_refs || !output)
+        return vren_fail(ctx, VREN_EINVAL, "shade_materials: null G-buffer, material set, reference or output");
+    st = vren_validate_material_set(ctx, materials, "shade_materials");
+    if (st) return st;
+    if (light_count > 0 && (!light_counts || !light_offsets || !light_indices || !light_positions || !point_lights))
+        return vren_fail(ctx, VREN_EINVAL, "shade_materials: null light buffer");
+    if (directional_light_count > 0 && !directional_lights)
+        return vren_fail(ctx, VREN_EINVAL, "shade_materials: null directional light buffer");
+    if (ctx->device < 0) return vren_fail(ctx, VREN_EINVAL, "shade_materials: context has no device");
+    hipStream_t s = (hipStream_t)stream;
+    FrameDev f;
+    memset(&f, 0, sizeof(f));
+    vren_frame_setup(&f, width, height, camera, cfg);
+    f.depth = depth;
+    f.normals = normals;
+    vren_frame_set_materials(&f, materials);
+    f.refs = const_cast<uint32_t*>(cluster_refs);
+    f.key_cap = key_capacity;
+    f.counts = const_cast<uint32_t*>(light_counts);
+    f.offsets = const_cast<uint32_t*>(light_offsets);
+    f.indices = const_cast<uint32_t*>(light_indices);
+    f.index_cap = index_capacity;
+    f.light_pos = light_positions;
+    f.lights = point_lights;
+    f.L = light_count;
+    f.dir_lights = directional_lights;
+    f.n_dir = directional_light_count;
+    f.out = output;
+    FrameDev* slot = vren_next_frame_slot(ctx);
+    st = vren_upload_frames(ctx, s, slot, &f, 1);
+    ChainWork w;
+    memset(&w, 0, sizeof(w));
+    w.total_tiles = f.Sx * f.Sy;
+    w.any_materials = 1;
     if (!st) st = launch_shade(ctx, s, slot, 1, w);
     return st;
 }
