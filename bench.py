@@ -256,7 +256,14 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     ms = _median_ms(lambda: ctx.bucket_sort(pairs_in, nb, pairs_out, offs, 65536, 0), reps)
     # counting-sort unit: count read 8 B + scatter read 8 B + write 8 B per pair (pairs in and out alone: 16 B)
     res["bucket_sort_65536"] = _hbm(ms, 24 * nb, n=nb, algorithmic_bytes_per_pair=24)
-    del keys, pairs_in, pairs_out, offs
+    del pairs_in, pairs_out, offs
+    # blelloch_scan::downsweep of the up-sweep tree (radix_sort.cpp:281-289's use): one read and one write per key
+    tree = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.reduce_u32(keys, n, tree, vren_amd.REDUCE_ADD)
+    work = torch.empty_like(tree)
+    ms = _median_ms(lambda: ctx.blelloch_downsweep_u32(work, n, 1, True), reps, before=lambda: work.copy_(tree))
+    res["blelloch_downsweep_u32"] = _hbm(ms, 8 * n, n=n, algorithmic_bytes_per_key=8)
+    del keys, tree, work
     # point-light animation of 2^24 lights in place, one 60 Hz frame at speed 10 per call (position and direction
     # read and written: 64 B/light)
     nl = 1 << 24
@@ -269,7 +276,141 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     res["bounce_point_lights_2p24"] = _hbm(ms, 64 * nl, n=nl, algorithmic_bytes_per_light=64)
     del lpos, ldir
     res.update(bench_shade_frame(ctx, vren_amd, reps))
+    res.update(bench_pyramid(ctx, vren_amd, reps))
     return res
+
+
+def bench_pyramid(ctx, vren_amd, reps, frames=64):
+    """Hi-Z max pyramid of 64 stacked 1920x1080 depth buffers in one call: the depth read once (4 B/px), level 0
+    and every smaller level written once (4 B per texel of the pyramid)."""
+    import torch
+    cam = make_camera(vren_amd)
+    d = torch.empty(frames * W * H, dtype=torch.float32, device="cuda")
+    n = torch.empty(W * H * 4, dtype=torch.int16, device="cuda")
+    for f in range(frames):
+        ctx.synth_frame(SEED_FRAMES + f, W, H, cam, 64, d[f * W * H:(f + 1) * W * H], n)
+    plen = int(ctx.lib.vren_depth_pyramid_length(W, H))
+    out = torch.empty(frames * plen, dtype=torch.float32, device="cuda")
+    ms = _median_ms(lambda: ctx.depth_pyramid(d, W, H, frames, out), reps)
+    nbytes = frames * (4 * W * H + 4 * plen)
+    return {"depth_pyramid_64x1080p": _hbm(ms, nbytes, images=frames, texels_per_pyramid=plen,
+                                           algorithmic_bytes_per_pixel=round((4 * W * H + 4 * plen) / (W * H), 3))}
+
+
+def bench_material_path(ctx, vren_amd, keep, args, plain_steps):
+    """The bench's 64 frames with the reference's material path in shade (texcoord + material-index G-buffer, 8
+    materials over 6 RGBA8 textures of 1x1 to 256x256 in every filter and address mode): batched and plain
+    frames/s."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    rng = np.random.default_rng(11)
+    tex_dev, tab = [], (vren_amd.Texture * 6)()
+    for k, (w, h) in enumerate([(256, 256), (64, 128), (7, 5), (1, 1), (33, 17), (128, 64)]):
+        t = torch.from_numpy(rng.integers(0, 256, (h, w, 4), dtype=np.uint8)).cuda()
+        tex_dev.append(t)
+        tab[k].texels, tab[k].width, tab[k].height = t.data_ptr(), w, h
+        tab[k].filter, tab[k].address_u, tab[k].address_v = k % 2, k % 3, (k + 1) % 3
+    mats = (vren_amd.Material * 8)()
+    for m in range(8):
+        mats[m].base_color_texture_idx, mats[m].metallic_roughness_texture_idx = (2 * m) % 6, (2 * m + 1) % 6
+        mats[m].metallic_factor, mats[m].roughness_factor = 0.1 * m, 0.3 + 0.08 * m
+        for c in range(4):
+            mats[m].base_color_factor[c] = 0.5 + 0.05 * c
+    d_tab = torch.from_numpy(np.frombuffer(bytes(tab), np.uint8).copy()).cuda()
+    d_mats = torch.from_numpy(np.frombuffer(bytes(mats), np.uint8).copy()).cuda()
+    gy, gx = torch.meshgrid(torch.arange(H, device="cuda"), torch.arange(W, device="cuda"), indexing="ij")
+    tc = torch.stack([gx.reshape(-1) / W * 3.0 - 1.0, gy.reshape(-1) / H * 2.5 - 0.7], dim=1).float().contiguous()
+    mi = ((gx // 97 + gy // 61) % 8).reshape(-1).to(torch.int16).contiguous()
+    sets, descs = [], []
+    for fd0 in keep[-1]:
+        ms = vren_amd.MaterialSet()
+        ms.texcoords, ms.material_indices = tc.data_ptr(), mi.data_ptr()
+        ms.materials, ms.material_count, ms.textures, ms.texture_count = d_mats.data_ptr(), 8, d_tab.data_ptr(), 6
+        sets.append(ms)
+        fd = vren_amd.FrameDesc.from_buffer_copy(fd0)
+        fd.materials = ctypes.pointer(ms)
+        descs.append(fd)
+    b = vren_amd.Batch(ctx, descs, vren_amd.default_cfg())
+    for _ in range(max(1, args.warmup)):
+        b.run(b.GRAPH)
+    torch.cuda.synchronize()
+    ms_b, _ = time_events(lambda: b.run(b.GRAPH), args.steps)
+    for _ in range(max(1, args.warmup)):
+        b.run(b.PER_FRAME)
+    _, wall_p = time_events(lambda: b.run(b.PER_FRAME), plain_steps)
+    shade_ms = 0.0
+    for _ in range(3):
+        shade_ms += b.run_profiled()[KERNELS.index("shade")] / 3
+    ctx.sync_status()
+    b.close()
+    fps = args.frames * args.steps / (ms_b * 1e-3)
+    plain = args.frames * plain_steps / (wall_p * 1e-3)
+    px = args.frames * W * H
+    print("[bench] material path: %.3f ms/step batched, %.1f frames/s plain" % (ms_b / args.steps, plain),
+          file=sys.stderr, flush=True)
+    # shade on the material path reads depth 4, normal 8, texcoord 8, material index 2, reference 4 and writes 16 B
+    # per pixel (the material and texture tables are a few hundred KB and stay in the caches)
+    return {"frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / args.steps, 4),
+            "plain_frames_per_s": round(plain, 2), "batched_over_plain": round(fps / plain, 3),
+            "shade": _hbm(shade_ms, 42 * px, algorithmic_bytes_per_pixel=42)}
+
+
+def bench_config4(ctx, vren_amd, args):
+    """The batched configuration: 64 frames, resolutions drawn from {640x360, 1280x720, 1920x1080, 2560x1440,
+    3840x2160}, light counts log-uniform in [256, 65536] (the same draw as the configuration-4 parity test),
+    batched (one graph replay) and plain (per frame launches + stream sync) frames/s."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(4444)
+    res = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+    keep, descs = [], []
+    total_px = 0
+    for f in range(64):
+        w, h = res[rng.integers(0, len(res))]
+        L = int(round(float(np.exp(rng.uniform(np.log(256.0), np.log(65536.0))))))
+        cam = vren_amd.Camera()
+        cam.position[2] = -50.0
+        cam.fov_y = float(np.float32(np.deg2rad(90.0)))
+        cam.aspect_ratio = float(np.float32(w) / np.float32(h))
+        cam.near_plane, cam.far_plane = 0.1, 100.0
+        t = dict(d=torch.empty(w * h, dtype=torch.float32, device="cuda"),
+                 n=torch.empty(w * h * 4, dtype=torch.int16, device="cuda"),
+                 a=torch.empty(w * h * 4, dtype=torch.float32, device="cuda"),
+                 m=torch.empty(w * h, dtype=torch.float32, device="cuda"),
+                 p=torch.empty(L * 4, dtype=torch.float32, device="cuda"),
+                 l=torch.empty(L * 4, dtype=torch.float32, device="cuda"),
+                 o=torch.empty(w * h * 4, dtype=torch.float32, device="cuda"))
+        ctx.synth_frame(7000 + f, w, h, cam, 64, t["d"], t["n"])
+        ctx.synth_materials(7000 + f, w * h, t["a"], t["m"])
+        ctx.synth_lights(7007 + f, L, t["p"], t["l"])
+        fd = vren_amd.FrameDesc()
+        fd.width, fd.height, fd.camera = w, h, cam
+        fd.depth, fd.normals = t["d"].data_ptr(), t["n"].data_ptr()
+        fd.albedo_roughness, fd.metallic = t["a"].data_ptr(), t["m"].data_ptr()
+        fd.light_positions, fd.point_lights, fd.light_count = t["p"].data_ptr(), t["l"].data_ptr(), L
+        fd.output = t["o"].data_ptr()
+        descs.append(fd)
+        keep.append(t)
+        total_px += w * h
+    torch.cuda.synchronize()
+    b = vren_amd.Batch(ctx, descs, vren_amd.default_cfg())
+    for _ in range(max(1, args.warmup)):
+        b.run(b.GRAPH)
+    torch.cuda.synchronize()
+    steps = max(3, args.steps // 2)
+    ms_b, _ = time_events(lambda: b.run(b.GRAPH), steps)
+    b.run(b.PER_FRAME)
+    _, wall_p = time_events(lambda: b.run(b.PER_FRAME), 2)
+    ctx.sync_status()
+    b.close()
+    fps = 64 * steps / (ms_b * 1e-3)
+    plain = 64 * 2 / (wall_p * 1e-3)
+    print("[bench] configuration 4: %.3f ms/step batched, %.1f frames/s plain" % (ms_b / steps, plain),
+          file=sys.stderr, flush=True)
+    return {"frames": 64, "pixels": total_px, "frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / steps, 4),
+            "plain_frames_per_s": round(plain, 2), "batched_over_plain": round(fps / plain, 3),
+            "megapixels_per_s": round(total_px * steps / (ms_b * 1e-3) / 1e6, 1)}
 
 
 def bench_shade_frame(ctx, vren_amd, reps):
@@ -344,23 +485,27 @@ def bench_shade_frame(ctx, vren_amd, reps):
     return res
 
 
-def cpu_baseline(frames=2, sort_log2=22):
-    """The CPU restatement (oracle/) on one host thread: the chain on one 1080p frame with the reference's fp32
-    shade formula, and std::stable_sort of configuration-1 pairs. Inputs are generated outside the timing."""
+def _cpu_threads():
+    """std::thread::hardware_concurrency() of the host, capped at this GPU's share of it (16 cores)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _chain_frames(co, ol, seeds, threads):
+    """Wall seconds of the oracle chain (BVH, clusters, assignment, the reference's fp32 shade) over one 1080p frame
+    per seed, `threads` frames at a time, each frame on one host thread; inputs generated before the timing."""
+    import concurrent.futures
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as ol
-    o = ol.Oracle()
-    co = ol.ClusteredOracle(o)
     cam = ol.camera(W, H)
     cf = ol.cfg()
-    total = 0.0
-    for f in range(frames):
-        seed = SEED_FRAMES + f
+
+    def synth(seed):
         depth, normals = co.synth_frame(seed, W, H, cam)
         ar, me = co.synth_materials(seed, W * H)
         pos, lig = co.synth_lights(seed + 7, L_LIGHTS)
-        t0 = time.perf_counter()
+        return depth, normals, ar, me, pos, lig
+
+    def chain(inp):
+        depth, normals, ar, me, pos, lig = inp
         vp, codes, bvh = co.light_bvh(pos, lig, cam, cf)
         keys, U, refs = co.unique_clusters(W, H, cam, cf, depth, normals)
         counts, offsets, idx, tot = co.assign(W, H, cam, cf, keys, U, bvh, codes, vp)
@@ -369,21 +514,59 @@ def cpu_baseline(frames=2, sort_log2=22):
         cc[:counts.size] = counts
         oo[:offsets.size] = offsets
         co.shade(W, H, cam, cf, depth, normals, ar, me, refs, 1 << 20, cc, oo, idx, pos, lig, f32=True)
-        total += time.perf_counter() - t0
-    n = 1 << sort_log2
-    keys = o.pcg32(42, 54, n)
-    vals = np.arange(n, dtype=np.uint32)
-    t0 = time.perf_counter()
-    o.stable_sort_kv(keys, vals)
-    st = time.perf_counter() - t0
-    return {
-        "value": round(frames / total, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-        "sample": "%d frames of 1920x1080 / 4096 lights through the oracle chain (BVH, clusters, assignment, "
-                  "reference fp32 shade), one thread, input generation excluded" % frames,
-        "sort": {"value": round(n / st / 1e9, 4), "unit": "Gpairs/s", "cores": 1,
-                 "sample": "std::stable_sort of 2^%d configuration-1 (key, index) pairs" % sort_log2},
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        inputs = list(ex.map(synth, seeds))
+        t0 = time.perf_counter()
+        list(ex.map(chain, inputs))
+        return time.perf_counter() - t0
+
+
+def cpu_baseline(sort_log2=(22, 26), scan_log2=26):
+    """The CPU restatement (oracle/) on the host: 1 thread and N = hardware_concurrency() threads (capped at the
+    GPU's 16-core share; std::thread, no TBB). The chain runs one 1080p frame per thread (N frames at once for the
+    N-thread figure); std::stable_sort of configuration-1 (key, index) pairs, std::exclusive_scan and std::accumulate
+    of configuration-1 keys are split over the threads (chunks, then merge levels / a prefix over chunk totals).
+    Input generation is outside every timing; each figure is the best of 3 runs (the chain: one run)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    o = ol.Oracle()
+    co = ol.ClusteredOracle(o)
+    nt = _cpu_threads()
+    t1 = _chain_frames(co, ol, [SEED_FRAMES], 1)
+    tn = _chain_frames(co, ol, [SEED_FRAMES + f for f in range(nt)], nt)
+    res = {
+        "value": round(nt / tn, 3), "unit": "frames/s", "cores": nt, "kind": "port",
+        "sample": "%d frames of 1920x1080 / 4096 lights through the oracle chain (BVH, clusters, assignment, reference "
+                  "fp32 shade), one frame per thread, %d threads at once; input generation excluded" % (nt, nt),
+        "single_thread": {"value": round(1.0 / t1, 3), "unit": "frames/s", "cores": 1,
+                          "sample": "1 frame of the same chain on one thread"},
         "host_cpu": _cpu_model(),
     }
+    sorts = {}
+    for lg in sort_log2:
+        n = 1 << lg
+        keys = o.pcg32(42, 54, n)
+        vals = np.arange(n, dtype=np.uint32)
+        entry = {}
+        for th in (1, nt):
+            best = min(o.cpu_stable_sort_pairs(keys, vals, th)[0] for _ in range(1 if lg >= 26 and th == 1 else 3))
+            entry["threads_%d" % th] = round(n / best / 1e9, 4)
+        sorts["2^%d" % lg] = entry
+    res["sort"] = {"unit": "Gpairs/s", "cores": nt, "sample": "std::stable_sort of configuration-1 (key, index) "
+                   "pairs, per thread count", "results": sorts}
+    n = 1 << scan_log2
+    x = o.pcg32(42, 54, n)
+    scan, red = {}, {}
+    for th in (1, nt):
+        scan["threads_%d" % th] = round(n / min(o.cpu_exclusive_scan(x, th)[0] for _ in range(3)) / 1e9, 4)
+        red["threads_%d" % th] = round(n / min(o.cpu_reduce_add(x, th)[0] for _ in range(3)) / 1e9, 4)
+    res["exclusive_scan"] = {"unit": "Gkeys/s", "cores": nt, "sample": "std::exclusive_scan of 2^%d uint32 "
+                             "configuration-1 keys" % scan_log2, "results": scan}
+    res["reduce_add"] = {"unit": "Gkeys/s", "cores": nt, "sample": "std::accumulate of 2^%d uint32 configuration-1 "
+                         "keys" % scan_log2, "results": red}
+    return res
 
 
 def _cpu_model():
@@ -434,6 +617,7 @@ def main():
     ap.add_argument("--no-rows", action="store_true", help="skip reduce, scan, bucket sort and the shade adjoint")
     ap.add_argument("--no-key-variant", action="store_true",
                     help="skip the second batch with the 32-slice x 6-bin cluster-key encoder")
+    ap.add_argument("--no-config4", action="store_true", help="skip the 64-frame heterogeneous batch")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB with one plain device copy kernel (scales HBM counters)")
     args = ap.parse_args()
@@ -516,6 +700,14 @@ def main():
     if not args.no_key_variant:
         sliced = bench_key_variant(ctx, vren_amd, keep[-1], args, plain_steps)
 
+    material = None
+    if not args.no_key_variant:
+        material = bench_material_path(ctx, vren_amd, keep, args, plain_steps)
+    config4 = None
+    if not args.no_config4:
+        print("[bench] configuration 4 (64 heterogeneous frames)", file=sys.stderr, flush=True)
+        config4 = bench_config4(ctx, vren_amd, args)
+
     sort = None
     if not args.no_sort:
         print("[bench] sort 2^%d pairs" % args.sort_log2, file=sys.stderr, flush=True)
@@ -544,6 +736,8 @@ def main():
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(value / plain_fps, 3),
         "config_key_encoder": sliced,
+        "config_material_path": material,
+        "config4_heterogeneous": config4,
         "kernel_ms": kernel_ms,
         "roofline": roofline,
         "shade_valu": shade_valu,

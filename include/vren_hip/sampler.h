@@ -10,7 +10,9 @@
  *   LINEAR:  i0 = floor(v), i1 = i0 + 1, alpha = v - floor(v), v = s * size - 0.5 rounded once (a fused
  *            multiply-add, so every translation unit computes the same v whatever its contraction mode);
  * then every integer coordinate is wrapped: REPEAT i mod size; MIRRORED_REPEAT (size - 1) - mirror((i mod 2 size)
- * - size), mirror(a) = a >= 0 ? a : -(1 + a); CLAMP_TO_EDGE clamp(i, 0, size - 1).
+ * - size), mirror(a) = a >= 0 ? a : -(1 + a); CLAMP_TO_EDGE clamp(i, 0, size - 1). The wrap is evaluated in float
+ * (vren_fmod_floor: exact for coordinates below 2^24 in magnitude), which the GPU does in a few instructions where
+ * a 64-bit integer remainder takes a long software sequence.
  * A driver's sampler converts u to fixed point (8 bits of sub-texel precision on current GPUs) before it takes
  * the floor and the fraction; that rounding is the driver's and no test can pin it without one. Here u, floor and
  * the fraction are the float operations above, shared by the kernels and the CPU oracle so that both pick the same
@@ -22,32 +24,47 @@
 #include "capi.h" /* VREN_FILTER_*, VREN_ADDRESS_* */
 #include "vmath.h"
 
-/* floor(x) as an integer, saturated to +-2^40 (far beyond any texture size times a usable repeat count) */
-VREN_HD int64_t vren_floor_i64(float x)
+/* x mod n for an integer-valued x and a period n, in float arithmetic: r = x - floor(x / n) n (one fused
+ * multiply-add), brought into [0, n) by one correction either way. Exact for |x| < 2^24 (every step is an integer
+ * below 2^24); beyond, the same float steps define it on both sides. */
+VREN_HD float vren_fmod_floor(float x, float n)
 {
-    const float f = vren_floorf(x);
-    if (!(f > -1099511627776.0f)) return f != f ? 0 : -1099511627776ll;
-    if (f > 1099511627776.0f) return 1099511627776ll;
-    return (int64_t)f;
+    const float q = vren_floorf(x / n);
+    float r = __builtin_fmaf(-q, n, x);
+    if (r < 0.0f) r += n;
+    if (r >= n) r -= n;
+    return r;
 }
 
-VREN_HD uint32_t vren_wrap_texel(int64_t i, uint32_t size, uint32_t mode)
+/* the integer coordinate of texel floor(v) (and of its right neighbour) wrapped by `mode` for a texture side of
+ * `size` texels: REPEAT floor(v) mod size; MIRRORED_REPEAT t = floor(v) mod 2 size, t < size ? t : 2 size - 1 - t;
+ * CLAMP_TO_EDGE clamp(floor(v), 0, size - 1). All in float and 32-bit integers (no 64-bit division). */
+VREN_HD void vren_wrap_pair(float v, uint32_t size, uint32_t mode, uint32_t* a, uint32_t* b)
 {
-    const int64_t n = (int64_t)size;
-    if (mode == VREN_ADDRESS_CLAMP_TO_EDGE) return (uint32_t)(i < 0 ? 0 : (i >= n ? n - 1 : i));
-    if (mode == VREN_ADDRESS_MIRRORED_REPEAT) {
-        int64_t t = i % (2 * n);
-        if (t < 0) t += 2 * n;
-        const int64_t a = t - n;
-        const int64_t m = a >= 0 ? a : -(1 + a);
-        return (uint32_t)(n - 1 - m);
+    const float n = (float)size;
+    if (mode == VREN_ADDRESS_CLAMP_TO_EDGE) {
+        const float f = vren_floorf(v);
+        const float f0 = f < 0.0f ? 0.0f : (f > n - 1.0f ? n - 1.0f : f);
+        const float f1 = f + 1.0f < 0.0f ? 0.0f : (f + 1.0f > n - 1.0f ? n - 1.0f : f + 1.0f);
+        *a = (uint32_t)f0;
+        *b = (uint32_t)f1;
+        return;
     }
-    int64_t r = i % n;
-    if (r < 0) r += n;
-    return (uint32_t)r;
+    const uint32_t period = mode == VREN_ADDRESS_MIRRORED_REPEAT ? 2u * size : size;
+    uint32_t t0 = (uint32_t)vren_fmod_floor(vren_floorf(v), (float)period);
+    if (t0 >= period) t0 = period - 1u; /* never taken for exact arithmetic; keeps the index in range */
+    const uint32_t t1 = t0 + 1u == period ? 0u : t0 + 1u;
+    if (mode == VREN_ADDRESS_MIRRORED_REPEAT) {
+        *a = t0 < size ? t0 : 2u * size - 1u - t0;
+        *b = t1 < size ? t1 : 2u * size - 1u - t1;
+        return;
+    }
+    *a = t0;
+    *b = t1;
 }
 
-/* texel coordinates along one axis: *i0 (and *i1, *alpha for LINEAR; *i1 = *i0 and *alpha = 0 for NEAREST) */
+/* texel coordinates along one axis: *i0 (and *i1, *alpha for LINEAR; *i1 = *i0 and *alpha = 0 for NEAREST). Texture
+ * sides up to 2^24 texels. */
 VREN_HD void vren_texel_axis(float s, uint32_t size, uint32_t filter, uint32_t mode, uint32_t* i0, uint32_t* i1,
                              float* alpha)
 {
@@ -60,14 +77,12 @@ VREN_HD void vren_texel_axis(float s, uint32_t size, uint32_t filter, uint32_t m
     }
     if (filter == VREN_FILTER_LINEAR) {
         const float v = __builtin_fmaf(s, (float)size, -0.5f);
-        const float fl = vren_floorf(v);
-        const int64_t k = vren_floor_i64(v);
-        *alpha = v - fl;
-        *i0 = vren_wrap_texel(k, size, mode);
-        *i1 = vren_wrap_texel(k + 1, size, mode);
+        *alpha = v - vren_floorf(v);
+        vren_wrap_pair(v, size, mode, i0, i1);
         return;
     }
-    *i0 = vren_wrap_texel(vren_floor_i64(u), size, mode);
+    uint32_t unused;
+    vren_wrap_pair(u, size, mode, i0, &unused);
     *i1 = *i0;
     *alpha = 0.0f;
 }

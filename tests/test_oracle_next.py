@@ -275,3 +275,31 @@ def test_cpu_baseline_helpers_give_serial_results(oracle, threads):
     assert np.array_equal(sc, oracle.exclusive_scan_u32(x))
     _, r = oracle.cpu_reduce_add(x, threads)
     assert r == int(np.sum(x, dtype=np.uint64) & 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("size", [1, 7, 16, 200])
+def test_texel_wrap_matches_integer_definition(co, mode, size):
+    """Texture whose texel x holds R = x: over random coordinates up to |u| ~ 2^22, NEAREST selects
+    wrap(floor(fl32(s * size))) exactly, with the wrap computed here on Python integers (REPEAT: mod size;
+    MIRRORED_REPEAT: t = i mod 2 size, t < size ? t : 2 size - 1 - t; CLAMP_TO_EDGE: clamp)."""
+    tex = np.zeros((1, size, 4), np.uint8)
+    tex[0, :, 0] = np.arange(size) % 256
+    tex[0, :, 1] = np.arange(size) // 256
+    rng = np.random.default_rng(size * 3 + mode)
+    s = np.concatenate([rng.uniform(-3, 3, 4000), rng.uniform(-2e4, 2e4, 4000) / size,
+                        (np.arange(-50, 50) / size).astype(np.float64)]).astype(F32)
+    mats = _materials((0, 0, 1.0, 1.0, (1.0, 1.0, 1.0, 1.0)))
+    tc = np.stack([s, np.zeros_like(s)], axis=1)
+    ar, _ = co.resolve_materials(tc, np.zeros(s.size, np.uint16), mats, [(tex, 0, mode, mode)])
+    got = np.rint(ar[:, 0] * 255).astype(np.int64) + 256 * np.rint(ar[:, 3] * 255).astype(np.int64)
+    for k in range(s.size):
+        i = int(np.floor(np.float64(np.float32(s[k]) * np.float32(size))))
+        if mode == 0:
+            want = i % size
+        elif mode == 1:
+            t = i % (2 * size)
+            want = t if t < size else 2 * size - 1 - t
+        else:
+            want = min(max(i, 0), size - 1)
+        assert got[k] == want, (s[k], got[k], want)
