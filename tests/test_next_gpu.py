@@ -210,7 +210,8 @@ def test_batch_mixes_material_and_resolved_frames(vctx, co):
 # ---- depth pyramid -------------------------------------------------------------------------------------------------
 
 @pytest.mark.parametrize("W,H,images", [(1, 1, 1), (2, 1, 3), (5, 100, 2), (64, 64, 1), (333, 200, 4),
-                                        (1920, 1080, 1), (4096, 3, 1), (640, 360, 64), (3840, 2160, 1)])
+                                        (1920, 1080, 1), (1920, 1080, 3), (4096, 3, 1), (640, 360, 64),
+                                        (3840, 2160, 1), (65535, 2, 1)])
 def test_depth_pyramid_bit_exact(vctx, co, W, H, images):
     """Level 0 is the depth copy, every level the GLSL max over its in-bounds 2x2 texels; odd sizes drop the last
     row / column; NaN, negative and -0 depths in the input. `images` stacked depth buffers in one call."""

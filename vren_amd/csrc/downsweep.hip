@@ -9,10 +9,11 @@
 //     out[i] = P + sum over the set bits l of i of d[((i >> (l + 1)) << (l + 1)) + 2^l - 1],
 // P being the root's prefix. That is what is computed here, with each element read once and written once:
 //   1. (blocks longer than one tile) one thread per 4096-element tile evaluates the closed form at the tile level,
-//      over the tiles' last elements, into a prefix array;
-//   2. one workgroup per tile holds 16 consecutive elements per thread in registers; the thread's chunk prefix is the
-//      closed form over the tile's chunk-last elements (staged in LDS, at most 8 reads), its 16 outputs the closed
-//      form over its own registers.
+//      over the tiles' last elements, into a prefix array (all its reads issued before the sum, no dependent chain);
+//   2. one workgroup per tile holds it in registers as rows of 1024 elements, 4 consecutive elements per thread (one
+//      16-byte access per row, consecutive threads on consecutive addresses); a thread's 4-element group prefix is
+//      the closed form over the tile's group-last elements (staged in LDS), its 4 outputs the closed form over its
+//      own registers.
 // Root prefix P: 0 when clear_last; otherwise the block's last element, except for blocks shorter than 1024, where
 // the reference's 1024-wide LDS pass pads the block with zeros and the padding's prefix, 0, reaches the block's
 // root (so such a block is always scanned from 0).
@@ -21,10 +22,16 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 16;                  // elements per thread
-constexpr int kTile = kThreads * kChunk;    // 4096 elements per workgroup
+constexpr int kGroup = 4;                  // consecutive elements per thread and row
+constexpr int kRowLen = kThreads * kGroup; // 1024
+constexpr int kRows = 4;
+constexpr int kTile = kRowLen * kRows;     // 4096 elements per workgroup
+constexpr int kMaxLogTiles = 32;
 
-// closed form at the tile level: prefix of tile k of each block
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// closed form at the tile level: prefix of tile k of each block. Every term is loaded (a term whose bit is clear
+// loads the block's last element and is dropped), so the loads are independent and issue together.
 __global__ void downsweep_tile_prefix_kernel(const uint32_t* __restrict__ data, uint32_t length, uint32_t tiles,
                                              uint32_t log_tiles, uint64_t n_total, int root_cleared,
                                              uint32_t* __restrict__ prefix)
@@ -34,27 +41,23 @@ __global__ void downsweep_tile_prefix_kernel(const uint32_t* __restrict__ data, 
     const uint64_t seg = g / tiles;
     const uint32_t k = (uint32_t)(g - seg * tiles);
     const uint32_t* base = data + seg * length;
+    uint32_t t[kMaxLogTiles];
+#pragma unroll
+    for (uint32_t l = 0; l < kMaxLogTiles; l++) {
+        const bool on = l < log_tiles && ((k >> l) & 1u);
+        // left sibling's last tile at level l
+        const uint64_t j = ((uint64_t)(k >> (l + 1)) << (l + 1)) + (1ull << l) - 1;
+        t[l] = base[on ? j * kTile + kTile - 1 : (uint64_t)length - 1];
+        if (!on) t[l] = 0;
+        if (l + 1 >= log_tiles) break;
+    }
     uint32_t p = root_cleared ? 0u : base[length - 1];
-    for (uint32_t l = 0; l < log_tiles; l++) {
-        if ((k >> l) & 1u) {
-            const uint64_t j = ((uint64_t)(k >> (l + 1)) << (l + 1)) + (1ull << l) - 1; // left sibling's last tile
-            p += base[j * kTile + kTile - 1];
-        }
+#pragma unroll
+    for (uint32_t l = 0; l < kMaxLogTiles; l++) {
+        p += t[l];
+        if (l + 1 >= log_tiles) break;
     }
     prefix[g] = p;
-}
-
-// out[j] of a 16-element chunk from its prefix p and its input e (the closed form over 4 levels, unrolled)
-__device__ __forceinline__ void chunk_downsweep(const uint32_t (&e)[kChunk], uint32_t p, uint32_t (&o)[kChunk])
-{
-#pragma unroll
-    for (int j = 0; j < kChunk; j++) {
-        uint32_t v = p;
-#pragma unroll
-        for (int l = 0; l < 4; l++)
-            if ((j >> l) & 1) v += e[((j >> (l + 1)) << (l + 1)) + (1 << l) - 1];
-        o[j] = v;
-    }
 }
 
 __global__ __launch_bounds__(kThreads) void downsweep_tile_kernel(uint32_t* __restrict__ data, uint32_t length,
@@ -62,7 +65,7 @@ __global__ __launch_bounds__(kThreads) void downsweep_tile_kernel(uint32_t* __re
                                                                   const uint32_t* __restrict__ prefix,
                                                                   int root_cleared)
 {
-    __shared__ uint32_t s_last[kThreads];
+    __shared__ uint32_t s_last[kTile / kGroup];
     const uint32_t tid = threadIdx.x;
     const uint64_t g = blockIdx.x;
     const uint64_t seg = g / tiles;
@@ -70,54 +73,51 @@ __global__ __launch_bounds__(kThreads) void downsweep_tile_kernel(uint32_t* __re
     uint32_t* tile = data + seg * length + (uint64_t)t * tile_len;
     const uint32_t root = prefix ? prefix[g] : (root_cleared ? 0u : tile[tile_len - 1]);
 
-    if (tile_len < kChunk) { // blocks of 1, 2, 4 or 8 elements: one element per thread
+    if (tile_len < kGroup) { // blocks of 1 or 2 elements: one element per thread
         uint32_t v = 0;
         if (tid < tile_len) {
             v = root;
-            for (uint32_t l = 0; (1u << l) < tile_len; l++)
-                if ((tid >> l) & 1u) v += tile[((tid >> (l + 1)) << (l + 1)) + (1u << l) - 1];
+            if (tid == 1) v += tile[0];
         }
         __syncthreads(); // every read of the block precedes every write
         if (tid < tile_len) tile[tid] = v;
         return;
     }
 
-    const uint32_t chunks = tile_len / kChunk;
-    uint32_t e[kChunk];
-    if (tid < chunks) {
-        uint32_t* c = tile + tid * kChunk;
-        if ((reinterpret_cast<uintptr_t>(c) & 15) == 0) {
+    // rows of row_len elements, row_len / 4 threads per row
+    const uint32_t row_len = tile_len < (uint32_t)kRowLen ? tile_len : (uint32_t)kRowLen;
+    const uint32_t rows = tile_len / row_len, row_groups = row_len / kGroup;
+    const bool active = tid < row_groups;
+    uint32_t e[kRows][kGroup];
 #pragma unroll
-            for (int q = 0; q < kChunk / 4; q++) {
-                uint4 v = ntload4(c + 4 * q);
-                e[4 * q] = v.x;
-                e[4 * q + 1] = v.y;
-                e[4 * q + 2] = v.z;
-                e[4 * q + 3] = v.w;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < kChunk; j++) e[j] = c[j];
+    for (uint32_t r = 0; r < kRows; r++) {
+        if (r < rows && active) {
+            const u32x4_a4 v = __builtin_nontemporal_load((const u32x4_a4*)(tile + r * row_len + kGroup * tid));
+            e[r][0] = v.x;
+            e[r][1] = v.y;
+            e[r][2] = v.z;
+            e[r][3] = v.w;
+            s_last[r * row_groups + tid] = v.w;
         }
-        s_last[tid] = e[kChunk - 1];
     }
     __syncthreads();
-    if (tid >= chunks) return;
-    // the chunk's prefix: closed form over the chunk-last elements (the tile's root is replaced by its prefix and
-    // never read as a left sibling)
-    uint32_t p = root;
-    for (uint32_t l = 0; (1u << l) < chunks; l++)
-        if ((tid >> l) & 1u) p += s_last[((tid >> (l + 1)) << (l + 1)) + (1u << l) - 1];
-    uint32_t o[kChunk];
-    chunk_downsweep(e, p, o);
-    uint32_t* c = tile + tid * kChunk;
-    if ((reinterpret_cast<uintptr_t>(c) & 15) == 0) {
+    if (!active) return;
+    const uint32_t groups = rows * row_groups;
 #pragma unroll
-        for (int q = 0; q < kChunk / 4; q++)
-            ntstore4(c + 4 * q, make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]));
-    } else {
-#pragma unroll
-        for (int j = 0; j < kChunk; j++) c[j] = o[j];
+    for (uint32_t r = 0; r < kRows; r++) {
+        if (r >= rows) break;
+        // the group's prefix: closed form over the group-last elements (the tile's root is replaced by its prefix
+        // and never read as a left sibling)
+        const uint32_t G = r * row_groups + tid;
+        uint32_t p = root;
+        for (uint32_t l = 0; (1u << l) < groups; l++)
+            if ((G >> l) & 1u) p += s_last[((G >> (l + 1)) << (l + 1)) + (1u << l) - 1];
+        u32x4_a4 o;
+        o.x = p;
+        o.y = p + e[r][0];
+        o.z = p + e[r][1];
+        o.w = p + e[r][1] + e[r][2];
+        __builtin_nontemporal_store(o, (u32x4_a4*)(tile + r * row_len + kGroup * tid));
     }
 }
 
