@@ -522,3 +522,45 @@ def _read_ptr(ptr, nbytes, dtype):
                        ctypes.c_int(2))
     assert rc == 0
     return out[:nbytes].view(dtype)
+
+
+def test_batch_light_count_sequence_and_animation(vctx, co):
+    """A frame sequence through the batched chain. (a) Light counts that change between frames: a batch is
+    specialised to its frames' light counts (its buffer sizes, BVH shapes, the BVH padding written at creation and
+    the captured graph), so a new count takes a new batch. The sequence 4096 -> 1000 -> 1 -> 40000 -> 0 -> 4096
+    rebuilds it each time, and every frame of every batch matches the oracle. (b) With the count fixed, the lights
+    are moved in place by bounce_point_lights between graph replays (the reference demo's per-frame animation
+    ahead of cluster_and_shade): every replay reads the moved lights and matches the oracle."""
+    import torch
+    import vren_amd
+    cf = ol.cfg()
+    for step, L in enumerate([4096, 1000, 1, 40000, 0, 4096]):
+        specs = [(640, 360, L, 9000 + 10 * step), (1280, 720, L, 9001 + 10 * step)]
+        descs, keep = make_frames_batch(vctx, co, specs)
+        b = vren_amd.Batch(vctx, descs, cf)
+        b.run(b.GRAPH)
+        vctx.sync_status()
+        _batch_check_all(b, keep, co, cf, "light count %d" % L)
+        b.close()
+    # (b) animation between replays of one captured graph
+    specs = [(640, 360, 3000, 9100), (1920, 1080, 4096, 9101)]
+    descs, keep = make_frames_batch(vctx, co, specs)
+    b = vren_amd.Batch(vctx, descs, cf)
+    rng = np.random.default_rng(9102)
+    dirs = []
+    for k in keep:
+        d = rng.normal(size=(k["L"], 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        dirs.append(to_dev(np.concatenate([d, np.zeros((k["L"], 1))], 1).astype(np.float32)))
+    for frame in range(3):
+        if frame:
+            for k, dd in zip(keep, dirs):
+                vctx.bounce_point_lights(k["p"], dd, k["L"], (-50.0, -50.0, -50.0), (50.0, 50.0, 50.0), 10.0,
+                                         1.0 / 60.0)
+        for k in keep:
+            k["o"].fill_(0xFF)
+        b.run(b.GRAPH)
+        torch.cuda.synchronize()
+        vctx.sync_status()
+        _batch_check_all(b, keep, co, cf, "animated frame %d" % frame)
+    b.close()

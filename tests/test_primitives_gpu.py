@@ -193,7 +193,7 @@ def test_radix_sort_headline_size_properties(vctx, oracle):
 
 
 @pytest.mark.parametrize("n_buckets,shift,keys_kind", [(65536, 0, "u16"), (64, 24, "u30"), (1, 0, "u16"),
-                                                        (65536, 24, "u32"), (4096, 7, "u32")])
+                                                        (65536, 24, "u32"), (4096, 7, "u32"), (65536, 0, "few")])
 def test_bucket_sort(vctx, oracle, n_buckets, shift, keys_kind):
     # bucket_sort.main (vren_test/vren_test/primitives/bucket_sort.cpp:182-192): N = 1, then x(4..6) while < 10^7,
     # key = rand() % 65536, value = i -- here seeded PCG32 draws and a fixed growth sequence. "u32": full 32-bit
@@ -207,6 +207,10 @@ def test_bucket_sort(vctx, oracle, n_buckets, shift, keys_kind):
         raw = oracle.pcg32(4242, 54, n)
         if keys_kind == "u16":
             keys, vals = raw % 65536, np.arange(n, dtype=np.uint32)
+        elif keys_kind == "few":
+            # three keys far apart: long runs of one bucket and long gaps of empty buckets between them, inside a
+            # tile's digit run and across tiles
+            keys, vals = (raw % 3) * 20011 + 7, np.arange(n, dtype=np.uint32)
         elif keys_kind == "u30":
             keys, vals = raw >> 2, np.arange(n, dtype=np.uint32)  # 30-bit codes for the 64 coarse buckets
         else:

@@ -9,24 +9,12 @@
 //            first reads the pairs, the last writes them; no atomics, no gather);
 //   offsets  bucket b starts at the first sorted position whose bucket is >= b, so each position i writes the
 //            starts of the buckets in (bucket(i - 1), bucket(i)], and the last position those after it (each start
-//            is written exactly once).
+//            is written exactly once). The final sort pass does this for every key whose predecessor it staged
+//            itself, and a small kernel over the tiles' digit runs for the first key of each run, so the sorted
+//            pairs are not read again.
 #include "radix_sort.hpp"
 
 namespace {
-
-__global__ __launch_bounds__(256) void bucket_offsets_kernel(const uint32_t* __restrict__ pairs, uint32_t n,
-                                                             uint32_t shift, uint32_t mask, uint32_t n_buckets,
-                                                             uint32_t* __restrict__ offsets)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t b = (pairs[2 * (uint64_t)i] >> shift) & mask;
-        // buckets (prev, b] start here; prev = -1 before the first position
-        const uint32_t first = i == 0 ? 0u : ((pairs[2 * (uint64_t)i - 2] >> shift) & mask) + 1u;
-        for (uint32_t c = first; c <= b; c++) offsets[c] = i;
-        if (i == n - 1)
-            for (uint32_t c = b + 1; c < n_buckets; c++) offsets[c] = n;
-    }
-}
 
 __global__ void bucket_fill_kernel(uint32_t* offsets, uint32_t n_buckets, uint32_t v)
 {
@@ -57,16 +45,17 @@ extern "C" VREN_EXPORT int vren_bucket_sort(vren_ctx* ctx, vren_stream_t stream,
         return VREN_OK;
     }
     const uint32_t bits = ilog2_u64(n_buckets);
+    if (bits == 0) { // one bucket: the pairs in order, starting at 0
+        VREN_HIP(ctx, hipMemcpyAsync(out_pairs, in_pairs, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+        bucket_fill_kernel<<<1, 256, 0, s>>>(bucket_offsets, 1u, 0u);
+        VREN_LAUNCH_CHECK(ctx);
+        return VREN_OK;
+    }
     // bits of the key at or above 32 are zero in the bucket number and need no sorting
     const uint32_t end_bit = std::min<uint32_t>(32u, key_shift + bits);
     const size_t rs_bytes = vren_radix_sort_scratch_bytes(n, 1);
     int st = vren_ensure_scratch(ctx, rs_bytes, s);
     if (st) return st;
-    st = vren_radix_sort_pairs_interleaved(ctx, s, in_pairs, out_pairs, n, key_shift, end_bit, ctx->scratch,
-                                           rs_bytes);
-    if (st) return st;
-    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)ctx->num_cus * 16, (n + 255) / 256));
-    bucket_offsets_kernel<<<blocks, 256, 0, s>>>(out_pairs, n, key_shift, n_buckets - 1, n_buckets, bucket_offsets);
-    VREN_LAUNCH_CHECK(ctx);
-    return VREN_OK;
+    return vren_radix_sort_pairs_interleaved(ctx, s, in_pairs, out_pairs, n, key_shift, end_bit, ctx->scratch, rs_bytes,
+                                             bucket_offsets, n_buckets);
 }

@@ -181,7 +181,19 @@ constexpr bool rs_sequential_staging()
 // digit-major tile counts of rs_upsweep_kernel), no tickets and no waiting.
 // IN_PAIRS: keys_in holds interleaved (key, value) pairs and vals_in is unused. OUT_PAIRS: the pass writes
 // interleaved pairs to keys_out and vals_out is unused. Both need VALS.
-template <int BITS, int KPT, int WAVES, bool VALS, bool RTS = false, bool IN_PAIRS = false, bool OUT_PAIRS = false>
+// Bucket starts written by a final pass over interleaved pairs (OUT_PAIRS and BSTARTS): bucket(k) = (k >> shift) &
+// mask. The staged keys of a tile are sorted by the whole bucket (the pass digit is the bucket's top bits and the
+// earlier pass left each digit run sorted by the rest), so a staged key whose staged predecessor has the same pass
+// digit has that key as its predecessor in the output too, and writes offsets[c] = its position for the buckets c in
+// (bucket(predecessor), bucket(key)]. The first key of each tile's digit run and the buckets past the last key are
+// left to bucket_run_starts_kernel, which reads them from the output.
+struct BucketStarts {
+    uint32_t* offsets;
+    uint32_t shift, mask;
+};
+
+template <int BITS, int KPT, int WAVES, bool VALS, bool RTS = false, bool IN_PAIRS = false, bool OUT_PAIRS = false,
+          bool BSTARTS = false>
 __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                                  const uint32_t* __restrict__ vals_in,
                                                                  uint32_t* __restrict__ keys_out,
@@ -189,7 +201,8 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
                                                                  uint32_t shift, uint32_t width,
                                                                  const uint32_t* __restrict__ digit_base,
                                                                  uint32_t* status, uint32_t* counter,
-                                                                 uint32_t* __restrict__ clear_next)
+                                                                 uint32_t* __restrict__ clear_next,
+                                                                 const BucketStarts bs)
 {
     constexpr int THREADS = WAVES * 64;
     constexpr int RADIX = 1 << BITS;
@@ -197,6 +210,7 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
     constexpr bool SEQ = rs_sequential_staging<KPT, WAVES, VALS>();
     static_assert(THREADS >= RADIX, "one thread per digit");
     static_assert(VALS || !(IN_PAIRS || OUT_PAIRS), "interleaved pairs carry values");
+    static_assert(OUT_PAIRS || !BSTARTS, "bucket starts come from a pass that writes pairs");
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[(VALS && !SEQ) ? TILE : 1];
     __shared__ uint32_t s_whist[WAVES * RADIX];
@@ -334,6 +348,18 @@ __global__ __launch_bounds__(WAVES * 64) void rs_onesweep_kernel(const uint32_t*
             k[j] = s_keys[tid + j * THREADS];
             o[j] = s_goff[__builtin_amdgcn_ubfe(k[j], shift, width)] + tid + j * THREADS;
         }
+        if constexpr (BSTARTS) {
+            // the staged predecessor, read before the staging is reused for values
+#pragma unroll
+            for (int j = 0; j < KPT; j++) {
+                const uint32_t i = tid + j * THREADS;
+                if (i == 0 || !(full || i < tile_count)) continue;
+                const uint32_t kp = s_keys[i - 1];
+                if (__builtin_amdgcn_ubfe(kp, shift, width) != __builtin_amdgcn_ubfe(k[j], shift, width)) continue;
+                const uint32_t b1 = (k[j] >> bs.shift) & bs.mask;
+                for (uint32_t c = ((kp >> bs.shift) & bs.mask) + 1u; c <= b1; c++) bs.offsets[c] = o[j];
+            }
+        }
         if (SEQ) {
             __syncthreads();
 #pragma unroll
@@ -452,10 +478,10 @@ int run_sort_rts(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, u
         if (st) return st;
         if (vals) {
             rs_onesweep_kernel<BITS, KPT, WAVES, true, true><<<tiles, WAVES * 64, 0, s>>>(
-                ksrc, vsrc, kdst, vdst, n, shift, width, tile_hist, nullptr, nullptr, nullptr);
+                ksrc, vsrc, kdst, vdst, n, shift, width, tile_hist, nullptr, nullptr, nullptr, BucketStarts{});
         } else {
             rs_onesweep_kernel<BITS, KPT, WAVES, false, true><<<tiles, WAVES * 64, 0, s>>>(
-                ksrc, nullptr, kdst, nullptr, n, shift, width, tile_hist, nullptr, nullptr, nullptr);
+                ksrc, nullptr, kdst, nullptr, n, shift, width, tile_hist, nullptr, nullptr, nullptr, BucketStarts{});
         }
         VREN_LAUNCH_CHECK(ctx);
         std::swap(ksrc, kdst);
@@ -497,10 +523,12 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
         uint32_t* clear_next = p + 1 < passes ? st[(p + 1) & 1] : nullptr;
         if (vals) {
             rs_onesweep_kernel<BITS, KPT, WAVES, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
-                ksrc, vsrc, kdst, vdst, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next);
+                ksrc, vsrc, kdst, vdst, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next,
+                BucketStarts{});
         } else {
             rs_onesweep_kernel<BITS, KPT, WAVES, false><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
-                ksrc, nullptr, kdst, nullptr, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next);
+                ksrc, nullptr, kdst, nullptr, n, shift, width, hist + p * RADIX, st[p & 1], counters + p, clear_next,
+                BucketStarts{});
         }
         VREN_LAUNCH_CHECK(ctx);
         std::swap(ksrc, kdst);
@@ -513,12 +541,41 @@ int run_sort(vren_ctx* ctx, hipStream_t s, uint32_t* keys, uint32_t* vals, uint6
     return VREN_OK;
 }
 
+// The bucket starts that the final pass leaves out: the first key of every tile's digit run (its output predecessor
+// belongs to another tile or digit) and the buckets past the last key. Run (t, d) starts at digit_base[d] plus the
+// inclusive count of digit d over tiles < t, which the pass's look-back states hold; one thread per (tile, digit).
+__global__ __launch_bounds__(256) void bucket_run_starts_kernel(const uint32_t* __restrict__ pairs, uint64_t n,
+                                                                const uint32_t* __restrict__ status,
+                                                                const uint32_t* __restrict__ digit_base, uint32_t tiles,
+                                                                uint32_t radix, const BucketStarts bs,
+                                                                uint32_t n_buckets)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (uint64_t)tiles * radix) return;
+    const uint32_t t = (uint32_t)(g / radix), d = (uint32_t)(g - (uint64_t)t * radix);
+    const uint32_t before = t == 0 ? 0u : status[(uint64_t)(t - 1) * radix + d] & RS_MASK;
+    const uint32_t upto = status[(uint64_t)t * radix + d] & RS_MASK;
+    if (upto > before) {
+        const uint64_t q = (uint64_t)digit_base[d] + before;
+        if (q < n) { // a run of real keys (the last tile's padding counts but is not in the output)
+            const uint32_t bq = (pairs[2 * q] >> bs.shift) & bs.mask;
+            const uint32_t first = q == 0 ? 0u : ((pairs[2 * q - 2] >> bs.shift) & bs.mask) + 1u;
+            for (uint32_t c = first; c <= bq; c++) bs.offsets[c] = (uint32_t)q;
+        }
+    }
+    if (g == 0) {
+        const uint32_t bl = (pairs[2 * (n - 1)] >> bs.shift) & bs.mask;
+        for (uint32_t c = bl + 1u; c < n_buckets; c++) bs.offsets[c] = (uint32_t)n;
+    }
+}
+
 // stable sort of interleaved (key, value) pairs on bits [begin_bit, end_bit), at most two digit passes: the first
 // pass reads the pairs and the last one writes them (a single pass does both); between two passes the keys and
-// values sit split in the scratch. Per pair: 8 B histogram read + 16 B per pass.
+// values sit split in the scratch. Per pair: 8 B histogram read + 16 B per pass. With bucket_offsets, the final
+// pass and bucket_run_starts_kernel also write the start of every bucket (k >> begin_bit) & (n_buckets - 1).
 template <int BITS, int KPT, int WAVES>
 int run_sort_pairs(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t begin_bit,
-                   uint32_t end_bit, char* scratch, const Layout& L)
+                   uint32_t end_bit, char* scratch, const Layout& L, uint32_t* bucket_offsets, uint32_t n_buckets)
 {
     constexpr int RADIX = 1 << BITS;
     constexpr int TILE = WAVES * 64 * KPT;
@@ -539,19 +596,34 @@ int run_sort_pairs(vren_ctx* ctx, hipStream_t s, const uint32_t* in, uint32_t* o
     VREN_LAUNCH_CHECK(ctx);
     rs_hexscan_kernel<BITS><<<1, 256, 0, s>>>(hist, passes);
     VREN_LAUNCH_CHECK(ctx);
+    const BucketStarts bs{bucket_offsets, begin_bit, n_buckets ? n_buckets - 1u : 0u};
     if (passes == 1) {
-        rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
-            in, nullptr, out, nullptr, n, begin_bit, end_bit - begin_bit, hist, st[0], counters, nullptr);
+        if (bucket_offsets)
+            rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, true, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                in, nullptr, out, nullptr, n, begin_bit, end_bit - begin_bit, hist, st[0], counters, nullptr, bs);
+        else
+            rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                in, nullptr, out, nullptr, n, begin_bit, end_bit - begin_bit, hist, st[0], counters, nullptr, bs);
         VREN_LAUNCH_CHECK(ctx);
-        return VREN_OK;
+    } else {
+        const uint32_t shift1 = begin_bit + BITS;
+        rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, false><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+            in, nullptr, tk, tv, n, begin_bit, BITS, hist, st[0], counters, st[1], BucketStarts{});
+        VREN_LAUNCH_CHECK(ctx);
+        if (bucket_offsets)
+            rs_onesweep_kernel<BITS, KPT, WAVES, true, false, false, true, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                tk, tv, out, nullptr, n, shift1, end_bit - shift1, hist + RADIX, st[1], counters + 1, nullptr, bs);
+        else
+            rs_onesweep_kernel<BITS, KPT, WAVES, true, false, false, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
+                tk, tv, out, nullptr, n, shift1, end_bit - shift1, hist + RADIX, st[1], counters + 1, nullptr, bs);
+        VREN_LAUNCH_CHECK(ctx);
     }
-    const uint32_t shift1 = begin_bit + BITS;
-    rs_onesweep_kernel<BITS, KPT, WAVES, true, false, true, false><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
-        in, nullptr, tk, tv, n, begin_bit, BITS, hist, st[0], counters, st[1]);
-    VREN_LAUNCH_CHECK(ctx);
-    rs_onesweep_kernel<BITS, KPT, WAVES, true, false, false, true><<<(uint32_t)tiles, WAVES * 64, 0, s>>>(
-        tk, tv, out, nullptr, n, shift1, end_bit - shift1, hist + RADIX, st[1], counters + 1, nullptr);
-    VREN_LAUNCH_CHECK(ctx);
+    if (bucket_offsets) {
+        const uint64_t threads = tiles * RADIX;
+        bucket_run_starts_kernel<<<(uint32_t)ceil_div_u64(threads, 256), 256, 0, s>>>(
+            out, n, st[(passes - 1) & 1], hist + (uint64_t)(passes - 1) * RADIX, (uint32_t)tiles, RADIX, bs, n_buckets);
+        VREN_LAUNCH_CHECK(ctx);
+    }
     return VREN_OK;
 }
 
@@ -598,18 +670,24 @@ int vren_radix_sort_with_config(vren_ctx* ctx, hipStream_t s, int cfg, uint32_t*
 
 int vren_radix_sort_pairs_interleaved(vren_ctx* ctx, hipStream_t s, const uint32_t* in_pairs, uint32_t* out_pairs,
                                       uint64_t n, uint32_t begin_bit, uint32_t end_bit, void* scratch,
-                                      size_t scratch_bytes)
+                                      size_t scratch_bytes, uint32_t* bucket_offsets, uint32_t n_buckets)
 {
     Layout L = layout_for(n, true, kMinTile, 256);
     if (scratch_bytes < L.total)
         return vren_fail(ctx, VREN_EINVAL, "radix_sort: scratch of %zu bytes, %zu required", scratch_bytes, L.total);
     if (n == 0) return VREN_OK;
     if (begin_bit >= end_bit) {
+        if (bucket_offsets) return vren_fail(ctx, VREN_EINVAL, "radix_sort: bucket starts need a bit range");
         VREN_HIP(ctx, hipMemcpyAsync(out_pairs, in_pairs, n * 8, hipMemcpyDeviceToDevice, s));
         return VREN_OK;
     }
-    // the default configuration's tile (8-bit digits, 23 keys per thread, 16 waves)
-    return run_sort_pairs<8, 23, 16>(ctx, s, in_pairs, out_pairs, n, begin_bit, end_bit, static_cast<char*>(scratch), L);
+    // the default configuration's tile (8-bit digits, 23 keys per thread, 16 waves); below 2^21 pairs, 11 keys per
+    // thread (more tiles to spread over the CUs: 2^20 pairs in 54 instead of 73 us for 65536 buckets)
+    if (n < (1u << 21))
+        return run_sort_pairs<8, 11, 16>(ctx, s, in_pairs, out_pairs, n, begin_bit, end_bit,
+                                         static_cast<char*>(scratch), L, bucket_offsets, n_buckets);
+    return run_sort_pairs<8, 23, 16>(ctx, s, in_pairs, out_pairs, n, begin_bit, end_bit, static_cast<char*>(scratch), L,
+                                     bucket_offsets, n_buckets);
 }
 
 extern "C" {

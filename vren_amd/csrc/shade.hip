@@ -68,13 +68,23 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
                         fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
         }
         w = r[3];
-        z = r[2] / w;
-        o.shaded = z < (double)fd.bg_threshold;
     }
-    // 1 / w for the x and y coordinates: hardware reciprocal and two Newton steps (~1 ulp, double)
+    // 1 / w: hardware reciprocal and two Newton steps (within ~1 ulp, double)
     double iw = __builtin_amdgcn_rcp(w);
     iw = iw + iw * (1.0 - w * iw);
     iw = iw + iw * (1.0 - w * iw);
+    // the view depth r[2] / w: the product r[2] * iw is within 2 ulp of the correctly rounded quotient the oracle
+    // takes, so it decides the background test whenever it lies more than ~45 ulp from the threshold; otherwise
+    // (and for a non-finite product, whose test is false) the test takes the IEEE division itself
+    const double bg = (double)fd.bg_threshold;
+    z = r[2] * iw;
+    const bool near = !(__builtin_fabs(z - bg) > 1e-14 * __builtin_fabs(bg));
+    if (__builtin_amdgcn_ballot_w64(near)) { // a wave-uniform branch: the division is skipped by almost every wave
+#pragma clang fp contract(off)
+        const double zd = r[2] / w;
+        z = near ? zd : z;
+    }
+    o.shaded = z < bg;
     const double q[4] = {r[0] * iw, r[1] * iw, z, 1.0};
     for (int i = 0; i < 3; i++)
         o.P[i] = ((fd.inv_view_d[0 * 4 + i] * q[0] + fd.inv_view_d[1 * 4 + i] * q[1]) + fd.inv_view_d[2 * 4 + i] * q[2]) +
@@ -128,19 +138,67 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
     }
 }
 
+// per-pixel constants of the forward shade, arranged for shade_light
+struct ShadeSurface {
+    float V[3], N[3];
+    float A[3];   // albedo (1 - metallic) / PI: the diffuse term of a light is (1 - F) A
+    float F0[3];
+    float k;      // roughness, the Schlick-GGX k (pbr.glsl:26-40)
+    float nn;     // 1 - |N|^2
+    float a2;     // roughness^2
+    float a2ggxV; // roughness^2 n.v / (n.v (1 - k) + k)
+    float NdotV4; // 4 n.v
+};
+
+// pbr_apply_light (pbr.glsl:42-73) for a light direction L (from the light towards the surface), the light's
+// colour and its attenuation: acc += max(((1 - F) A + F spec) colour att n.l, 0) per channel, with 1 - F taken as
+// (1 - F0)(1 - f5), which it equals (F = F0 + (1 - F0) f5), without the cancellation of 1 - F
+__device__ __forceinline__ void shade_light(const ShadeSurface& s, const float L[3], const float col[3], float att,
+                                            float acc[3])
+{
+    const float mL[3] = {-L[0], -L[1], -L[2]};
+    const float NdotL = fmaxf(dot3(s.N, mL), 0.0f);
+    if (NdotL <= 0.0f) return;
+    const float h[3] = {s.V[0] + mL[0], s.V[1] + mL[1], s.V[2] + mL[2]};
+    const float hh = dot3(h, h);
+    const float Nh = dot3(s.N, h);
+    const float irh = hh > 0.0f ? __builtin_amdgcn_rsqf(hh) : 0.0f; // 1 / |h|
+    float inner = 1.0f;
+    if (Nh > 0.0f && hh > 0.0f) {
+        const float c[3] = {s.N[1] * h[2] - s.N[2] * h[1], s.N[2] * h[0] - s.N[0] * h[2], s.N[0] * h[1] - s.N[1] * h[0]};
+        const float inv_hh = irh * irh;
+        inner = s.nn + dot3(c, c) * inv_hh + s.a2 * (Nh * Nh) * inv_hh;
+    }
+    const float HdotV = fmaxf(dot3(h, s.V) * irh, 0.0f);
+    const float om = 1.0f - HdotV;
+    const float om2 = om * om;
+    const float f5 = om2 * om2 * om;
+    const float om5 = 1.0f - f5;
+    // spec = NDF G / (4 n.v n.l + 0.0001) with NDF = a^2 / (PI inner^2) and G = ggxV n.l / (n.l (1 - k) + k):
+    // the three quotients share one reciprocal
+    const float spec = (s.a2ggxV * NdotL) *
+                       frcp(kPi * inner * inner * (NdotL * (1.0f - s.k) + s.k) * (s.NdotV4 * NdotL + 0.0001f));
+    const float w = att * NdotL;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float G = 1.0f - s.F0[c];
+        const float F = s.F0[c] + G * f5;
+        acc[c] += fmaxf(((G * om5) * s.A[c] + F * spec) * (col[c] * w), 0.0f);
+    }
+}
+
 constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup
 constexpr int kShadePPT = 1;     // pixels per thread of the forward shade (2 measured slower)
 
 // The pixel's list entries [off, end): light records from the LDS stage (STAGED, entry e at e - start, the
 // position's w holding 1 / radius) or from global memory; SMOOTH selects the windowed attenuation.
 template <bool STAGED, bool SMOOTH>
-__device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px, const Surface& s, uint32_t off,
+__device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px, const ShadeSurface& s, uint32_t off,
                                            uint32_t end, uint32_t start, const float4* s_lpos, const float4* s_lcol,
                                            float acc[3])
 {
     const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
     const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
-    const float Pf[3] = {(float)px.P[0], (float)px.P[1], (float)px.P[2]};
     for (uint32_t e = off; e < end; e++) {
         float4 lp, lc;
         if constexpr (STAGED) {
@@ -152,12 +210,6 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
             lp = lpos[li];
             lc = lcol[li];
             lp.w = frcp(lc.w);
-        }
-        // lights clearly beyond their radius contribute exactly 0: a float distance (error < 1e-3 at scene
-        // scale) rejects them before the double evaluation
-        {
-            const float fx = Pf[0] - lp.x, fy = Pf[1] - lp.y, fz = Pf[2] - lp.z;
-            if (fx * fx + fy * fy + fz * fz > lc.w * lc.w * 1.0001f + 1e-3f) continue;
         }
         // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
         // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
@@ -186,8 +238,8 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
         }
         if (!(att > 0.0f)) continue;
         const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
-        const float rad[3] = {lc.x * att, lc.y * att, lc.z * att};
-        apply_light(s, L, rad, acc);
+        const float col[3] = {lc.x, lc.y, lc.z};
+        shade_light(s, L, col, att, acc);
     }
 }
 
@@ -197,7 +249,7 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
                                             bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
                                             const float4* s_lpos, const float4* s_lcol, uint64_t p)
 {
-    Surface s;
+    ShadeSurface s;
     {
         const float V[3] = {(float)(fd.eye[0] - px.P[0]), (float)(fd.eye[1] - px.P[1]), (float)(fd.eye[2] - px.P[2])};
         const float il = __builtin_amdgcn_rsqf(dot3(V, V)); // |V| is scene-sized, never subnormal
@@ -207,17 +259,20 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     s.N[1] = normal_h2f(nb.x >> 16);
     s.N[2] = normal_h2f(nb.y & 0xFFFFu);
     s.nn = (float)(1.0 - ((double)s.N[0] * s.N[0] + (double)s.N[1] * s.N[1] + (double)s.N[2] * s.N[2]));
-    s.albedo[0] = ar.x;
-    s.albedo[1] = ar.y;
-    s.albedo[2] = ar.z;
-    s.rough = ar.w;
-    s.metal = metal;
-    s.a2 = s.rough * s.rough;
-    for (int c = 0; c < 3; c++) s.F0[c] = 0.04f * (1.0f - s.metal) + s.albedo[c] * s.metal;
-    s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
-    s.ggxV = s.NdotV * frcp(s.NdotV * (1.0f - s.rough) + s.rough);
+    const float rough = ar.w, km = 1.0f - metal;
+    s.k = rough;
+    s.a2 = rough * rough;
+    const float albedo[3] = {ar.x, ar.y, ar.z};
+    for (int c = 0; c < 3; c++) {
+        s.F0[c] = 0.04f * km + albedo[c] * metal;
+        s.A[c] = albedo[c] * km * kInvPi;
+    }
+    const float NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
+    s.a2ggxV = s.a2 * (NdotV * frcp(NdotV * (1.0f - rough) + rough));
+    s.NdotV4 = 4.0f * NdotV;
 
-    float acc[3] = {0.0f, 0.0f, 0.0f};
+    // the ambient term (shade.comp:133) first, the lights added to it
+    float acc[3] = {0.03f * albedo[0], 0.03f * albedo[1], 0.03f * albedo[2]};
     if (has_list) {
         // one loop per (staging, attenuation) pair: both are uniform over the workgroup, so no entry pays for the
         // other branch's registers or its test
@@ -234,16 +289,16 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
         const float4* D = reinterpret_cast<const float4*>(fd.dir_lights) + 2 * i;
         const float4 dir = D[0], col = D[1];
         const float L[3] = {dir.x, dir.y, dir.z};
-        const float rad[3] = {col.x, col.y, col.z};
-        apply_light(s, L, rad, acc);
+        const float c3[3] = {col.x, col.y, col.z};
+        shade_light(s, L, c3, 1.0f, acc);
     }
     // gamma (shade.comp:135-137): c^(1/2.2) = 2^(log2(c) / 2.2) on the transcendental unit (v_log_f32 /
     // v_exp_f32, ~1 ulp each: ~1e-7 relative on the result); c >= 0, and c = 0 gives 2^-inf = 0
     const float gexp = 1.0f / 2.2f;
     float4 o;
-    o.x = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[0] + acc[0]) * gexp);
-    o.y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[1] + acc[1]) * gexp);
-    o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(0.03f * s.albedo[2] + acc[2]) * gexp);
+    o.x = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(acc[0]) * gexp);
+    o.y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(acc[1]) * gexp);
+    o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(acc[2]) * gexp);
     o.w = 1.0f;
     const vf4 ov = {o.x, o.y, o.z, o.w};
     *((__attribute__((address_space(1))) vf4*)fd.out + p) = ov;
