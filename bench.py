@@ -400,6 +400,10 @@ def bench_config4(ctx, vren_amd, args):
     torch.cuda.synchronize()
     steps = max(3, args.steps // 2)
     ms_b, _ = time_events(lambda: b.run(b.GRAPH), steps)
+    acc = [0.0] * len(KERNELS)
+    for _ in range(3):
+        acc = [a + m for a, m in zip(acc, b.run_profiled())]
+    kernel_ms = {k: round(a / 3, 5) for k, a in zip(KERNELS, acc)}
     b.run(b.PER_FRAME)
     _, wall_p = time_events(lambda: b.run(b.PER_FRAME), 2)
     ctx.sync_status()
@@ -410,7 +414,7 @@ def bench_config4(ctx, vren_amd, args):
           file=sys.stderr, flush=True)
     return {"frames": 64, "pixels": total_px, "frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / steps, 4),
             "plain_frames_per_s": round(plain, 2), "batched_over_plain": round(fps / plain, 3),
-            "megapixels_per_s": round(total_px * steps / (ms_b * 1e-3) / 1e6, 1)}
+            "megapixels_per_s": round(total_px * steps / (ms_b * 1e-3) / 1e6, 1), "kernel_ms": kernel_ms}
 
 
 def bench_shade_frame(ctx, vren_amd, reps):

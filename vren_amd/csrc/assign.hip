@@ -7,7 +7,7 @@
 // Here: persistent kernels over the clusters of every frame (their number is read on the device; it never returns
 // to the host), taken in chunks of up to 64 whose per-cluster work (the cluster AABB, the list bounds) is done one
 // cluster per lane. COUNT runs two traversals per wave, one per 32-lane half (the BVH's fan-out), each from its own
-// LDS stack, and keeps lists of at most 16 lights, sorted by rank, in per-cluster slots; one workgroup per frame
+// LDS stack, and keeps lists of at most 64 lights, sorted by rank, in per-cluster slots; one workgroup per frame
 // scans the counts into offsets; WRITE copies the kept lists and traverses again only for longer ones, with the
 // whole wave testing two BVH blocks per step, one bit per hit light in a per-wave LDS bitmask, emitted in ascending
 // light index (the canonical order) by a popcount prefix over the bitmask words.
@@ -19,6 +19,10 @@
 namespace {
 
 constexpr uint32_t kStack = 384; // per-wave stack entries (levels are nondecreasing on the stack: <= 64 per level)
+// per-half stack entries of the counting pass: a half pops one block and pushes at most 32, so the stack holds at
+// most 31 entries of each level above the deepest and 32 of the deepest, 31 * 3 + 32 = 125 for the four block
+// levels of 2^20 lights
+constexpr uint32_t kStackHalf = 192;
 constexpr uint32_t kLevelShift = 28;
 constexpr uint32_t kAddrMask = (1u << kLevelShift) - 1u;
 
@@ -114,9 +118,10 @@ __device__ __forceinline__ bool test_leaf(const Box& c, const float4& o, float l
     return vren_sqrtf(vren_dot3(d, d)) < r;
 }
 
-// Traverse the BVH for one cluster with the whole wave, setting one bit per hit light in the wave's LDS bitmask, and
-// return the number of hit lights.
-__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* bitmask)
+// Traverse the BVH for one cluster with the whole wave, setting one bit per hit light in the wave's LDS bitmask (and
+// the word's bit in the summary, one bit per bitmask word), and return the number of hit lights.
+__device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack, uint32_t* bitmask,
+                             uint32_t* summary)
 {
     const uint32_t lane = lane_id();
     const uint32_t sub = lane & 31u;
@@ -167,7 +172,10 @@ __device__ uint32_t traverse(const FrameDev& fd, const Box& box, uint32_t* stack
                 if (next == VREN_BVH_LEAF_NODE && at_leaf && li < fd.L) {
                     if (!fd.leaf_pos) lp = vpos[li];
                     leaf_hit = test_leaf(box, lp, lo.x, hi.x);
-                    if (leaf_hit) atomicOr(&bitmask[li >> 5], 1u << (li & 31u));
+                    if (leaf_hit) {
+                        atomicOr(&bitmask[li >> 5], 1u << (li & 31u));
+                        atomicOr(&summary[li >> 10], 1u << ((li >> 5) & 31u));
+                    }
                 }
             } else if (next != VREN_BVH_INVALID_NODE && next != VREN_BVH_LEAF_NODE) {
                 push = test_aabb_aabb(box, lo, hi);
@@ -269,9 +277,9 @@ __device__ uint32_t traverse_halves(const TravFrame& tf, const Box& box, bool li
         const uint64_t pm = ballot(push) & half;
         if (push) {
             const uint32_t r = mbcnt(pm);
-            if (sp + r < kStack) stack[sp + r] = child;
+            if (sp + r < kStackHalf) stack[sp + r] = child;
         }
-        sp = min(sp + (uint32_t)__popcll(pm), kStack); // the level bound keeps it far below kStack
+        sp = min(sp + (uint32_t)__popcll(pm), kStackHalf); // the level bound keeps it below kStackHalf
         __builtin_amdgcn_wave_barrier();
     }
     return hits;
@@ -369,24 +377,33 @@ __device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restri
 
 
 // Emits the lights set in a wave's bitmask in ascending index to dst[0 ..) (nothing when dst is null), clearing
-// the mask. Returns the number emitted.
-__device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t words, uint32_t* dst, uint32_t cap)
+// the bitmask and its summary, and returns their number. Only the words the summary marks are read: lane l of a
+// step takes summary word s0 + l (bitmask words 32 (s0 + l) ..), counts its lights, and writes them after the
+// lights of the lanes before it, so a cluster's list costs its hit words, not the L / 32 words of the mask.
+__device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t* summary, uint32_t summary_words,
+                                                 uint32_t* dst, uint32_t cap)
 {
     const uint32_t lane = lane_id();
     uint32_t out = 0;
-    for (uint32_t w0 = 0; w0 < words; w0 += VREN_WAVE) {
-        const uint32_t wi = w0 + lane;
-        uint32_t bits = wi < words ? bitmask[wi] : 0u;
-        if (wi < words && bits) bitmask[wi] = 0u;
-        const uint32_t cnt = (uint32_t)__popc(bits);
+    for (uint32_t s0 = 0; s0 < summary_words; s0 += VREN_WAVE) {
+        const uint32_t si = s0 + lane;
+        const uint32_t sw = si < summary_words ? summary[si] : 0u;
+        if (sw) summary[si] = 0u;
+        uint32_t cnt = 0;
+        for (uint32_t m = sw; m; m &= m - 1u) cnt += (uint32_t)__popc(bitmask[si * 32u + __builtin_ctz(m)]);
         const uint32_t incl = wave_inclusive_sum(cnt);
         uint32_t pos = out + incl - cnt;
-        if (dst) {
-            while (bits) {
-                const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
-                if (pos < cap) dst[pos] = wi * 32u + b;
-                pos++;
+        for (uint32_t m = sw; m; m &= m - 1u) {
+            const uint32_t wi = si * 32u + (uint32_t)__builtin_ctz(m);
+            uint32_t bits = bitmask[wi];
+            bitmask[wi] = 0u;
+            if (dst) {
+                while (bits) {
+                    const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    if (pos < cap) dst[pos] = wi * 32u + b;
+                    pos++;
+                }
             }
         }
         out += shfl(incl, VREN_WAVE - 1);
@@ -406,10 +423,10 @@ __global__ __launch_bounds__(256, 8) void assign_count_kernel(const FrameDev* __
     const uint32_t wave = wave_id();
     const uint32_t lane = lane_id();
     const uint32_t hb = lane < 32 ? 0u : 1u; // half of the wave
-    uint32_t* stack = s_mem + (wave * 2 + hb) * kStack;
-    uint32_t* list = s_mem + wpb * 2 * kStack + (wave * 2 + hb) * VREN_LIST_SLOT;
-    Box* boxes = reinterpret_cast<Box*>(s_mem + wpb * 2 * (kStack + VREN_LIST_SLOT)) + wave * VREN_WAVE;
-    uint32_t* cl_frame = s_mem + wpb * 2 * (kStack + VREN_LIST_SLOT) + wpb * VREN_WAVE * 6 + wave * VREN_WAVE;
+    uint32_t* stack = s_mem + (wave * 2 + hb) * kStackHalf;
+    uint32_t* list = s_mem + wpb * 2 * kStackHalf + (wave * 2 + hb) * VREN_LIST_SLOT;
+    Box* boxes = reinterpret_cast<Box*>(s_mem + wpb * 2 * (kStackHalf + VREN_LIST_SLOT)) + wave * VREN_WAVE;
+    uint32_t* cl_frame = s_mem + wpb * 2 * (kStackHalf + VREN_LIST_SLOT) + wpb * VREN_WAVE * 6 + wave * VREN_WAVE;
     const uint32_t gw = blockIdx.x * wpb + wave;
     // the chunk's cluster AABBs and frames wait in LDS (one per lane) while the wave traverses two at a time
     for_each_cluster_chunk2(
@@ -429,12 +446,14 @@ __global__ __launch_bounds__(256, 8) void assign_count_kernel(const FrameDev* __
             const uint32_t h = traverse_halves(trav_frame(fd), box, live, stack, list);
             __builtin_amdgcn_wave_barrier();
             if (live && fd.list_slots && c < fd.slot_cap && h <= VREN_LIST_SLOT) {
-                // ascending light index: each entry's rank among the (distinct) entries of the half's list
-                const uint32_t sub = lane & 31u;
-                const uint32_t li = sub < h ? list[sub] : 0xFFFFFFFFu;
-                uint32_t rank = 0;
-                for (uint32_t j = 0; j < h; j++) rank += list[j] < li ? 1u : 0u;
-                if (sub < h) fd.list_slots[(size_t)c * VREN_LIST_SLOT + rank] = li;
+                // ascending light index: each entry's rank among the (distinct) entries of the half's list, entries
+                // sub and sub + 32 on lane sub of the half
+                for (uint32_t q = lane & 31u; q < h; q += 32u) {
+                    const uint32_t li = list[q];
+                    uint32_t rank = 0;
+                    for (uint32_t j = 0; j < h; j++) rank += list[j] < li ? 1u : 0u;
+                    fd.list_slots[(size_t)c * VREN_LIST_SLOT + rank] = li;
+                }
             }
             if (live && (lane & 31u) == 0) fd.counts[c] = h;
             __builtin_amdgcn_wave_barrier();
@@ -487,9 +506,12 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
     const uint32_t wpb = blockDim.x / VREN_WAVE;
     const uint32_t wave = wave_id();
     const uint32_t lane = lane_id();
+    const uint32_t summary_words = (mask_words + 31) / 32;
     uint32_t* stack = s_mem + wave * kStack;
     uint32_t* bitmask = s_mem + wpb * kStack + wave * mask_words;
+    uint32_t* summary = s_mem + wpb * (kStack + mask_words) + wave * summary_words;
     for (uint32_t i = lane; i < mask_words; i += VREN_WAVE) bitmask[i] = 0u;
+    for (uint32_t i = lane; i < summary_words; i += VREN_WAVE) summary[i] = 0u;
     __builtin_amdgcn_wave_barrier();
     const uint32_t gw = blockIdx.x * wpb + wave;
     for_each_cluster_chunk<ListBounds>(
@@ -505,11 +527,11 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
                 return;
             }
             const Box box = cluster_aabb(fd, fd.keys[c]);
-            traverse(fd, box, stack, bitmask);
+            traverse(fd, box, stack, bitmask, summary);
             __builtin_amdgcn_wave_barrier();
             // emit set bits in ascending light index, clearing the mask as we go
             const uint32_t room = out < fd.index_cap ? fd.index_cap - out : 0u;
-            emit_bitmask(bitmask, (fd.L + 31) / 32, fd.indices + out, room);
+            emit_bitmask(bitmask, summary, ((fd.L + 31) / 32 + 31) / 32, fd.indices + out, room);
             __builtin_amdgcn_wave_barrier();
         });
 }
@@ -523,12 +545,12 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
 {
     if (w.max_L == 0) return VREN_OK;
     const uint32_t mask_words = (w.max_L + 31) / 32;
-    const size_t per_wave = (size_t)(kStack + mask_words) * 4;
+    const size_t per_wave = (size_t)(kStack + mask_words + (mask_words + 31) / 32) * 4;
     uint32_t wpb = 4;
     while (wpb > 1 && per_wave * wpb > 150 * 1024) wpb--;
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
     // two stacks and two lists per wave (one per half), the chunk's boxes and cluster frames
-    const size_t count_per_wave = (size_t)2 * (kStack + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
+    const size_t count_per_wave = (size_t)2 * (kStackHalf + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
     assign_count_kernel<<<persistent_blocks(ctx, 4), 4 * VREN_WAVE, count_per_wave * 4, s>>>(d_frames, F, mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);

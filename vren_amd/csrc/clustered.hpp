@@ -9,7 +9,7 @@
 #include "common.hpp"
 
 // light lists of at most this many entries are kept by the counting traversal and copied by the writing pass
-#define VREN_LIST_SLOT 16u
+#define VREN_LIST_SLOT 64u
 
 struct FrameDev {
     // sizes
