@@ -28,21 +28,25 @@ def _err(got, ref):
     return np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), SHADE_ATOL / SHADE_RTOL)
 
 
-def make_materials(seed, n_materials=8):
+def make_materials(seed, n_materials=8, n_textures=6):
     """Textures of assorted sizes in every (filter, address mode) pairing and n_materials materials over them. The
-    metallic-roughness textures keep green >= 40, so every in-table material has a roughness > 0."""
+    metallic-roughness textures keep green >= 40, so every in-table material has a roughness > 0. More textures
+    than 6 repeat the sizes with other (filter, address) pairings."""
     import vren_amd
     rng = np.random.default_rng(seed)
     textures = []
-    for k, (w, h) in enumerate([(256, 256), (64, 128), (7, 5), (1, 1), (33, 17), (128, 64)]):
+    sizes = [(256, 256), (64, 128), (7, 5), (1, 1), (33, 17), (128, 64)]
+    for k in range(n_textures):
+        w, h = sizes[k % 6]
         t = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
         if k % 2 == 1:
             t[..., 1] = rng.integers(40, 256, (h, w), dtype=np.uint8)
-        textures.append((t, k % 2 if k < 4 else 1 - k % 2, k % 3, (k + 1) % 3))
+        j = k % 6 + k // 6
+        textures.append((t, j % 2 if k % 6 < 4 else 1 - j % 2, j % 3, (j + 1) % 3))
     mats = (vren_amd.Material * n_materials)()
     for m in range(n_materials):
-        mats[m].base_color_texture_idx = int(rng.integers(0, 6)) // 2 * 2      # an even texture
-        mats[m].metallic_roughness_texture_idx = int(rng.integers(0, 3)) * 2 + 1  # an odd one
+        mats[m].base_color_texture_idx = int(rng.integers(0, n_textures)) // 2 * 2      # an even texture
+        mats[m].metallic_roughness_texture_idx = int(rng.integers(0, n_textures // 2)) * 2 + 1  # an odd one
         mats[m].metallic_factor = float(rng.uniform(0.0, 1.0))
         mats[m].roughness_factor = float(rng.uniform(0.3, 1.0))
         for c in range(3):
@@ -86,15 +90,17 @@ def _lists_on_gpu(vctx, co, W, H, cam, cf, depth, normals, pos, lig, key_cap=1 <
     return run_assign(vctx, co, W, H, cam, cf, depth, normals, pos, lig, key_cap, index_cap)
 
 
-def test_material_resolve_through_ambient_only_shade(vctx, co):
+@pytest.mark.parametrize("n_materials,n_textures", [(8, 6), (100, 70)])
+def test_material_resolve_through_ambient_only_shade(vctx, co, n_materials, n_textures):
     """With no lights the radiance is the ambient term alone, (0.03 albedo)^(1/2.2) (shade.comp:133-137), so every
     shaded pixel's resolved albedo can be read back from the output: it must match the FP64 resolve of its
-    texcoord and material index (texel selection, filtering, factors) to 1e-5."""
+    texcoord and material index (texel selection, filtering, factors) to 1e-5. The larger tables run past the
+    entries the kernel stages in LDS, so both the staged and the global lookups are checked."""
     import vren_amd
     W, H = 1024, 512
     cam = ol.camera(W, H)
     depth, normals = co.synth_frame(42 + 2000, W, H, cam)
-    textures, mats = make_materials(11)
+    textures, mats = make_materials(11, n_materials, n_textures)
     tc, mi = synth_texcoords(12, W, H, len(mats))
     dm = DeviceMaterials(vren_amd, tc, mi, textures, mats)
     refs = np.zeros(W * H, np.uint32)
