@@ -395,6 +395,9 @@ def bench_config4(ctx, vren_amd, args):
         total_px += w * h
     torch.cuda.synchronize()
     b = vren_amd.Batch(ctx, descs, vren_amd.default_cfg())
+    shape = None
+    if not args.no_tune:
+        shape, _ = b.autotune(reps=2)  # this batch's size class has its own shape
     for _ in range(max(1, args.warmup)):
         b.run(b.GRAPH)
     torch.cuda.synchronize()
@@ -414,7 +417,8 @@ def bench_config4(ctx, vren_amd, args):
           file=sys.stderr, flush=True)
     return {"frames": 64, "pixels": total_px, "frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / steps, 4),
             "plain_frames_per_s": round(plain, 2), "batched_over_plain": round(fps / plain, 3),
-            "megapixels_per_s": round(total_px * steps / (ms_b * 1e-3) / 1e6, 1), "kernel_ms": kernel_ms}
+            "megapixels_per_s": round(total_px * steps / (ms_b * 1e-3) / 1e6, 1), "kernel_ms": kernel_ms,
+            "chain_shape": list(shape) if shape else None}
 
 
 def bench_shade_frame(ctx, vren_amd, reps):
@@ -616,7 +620,7 @@ def main():
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--no-sort", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the radix-sort and chain-shape autotuners")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip reduce, scan, bucket sort and the shade adjoint")
     ap.add_argument("--no-key-variant", action="store_true",
@@ -647,6 +651,13 @@ def main():
     print("[bench] building %d frames of %dx%d with %d lights" % (args.frames, W, H, L_LIGHTS), file=sys.stderr,
           flush=True)
     batch, keep = build_batch(ctx, vren_amd, args.frames)
+    chain_tune = None
+    if not args.no_tune:
+        # the chain autotuner: every launch shape timed on this batch, the fastest kept (setup, not timed)
+        shape, tms = batch.autotune(reps=3)
+        chain_tune = {"shade_pixels_per_thread": shape[0], "assign_waves_per_cu": shape[1],
+                      "light_sort_bits": shape[2], "tuned_ms_per_batched_run": round(tms, 4)}
+        print("[bench] chain shape %s (%.3f ms per batched run)" % (shape, tms), file=sys.stderr, flush=True)
 
     # batched: one graph replay per step
     for _ in range(max(1, args.warmup)):
@@ -742,6 +753,7 @@ def main():
         "config_key_encoder": sliced,
         "config_material_path": material,
         "config4_heterogeneous": config4,
+        "chain_shape": chain_tune,
         "kernel_ms": kernel_ms,
         "roofline": roofline,
         "shade_valu": shade_valu,

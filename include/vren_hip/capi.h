@@ -398,6 +398,27 @@ VREN_EXPORT int vren_batch_run_profiled(vren_batch* batch, vren_stream_t stream,
 VREN_EXPORT int vren_batch_frame_outputs(vren_batch* batch, uint32_t frame, vren_frame_outputs* out);
 VREN_EXPORT void vren_batch_destroy(vren_batch* batch);
 
+/* The chain's launch shapes, picked by the chain autotuner. No reference counterpart: the reference fixes its
+ * workgroup sizes in the shaders (32x32 tiles, 1024-thread sorts, one wave32 per cluster); here a batch runs its
+ * chain with one shape, chosen per size class. Every shape gives the same bytes. Zero fields mean the defaults. */
+typedef struct vren_chain_shape {
+    uint32_t shade_pixels_per_thread; /* 1 or 2: 32x8 strips per 256-thread shade workgroup (default 1) */
+    uint32_t assign_waves_per_cu;     /* 8, 16, 24 or 32: persistent light-assignment waves per CU (default 32) */
+    uint32_t light_sort_bits;         /* 4 or 8: digit width of each frame's Morton-code sort (default 8) */
+    uint32_t reserved;
+} vren_chain_shape;
+/* the batch's current shape (the context's tuned shape for the batch's size class when one exists, else the
+ * defaults) and a forced one (invalid fields: VREN_EINVAL); a new shape drops the captured graph */
+VREN_EXPORT int vren_batch_get_shape(vren_batch* batch, vren_chain_shape* shape);
+VREN_EXPORT int vren_batch_set_shape(vren_batch* batch, const vren_chain_shape* shape);
+/* times the batched chain (VREN_RUN_BATCHED, `reps` timed runs after one warm-up) with every shape of the grid
+ * {1, 2} x {8, 16, 24, 32} x {4, 8}, keeps the fastest for the batch and caches it in the context under the batch's
+ * size class (log2 of its pixels, log2 of its largest light count, log2 of its frame count), where batches created
+ * later in that class find it; vren_autotune_save / _load keep the chain shapes beside the radix configurations.
+ * Synchronises the stream; the outputs hold the last run's results. */
+VREN_EXPORT int vren_batch_autotune(vren_batch* batch, vren_stream_t stream, uint32_t reps, vren_chain_shape* best,
+                                    float* best_ms);
+
 /* ---------------------------------------------------------------------------------------------------------
  * Depth-buffer pyramid (depth_buffer_pyramid.hpp:19-80, depth_buffer_pyramid.cpp:14-23, 177-299)
  * The Hi-Z max pyramid of a D32 depth buffer: levels 0 .. n-1, n = floor(log2(max(W, H))) + 1, level l of

@@ -564,3 +564,52 @@ def test_batch_light_count_sequence_and_animation(vctx, co):
         vctx.sync_status()
         _batch_check_all(b, keep, co, cf, "animated frame %d" % frame)
     b.close()
+
+
+def test_batch_chain_shapes_identical_and_autotuned(vctx, co, tmp_path):
+    """Every chain shape (shade strips per workgroup, persistent assignment waves per CU, light-sort digit width)
+    gives the same bytes as the default, in graph replays too; the autotuner keeps one of them for the batch and
+    caches it for the size class (a new batch over the same frames starts with it), and the choice survives an
+    autotune save / load into another context."""
+    import torch
+    import vren_amd
+    specs = [(1280, 720, 5000, 9300), (640, 360, 300, 9301), (333, 200, 1, 9302)]
+    descs, keep = make_frames_batch(vctx, co, specs)
+    b = vren_amd.Batch(vctx, descs, ol.cfg())
+    assert b.shape() == (0, 0, 0)
+    b.run(b.BATCHED)
+    torch.cuda.synchronize()
+    want = [k["o"].clone() for k in keep]
+    for shape in [(1, 8, 4), (2, 16, 8), (2, 32, 4), (1, 24, 8), (2, 8, 8)]:
+        b.set_shape(*shape)
+        assert b.shape() == shape
+        for mode in (b.BATCHED, b.GRAPH, b.GRAPH):
+            for k in keep:
+                k["o"].fill_(0xFF)
+            b.run(mode)
+            torch.cuda.synchronize()
+            for i, (a, k) in enumerate(zip(want, keep)):
+                assert torch.equal(a, k["o"]), (shape, mode, i)
+    with pytest.raises(vren_amd.VrenError):
+        b.set_shape(3, 8, 4)
+    best, ms = b.autotune(reps=1)
+    assert best[0] in (1, 2) and best[1] in (8, 16, 24, 32) and best[2] in (4, 8) and ms > 0
+    assert b.shape() == best
+    for k in keep:
+        k["o"].fill_(0xFF)
+    b.run(b.GRAPH)
+    torch.cuda.synchronize()
+    for i, (a, k) in enumerate(zip(want, keep)):
+        assert torch.equal(a, k["o"]), ("tuned", i)
+    b2 = vren_amd.Batch(vctx, descs, ol.cfg())
+    assert b2.shape() == best
+    path = str(tmp_path / "tuned.txt")
+    vctx.check(vctx.lib.vren_autotune_save(vctx.h, path.encode()))
+    other = vren_amd.Context()
+    other.check(other.lib.vren_autotune_load(other.h, path.encode()))
+    b3 = vren_amd.Batch(other, descs, ol.cfg())
+    assert b3.shape() == best
+    b3.close()
+    other.close()
+    b2.close()
+    b.close()

@@ -11,8 +11,8 @@ from . import _lib
 from ._lib import (ADDRESS_CLAMP_TO_EDGE, ADDRESS_MIRRORED_REPEAT, ADDRESS_REPEAT, ATTEN_REFERENCE, ATTEN_SMOOTH,
                    BVH_INVALID_NODE, BVH_LEAF_NODE, CLUSTER_AABB_REFERENCE, CLUSTER_AABB_TIGHT, FILTER_LINEAR,
                    FILTER_NEAREST, KEYS_REFERENCE, KEYS_SLICED, MORTON_GIVEN_AABB, MORTON_LIGHTS_AABB, REDUCE_ADD,
-                   REDUCE_MAX, REDUCE_MIN, Camera, ClusterCfg, FrameDesc, FrameOutputs, Material, MaterialSet,
-                   Texture, VrenError)
+                   REDUCE_MAX, REDUCE_MIN, Camera, ChainShape, ClusterCfg, FrameDesc, FrameOutputs, Material,
+                   MaterialSet, Texture, VrenError)
 
 __all__ = ["Context", "Camera", "ClusterCfg", "FrameDesc", "FrameOutputs", "VrenError", "lib", "default_cfg",
            "REDUCE_ADD", "REDUCE_MIN", "REDUCE_MAX", "KEYS_REFERENCE", "KEYS_SLICED", "CLUSTER_AABB_REFERENCE",
@@ -247,6 +247,23 @@ class Batch:
         o = FrameOutputs()
         self.ctx.check(self.ctx.lib.vren_batch_frame_outputs(self.h, f, ctypes.byref(o)))
         return o
+
+    def shape(self):
+        """(shade pixels per thread, assignment waves per CU, light-sort digit bits); 0 = the default."""
+        s = ChainShape()
+        self.ctx.check(self.ctx.lib.vren_batch_get_shape(self.h, ctypes.byref(s)))
+        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits)
+
+    def set_shape(self, shade_ppt, assign_wpc, light_bits):
+        s = ChainShape(shade_ppt, assign_wpc, light_bits, 0)
+        return self.ctx.check(self.ctx.lib.vren_batch_set_shape(self.h, ctypes.byref(s)))
+
+    def autotune(self, reps=3):
+        """Times every chain shape on this batch, keeps the fastest; returns (shape tuple, ms per batched run)."""
+        s, ms = ChainShape(), ctypes.c_float()
+        self.ctx.check(self.ctx.lib.vren_batch_autotune(self.h, self.ctx.stream(), reps, ctypes.byref(s),
+                                                        ctypes.byref(ms)))
+        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits), ms.value
 
     def close(self):
         if self.h:

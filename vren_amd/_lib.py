@@ -60,6 +60,11 @@ class FrameDesc(ctypes.Structure):
                 ("key_capacity", c_uint32), ("index_capacity", c_uint32), ("materials", P(MaterialSet))]
 
 
+class ChainShape(ctypes.Structure):
+    _fields_ = [("shade_pixels_per_thread", c_uint32), ("assign_waves_per_cu", c_uint32),
+                ("light_sort_bits", c_uint32), ("reserved", c_uint32)]
+
+
 class FrameOutputs(ctypes.Structure):
     _fields_ = [("view_positions", c_void_p), ("light_codes", c_void_p), ("bvh", c_void_p),
                 ("bvh_length", c_uint32), ("cluster_keys", c_void_p), ("cluster_count", c_void_p),
@@ -131,6 +136,9 @@ PROTOTYPES = {
     "vren_batch_run_profiled": (c_int, [c_void_p, c_void_p, P(c_float)]),
     "vren_batch_frame_outputs": (c_int, [c_void_p, c_uint32, P(FrameOutputs)]),
     "vren_batch_destroy": (None, [c_void_p]),
+    "vren_batch_get_shape": (c_int, [c_void_p, P(ChainShape)]),
+    "vren_batch_set_shape": (c_int, [c_void_p, P(ChainShape)]),
+    "vren_batch_autotune": (c_int, [c_void_p, c_void_p, c_uint32, P(ChainShape), P(c_float)]),
     "vren_synth_keys": (c_int, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint64, c_void_p]),
     "vren_synth_lights": (c_int, [c_void_p, c_void_p, c_uint64, c_uint32, c_void_p, c_void_p]),
     "vren_synth_frame": (c_int, [c_void_p, c_void_p, c_uint64, c_uint32, c_uint32, P(Camera), c_uint32, c_void_p,

@@ -536,8 +536,13 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
         });
 }
 
-// enough resident waves per CU (up to 32) to hide the dependent node loads of the traversals
-uint32_t persistent_blocks(vren_ctx* ctx, uint32_t wpb) { return (uint32_t)ctx->num_cus * (32u / wpb); }
+// enough resident waves per CU (32 by default, the chain shape's choice otherwise) to hide the dependent node loads
+// of the traversals
+uint32_t persistent_blocks(vren_ctx* ctx, uint32_t wpb, uint32_t waves_per_cu)
+{
+    const uint32_t wpc = waves_per_cu ? waves_per_cu : 32u;
+    return (uint32_t)ctx->num_cus * (wpc / wpb > 0 ? wpc / wpb : 1u);
+}
 
 } // namespace
 
@@ -551,13 +556,15 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
     // two stacks and two lists per wave (one per half), the chunk's boxes and cluster frames
     const size_t count_per_wave = (size_t)2 * (kStackHalf + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
-    assign_count_kernel<<<persistent_blocks(ctx, 4), 4 * VREN_WAVE, count_per_wave * 4, s>>>(d_frames, F, mask_words);
+    assign_count_kernel<<<persistent_blocks(ctx, 4, w.assign_wpc), 4 * VREN_WAVE, count_per_wave * 4, s>>>(d_frames, F,
+                                                                                                    mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     assign_offsets_kernel<<<F, 1024, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
-    assign_write_kernel<<<persistent_blocks(ctx, wpb), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F, mask_words);
+    assign_write_kernel<<<persistent_blocks(ctx, wpb, w.assign_wpc), wpb * VREN_WAVE, per_wave * wpb, s>>>(d_frames, F,
+                                                                                                   mask_words);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

@@ -1,4 +1,5 @@
-// autotune.hip -- project-written autotuner for the onesweep radix sort.
+// autotune.hip -- project-written autotuner for the onesweep radix sort (the chain's shapes are tuned by
+// vren_batch_autotune in batch.hip; both kinds of choice are saved to and loaded from the same file here).
 //
 // The reference fixes the sort at 4-bit digits, 1024-thread workgroups and one key per thread
 // (radix_sort.hpp:15-16). Here the digit width (4, 6, 8 bits), keys per thread (8, 15, 23) and waves per
@@ -148,6 +149,9 @@ VREN_EXPORT int vren_autotune_save(vren_ctx* ctx, const char* path)
         vren_radix_config_info(kv.second, &b, &k, &w, &a);
         fprintf(f, "radix %llu %d %d %d %d\n", (unsigned long long)kv.first, b, k, w, a);
     }
+    for (auto& kv : ctx->chain_tuned)
+        fprintf(f, "chain %llu %u %u %u\n", (unsigned long long)kv.first, kv.second.shade_pixels_per_thread,
+                kv.second.assign_waves_per_cu, kv.second.light_sort_bits);
     fclose(f);
     return VREN_OK;
 }
@@ -157,13 +161,21 @@ VREN_EXPORT int vren_autotune_load(vren_ctx* ctx, const char* path)
     if (!ctx || !path) return VREN_EINVAL;
     FILE* f = fopen(path, "r");
     if (!f) return vren_fail(ctx, VREN_EINVAL, "autotune_load: cannot open %s", path);
-    unsigned long long key;
-    int b, k, w, a;
-    while (fscanf(f, " radix %llu %d %d %d %d", &key, &b, &k, &w, &a) == 5) {
-        for (int i = 0; i < vren_radix_config_count(); i++) {
-            int b2, k2, w2, a2;
-            vren_radix_config_info(i, &b2, &k2, &w2, &a2);
-            if (b == b2 && k == k2 && w == w2 && a == a2) ctx->radix_tuned[(uint64_t)key] = i;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        unsigned long long key;
+        int b, k, w, a;
+        unsigned p, q, r;
+        if (sscanf(line, " radix %llu %d %d %d %d", &key, &b, &k, &w, &a) == 5) {
+            for (int i = 0; i < vren_radix_config_count(); i++) {
+                int b2, k2, w2, a2;
+                vren_radix_config_info(i, &b2, &k2, &w2, &a2);
+                if (b == b2 && k == k2 && w == w2 && a == a2) ctx->radix_tuned[(uint64_t)key] = i;
+            }
+        } else if (sscanf(line, " chain %llu %u %u %u", &key, &p, &q, &r) == 4) {
+            // only shapes a batch accepts (the same rules as vren_batch_set_shape)
+            if ((p == 1 || p == 2) && (q == 8 || q == 16 || q == 24 || q == 32) && (r == 4 || r == 8))
+                ctx->chain_tuned[(uint64_t)key] = vren_chain_shape{p, q, r, 0u};
         }
     }
     fclose(f);

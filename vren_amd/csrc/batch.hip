@@ -32,11 +32,57 @@ struct vren_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;
+    uint64_t size_class = 0;        // chain autotuner key (batch_size_class)
+    vren_chain_shape shape{};       // the chain's launch shape (zero fields: defaults)
 };
 
 namespace {
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+uint32_t ilog2_floor(uint64_t v)
+{
+    uint32_t l = 0;
+    while (v > 1) {
+        v >>= 1;
+        l++;
+    }
+    return l;
+}
+
+// the chain autotuner's size class: log2 of the batch's pixels, of its largest light count and of its frame count
+uint64_t batch_size_class(uint64_t pixels, uint32_t max_L, uint32_t frames)
+{
+    return ((uint64_t)ilog2_floor(pixels) << 16) | ((uint64_t)ilog2_floor((uint64_t)max_L + 1) << 8) |
+           ilog2_floor(frames);
+}
+
+bool valid_shape(const vren_chain_shape& s)
+{
+    return (s.shade_pixels_per_thread == 0 || s.shade_pixels_per_thread == 1 || s.shade_pixels_per_thread == 2) &&
+           (s.assign_waves_per_cu == 0 || s.assign_waves_per_cu == 8 || s.assign_waves_per_cu == 16 ||
+            s.assign_waves_per_cu == 24 || s.assign_waves_per_cu == 32) &&
+           (s.light_sort_bits == 0 || s.light_sort_bits == 4 || s.light_sort_bits == 8);
+}
+
+void apply_shape(ChainWork& w, const vren_chain_shape& s)
+{
+    w.shade_ppt = s.shade_pixels_per_thread;
+    w.assign_wpc = s.assign_waves_per_cu;
+    w.light_bits = s.light_sort_bits;
+}
+
+// a new shape invalidates the captured graph (its kernels and grids were captured with the old one)
+void set_batch_shape(vren_batch* b, const vren_chain_shape& s)
+{
+    b->shape = s;
+    apply_shape(b->w, s);
+    for (ChainWork& wf : b->w_frame) apply_shape(wf, s);
+    if (b->exec) vren_teardown(hipGraphExecDestroy(b->exec));
+    if (b->graph) vren_teardown(hipGraphDestroy(b->graph));
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
 
 // live_slots: the leaf-kernel grid over the frames' light-holding leaf groups (their padding groups were written
 // when the batch was created)
@@ -199,6 +245,11 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
     b->total_live_slots = live_base;
     b->w.max_L = max_L;
     b->w.any_materials = any_materials;
+    {
+        uint64_t pixels = 0;
+        for (uint32_t i = 0; i < n_frames; i++) pixels += (uint64_t)frames[i].width * frames[i].height;
+        b->size_class = batch_size_class(pixels, max_L, n_frames);
+    }
     // per-frame tables: same buffers, bases 0, the workspace viewed from its start
     std::vector<FrameDev> single(b->host);
     b->w_frame.resize(n_frames);
@@ -213,6 +264,11 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         wf.total_leaf_slots = b->leaf_slots[i];
         wf.max_L = single[i].L;
         wf.any_materials = single[i].texcoords != nullptr;
+    }
+    {
+        // the shape the context tuned for this size class, if any
+        auto it = ctx->chain_tuned.find(b->size_class);
+        set_batch_shape(b, it != ctx->chain_tuned.end() ? it->second : vren_chain_shape{});
     }
     hipStream_t s = nullptr;
     st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);
@@ -309,6 +365,70 @@ VREN_EXPORT int vren_batch_frame_outputs(vren_batch* b, uint32_t frame, vren_fra
     out->light_indices = f.indices;
     out->index_capacity = f.index_cap;
     out->assigned_total = f.assigned_total;
+    return VREN_OK;
+}
+
+VREN_EXPORT int vren_batch_get_shape(vren_batch* b, vren_chain_shape* shape)
+{
+    if (!b || !shape) return VREN_EINVAL;
+    *shape = b->shape;
+    return VREN_OK;
+}
+
+VREN_EXPORT int vren_batch_set_shape(vren_batch* b, const vren_chain_shape* shape)
+{
+    if (!b || !shape) return VREN_EINVAL;
+    if (!valid_shape(*shape))
+        return vren_fail(b->ctx, VREN_EINVAL, "batch_set_shape: shade 1|2, assign waves 8|16|24|32, light bits 4|8");
+    set_batch_shape(b, *shape);
+    return VREN_OK;
+}
+
+VREN_EXPORT int vren_batch_autotune(vren_batch* b, vren_stream_t stream, uint32_t reps, vren_chain_shape* best,
+                                    float* best_ms)
+{
+    if (!b) return VREN_EINVAL;
+    vren_ctx* ctx = b->ctx;
+    if (reps == 0 || reps > 100) return vren_fail(ctx, VREN_EINVAL, "batch_autotune: reps must be in [1, 100]");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) vren_teardown(hipEventDestroy(e0));
+        return vren_fail(ctx, VREN_EHIP, "batch_autotune: event creation failed");
+    }
+    const vren_chain_shape before = b->shape;
+    vren_chain_shape top{};
+    float top_ms = -1.0f;
+    int st = VREN_OK;
+    const uint32_t ppts[2] = {1, 2}, wpcs[4] = {8, 16, 24, 32}, bits[2] = {4, 8};
+    for (uint32_t a = 0; a < 2 && !st; a++)
+        for (uint32_t c = 0; c < 4 && !st; c++)
+            for (uint32_t d = 0; d < 2 && !st; d++) {
+                const vren_chain_shape cand{ppts[a], wpcs[c], bits[d], 0u};
+                set_batch_shape(b, cand);
+                st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots); // warm-up
+                if (!st && hipEventRecord(e0, s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch_autotune: record");
+                for (uint32_t r = 0; r < reps && !st; r++) st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots);
+                float ms = 0.0f;
+                if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    st = vren_fail(ctx, VREN_EHIP, "batch_autotune: timing a shape failed");
+                ms /= (float)reps;
+                if (!st && (top_ms < 0.0f || ms < top_ms)) {
+                    top = cand;
+                    top_ms = ms;
+                }
+            }
+    vren_teardown(hipEventDestroy(e0));
+    vren_teardown(hipEventDestroy(e1));
+    if (st) {
+        set_batch_shape(b, before);
+        return st;
+    }
+    set_batch_shape(b, top);
+    ctx->chain_tuned[b->size_class] = top;
+    if (best) *best = top;
+    if (best_ms) *best_ms = top_ms;
     return VREN_OK;
 }
 

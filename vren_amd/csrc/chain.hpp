@@ -31,6 +31,11 @@ struct ChainWork {
     uint32_t total_leaf_slots;
     uint32_t max_L;
     uint32_t any_materials; // some frame of the table takes the material path (shade picks its kernel by it)
+    // launch shape (vren_chain_shape; 0 = the default): shade strips per workgroup, persistent assignment waves
+    // per CU, digit width of the light sort
+    uint32_t shade_ppt = 0;
+    uint32_t assign_wpc = 0;
+    uint32_t light_bits = 0;
     // optional per-kernel timing: an event is recorded after every kernel of the chain
     hipEvent_t* events = nullptr;
     int* n_events = nullptr;

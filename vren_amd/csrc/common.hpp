@@ -173,6 +173,8 @@ struct vren_ctx {
     // radix-sort autotuner: forced configuration (-1 = none) and tuned choices keyed by problem bucket
     int radix_forced = -1;
     std::map<uint64_t, int> radix_tuned;
+    // chain autotuner: tuned batch shapes keyed by size class (batch.hip)
+    std::map<uint64_t, vren_chain_shape> chain_tuned;
 };
 
 #define VREN_FRAME_SLOTS 256

@@ -31,7 +31,9 @@ __device__ __forceinline__ uint32_t morton_interleave(uint32_t qx, uint32_t qy, 
     return code;
 }
 
-// one stable LSD pass over a segment of L (code, index) pairs, src -> dst (generic pointers: LDS or global)
+// one stable LSD pass of a BITS-bit digit (BITS <= 8) over a segment of L (code, index) pairs, src -> dst (generic
+// pointers: LDS or global)
+template <int BITS>
 __device__ void block_sort_pass(const uint2* src, uint2* dst, uint32_t L, uint32_t shift, uint32_t width,
                                 uint32_t* s_whist, uint32_t* s_base, uint32_t* s_cnt, uint32_t* s_scan)
 {
@@ -58,7 +60,7 @@ __device__ void block_sort_pass(const uint2* src, uint2* dst, uint32_t L, uint32
             e[j] = pos < cnt ? src[c0 + pos] : make_uint2(0u, 0u);
             digit[j] = (e[j].x >> shift) & mask;
         }
-        rank_wave_keys<8, kLKPT>(digit, wbase, cnt, &s_whist[wave * 256], local);
+        rank_wave_keys<BITS, kLKPT>(digit, wbase, cnt, &s_whist[wave * 256], local);
         __syncthreads();
         if (tid < 256) {
             uint32_t run = 0;
@@ -84,6 +86,8 @@ __device__ void block_sort_pass(const uint2* src, uint2* dst, uint32_t L, uint32
     }
 }
 
+// BITS: digit width of the Morton-code sort (8 by default; 4 is a chain-shape choice)
+template <int BITS>
 __global__ __launch_bounds__(kLT) void lights_kernel(const FrameDev* __restrict__ frames)
 {
     __shared__ uint2 s_a[kLdsCap];
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(kLT) void lights_kernel(const FrameDev* __restrict_
     // 3. Morton codes (discretize_point_light_positions.comp:26-43), paired with the light index
     const uint32_t bits = fd.morton_bits;
     const uint32_t total_bits = 3 * bits;
-    const uint32_t passes = (total_bits + 7) / 8;
+    const uint32_t passes = (total_bits + BITS - 1) / BITS;
     const float scale = (float)(1u << bits);
     const bool in_lds = L <= kLdsCap;
     uint2* A;
@@ -180,8 +184,9 @@ __global__ __launch_bounds__(kLT) void lights_kernel(const FrameDev* __restrict_
     uint2* src = A;
     uint2* dst = B;
     for (uint32_t p = 0; p < passes; p++) {
-        uint32_t shift = 8 * p;
-        block_sort_pass(src, dst, L, shift, min(8u, total_bits - shift), s_whist, s_base, s_cnt, s_scan);
+        uint32_t shift = BITS * p;
+        block_sort_pass<BITS>(src, dst, L, shift, min((uint32_t)BITS, total_bits - shift), s_whist, s_base, s_cnt,
+                              s_scan);
         uint2* t = src;
         src = dst;
         dst = t;
@@ -320,7 +325,8 @@ __global__ __launch_bounds__(1024) void bvh_upper_kernel(const FrameDev* __restr
 
 int launch_lights(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_t F, const ChainWork& w)
 {
-    lights_kernel<<<F, kLT, 0, s>>>(d_frames);
+    if (w.light_bits == 4) lights_kernel<4><<<F, kLT, 0, s>>>(d_frames);
+    else lights_kernel<8><<<F, kLT, 0, s>>>(d_frames);
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;

@@ -188,7 +188,7 @@ __device__ __forceinline__ void shade_light(const ShadeSurface& s, const float L
 }
 
 constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup
-constexpr int kShadePPT = 1;     // pixels per thread of the forward shade (2 measured slower)
+constexpr int kShadePPT = 1;     // default pixels per thread of the forward shade (2 is a tuner choice)
 
 // The pixel's list entries [off, end): light records from the LDS stage (STAGED, entry e at e - start, the
 // position's w holding 1 / radius) or from global memory; SMOOTH selects the windowed attenuation.
@@ -908,11 +908,16 @@ int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_
                                                                     w.total_tiles, w.tile_cl, w.tile_range);
         VREN_LAUNCH_CHECK(ctx);
     }
-    const uint32_t grid = w.total_tiles * (4 / kShadePPT);
-    if (w.any_materials)
-        shade_kernel<kShadePPT, true><<<grid, 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.tile_range);
-    else
-        shade_kernel<kShadePPT, false><<<grid, 256, 0, s>>>(d_frames, F > 1 ? w.tile_frame : nullptr, w.tile_range);
+    const uint32_t* tf = F > 1 ? w.tile_frame : nullptr;
+    if (w.shade_ppt == 2) {
+        const uint32_t grid = w.total_tiles * 2;
+        if (w.any_materials) shade_kernel<2, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+        else shade_kernel<2, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+    } else {
+        const uint32_t grid = w.total_tiles * (4 / kShadePPT);
+        if (w.any_materials) shade_kernel<kShadePPT, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+        else shade_kernel<kShadePPT, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+    }
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
     return VREN_OK;
