@@ -187,37 +187,60 @@ __device__ __forceinline__ void shade_light(const ShadeSurface& s, const float L
     }
 }
 
-constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup
+constexpr uint32_t kStage = 512; // lights staged in LDS per workgroup of the adjoint
+
+// A light as the forward shade's loop reads it from LDS: the position and r^2 already in double (the loop's
+// distance test and the cancelling difference r^2 - |d|^2 are taken in double), the colour and radius, 1 / r.
+// A light past the table or with r <= 0 (or NaN) gets rr = -1, which no squared distance is below.
+struct alignas(16) StagedLight {
+    double x, y, z, rr;
+    float4 col; // rgb, radius
+    float inv_r, pad0, pad1, pad2;
+};
+constexpr uint32_t kFwdStage = 256; // lights staged per forward workgroup (16 KB; tiles stage ~6 on average)
 constexpr int kShadePPT = 1;     // default pixels per thread of the forward shade (2 is a tuner choice)
 
 // The pixel's list entries [off, end): light records from the LDS stage (STAGED, entry e at e - start, the
 // position's w holding 1 / radius) or from global memory; SMOOTH selects the windowed attenuation.
+__device__ __forceinline__ StagedLight stage_light(const FrameDev& fd, uint32_t li)
+{
+    StagedLight sl;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f), c = p;
+    if (li < fd.L) {
+        const vf4 cv = *((gvf4*)fd.lights + li), pv = *((gvf4*)fd.light_pos + li);
+        c = make_float4(cv.x, cv.y, cv.z, cv.w);
+        p = make_float4(pv.x, pv.y, pv.z, pv.w);
+    }
+    sl.x = (double)p.x;
+    sl.y = (double)p.y;
+    sl.z = (double)p.z;
+    sl.rr = c.w > 0.0f ? (double)c.w * (double)c.w : -1.0;
+    sl.col = c;
+    sl.inv_r = frcp(c.w);
+    sl.pad0 = sl.pad1 = sl.pad2 = 0.0f;
+    return sl;
+}
+
 template <bool STAGED, bool SMOOTH>
 __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px, const ShadeSurface& s, uint32_t off,
-                                           uint32_t end, uint32_t start, const float4* s_lpos, const float4* s_lcol,
-                                           float acc[3])
+                                           uint32_t end, uint32_t start, const StagedLight* s_light, float acc[3])
 {
-    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
-    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
     for (uint32_t e = off; e < end; e++) {
-        float4 lp, lc;
+        StagedLight sl;
         if constexpr (STAGED) {
-            lp = s_lpos[e - start];
-            lc = s_lcol[e - start];
+            sl = s_light[e - start];
         } else {
             const uint32_t li = fd.indices[e];
             if (li >= fd.L) continue;
-            lp = lpos[li];
-            lc = lcol[li];
-            lp.w = frcp(lc.w);
+            sl = stage_light(fd, li);
         }
         // d = p - light (pbr.glsl:87) and |d| in double; the attenuation is evaluated through (r - |d|) / r,
         // which is where 1 - (|d|/r)^1.5 (or 1 - (|d|/r)^4) cancels near the light's radius
-        const double d[3] = {px.P[0] - (double)lp.x, px.P[1] - (double)lp.y, px.P[2] - (double)lp.z};
+        const double d[3] = {px.P[0] - sl.x, px.P[1] - sl.y, px.P[2] - sl.z};
         const double dist2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-        const float r = lc.w;
-        const double rr = (double)r * (double)r;
-        if (!(dist2 > 0.0) || !(r > 0.0f) || !(dist2 < rr)) continue;
+        const float r = sl.col.w;
+        const double rr = sl.rr;
+        if (!(dist2 > 0.0) || !(dist2 < rr)) continue;
         const float d2f = (float)dist2;
         const float inv = __builtin_amdgcn_rsqf(d2f); // 1 / |d|
         const float s0 = d2f * inv;                     // |d|
@@ -232,13 +255,13 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
             const float w = omx * (1.0f + x) * (1.0f + x * x);
             att = w * w;
         } else {
-            // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5), with x = |d| / r (lp.w holds 1 / r)
-            const float x = s0 * lp.w;
+            // 1 - x^1.5 = (1 - x)(1 + x + x^2) / (1 + x^1.5), with x = |d| / r
+            const float x = s0 * sl.inv_r;
             att = num * (1.0f + x + x * x) * frcp((r + s0) * r * (1.0f + x * __builtin_amdgcn_sqrtf(x)));
         }
         if (!(att > 0.0f)) continue;
         const float L[3] = {(float)d[0] * inv, (float)d[1] * inv, (float)d[2] * inv};
-        const float col[3] = {lc.x, lc.y, lc.z};
+        const float col[3] = {sl.col.x, sl.col.y, sl.col.z};
         shade_light(s, L, col, att, acc);
     }
 }
@@ -247,7 +270,7 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
 // entry e at e - start) or from global memory.
 __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar, float metal,
                                             bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
-                                            const float4* s_lpos, const float4* s_lcol, uint64_t p)
+                                            const StagedLight* s_light, uint64_t p)
 {
     ShadeSurface s;
     {
@@ -278,11 +301,11 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
         // other branch's registers or its test
         const bool smooth = fd.attenuation == VREN_ATTEN_SMOOTH;
         if (staged) {
-            if (smooth) light_loop<true, true>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
-            else light_loop<true, false>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
+            if (smooth) light_loop<true, true>(fd, px, s, off, end, start, s_light, acc);
+            else light_loop<true, false>(fd, px, s, off, end, start, s_light, acc);
         } else {
-            if (smooth) light_loop<false, true>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
-            else light_loop<false, false>(fd, px, s, off, end, start, s_lpos, s_lcol, acc);
+            if (smooth) light_loop<false, true>(fd, px, s, off, end, start, s_light, acc);
+            else light_loop<false, false>(fd, px, s, off, end, start, s_light, acc);
         }
     }
     for (uint32_t i = 0; i < fd.n_dir; i++) {
@@ -395,9 +418,8 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
                                                     const uint2* __restrict__ tile_range)
 {
     constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
-    static_assert(kStage == 2 * 256, "two staged entries per thread");
-    __shared__ float4 s_lpos[kStage];
-    __shared__ float4 s_lcol[kStage];
+    static_assert(kFwdStage == 256, "one staged entry per thread");
+    __shared__ StagedLight s_light[kFwdStage];
     __shared__ uint32_t s_min[4], s_max[4];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / BPT, sub = blockIdx.x % BPT;
@@ -407,16 +429,12 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
     if (t >= fd.Sx * fd.Sy) return; // padding after the frame's tiles (the whole workgroup)
     const uint32_t ti = t % fd.Sx, tj = t / fd.Sx;
     uint32_t start = 0, nstage = 0;
-    uint32_t li[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t li = 0xFFFFFFFFu;
     if (tile_range) {
         const uint2 rg = tile_range[g];
         start = rg.x;
         nstage = rg.y > rg.x ? rg.y - rg.x : 0u;
-        if (nstage <= kStage) {
-#pragma unroll
-            for (int k = 0; k < 2; k++)
-                if (tid + 256u * k < nstage) li[k] = *((guint*)fd.indices + start + tid + 256u * k);
-        }
+        if (nstage <= kFwdStage && tid < nstage) li = *((guint*)fd.indices + start + tid);
     }
     PixelIn px[PPT];
     uint2 nb[PPT];
@@ -502,38 +520,16 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
         const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
         start = a == 0xFFFFFFFFu ? 0u : a;
         nstage = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
-        if (nstage <= kStage) {
-#pragma unroll
-            for (int k = 0; k < 2; k++)
-                if (tid + 256u * k < nstage) li[k] = *((guint*)fd.indices + start + tid + 256u * k);
-        }
+        if (nstage <= kFwdStage && tid < nstage) li = *((guint*)fd.indices + start + tid);
     }
-    const bool staged = nstage <= kStage;
-    if (staged) {
-        const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
-        const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const uint32_t i = tid + 256u * k;
-            if (i < nstage) {
-                float4 q = make_float4(0.f, 0.f, 0.f, 0.f), c = q; // radius 0: never contributes
-                if (li[k] < fd.L) {
-                    const vf4 cv = *((gvf4*)fd.lights + li[k]), pv = *((gvf4*)fd.light_pos + li[k]);
-                    c = make_float4(cv.x, cv.y, cv.z, cv.w);
-                    q = make_float4(pv.x, pv.y, pv.z, pv.w);
-                    q.w = frcp(c.w); // the light position's w is unused: it carries 1 / radius
-                }
-                s_lpos[i] = q;
-                s_lcol[i] = c;
-            }
-        }
-    }
+    const bool staged = nstage <= kFwdStage;
+    if (staged && tid < nstage) s_light[tid] = stage_light(fd, li);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PPT; q++)
         if (px[q].shaded)
-            shade_pixel(fd, px[q], nb[q], ar[q], metal[q], has_list[q], my_off[q], my_end[q], staged, start, s_lpos,
-                        s_lcol, p[q]);
+            shade_pixel(fd, px[q], nb[q], ar[q], metal[q], has_list[q], my_off[q], my_end[q], staged, start, s_light,
+                        p[q]);
 }
 
 
