@@ -26,6 +26,10 @@ struct FrameDev {
     double inv_view_d[16];
     double eye[3];
     double inv_W, inv_H;    // 1 / W, 1 / H (shade: pixel x/y only feed the zero coefficients of the z/w rows)
+    // inv_proj's x row depends on NDC x alone and its y row on NDC y alone (every perspective camera): shade then
+    // takes r_x = rec_xy[0] gx + rec_xy[1] and r_y = rec_xy[2] gy + rec_xy[3], the NDC transform folded in
+    uint32_t proj_xy_sparse;
+    double rec_xy[4];
     float near_plane, far_plane;
     float slice_a;          // 2 tan(fov/2) / Sy + 1
     float log_a;            // vren_logf(slice_a)

@@ -70,6 +70,15 @@ void vren_frame_setup(FrameDev* f, uint32_t W, uint32_t H, const vren_camera* ca
     for (int i = 0; i < 3; i++) f->eye[i] = (double)cam->position[i];
     f->inv_W = 1.0 / (double)W;
     f->inv_H = 1.0 / (double)H;
+    {
+        // r_x = ip[0] (2 gx / W - 1) + ip[12], r_y = ip[5] (1 - 2 gy / H) + ip[13] (column-major ip)
+        const double* ip = f->inv_proj_d;
+        f->proj_xy_sparse = (ip[4] == 0.0 && ip[8] == 0.0 && ip[1] == 0.0 && ip[9] == 0.0) ? 1u : 0u;
+        f->rec_xy[0] = 2.0 * ip[0] * f->inv_W;
+        f->rec_xy[1] = ip[12] - ip[0];
+        f->rec_xy[2] = -2.0 * ip[5] * f->inv_H;
+        f->rec_xy[3] = ip[5] + ip[13];
+    }
     f->near_plane = cam->near_plane;
     f->far_plane = cam->far_plane;
     float tan_half = (float)std::tan((double)(cam->fov_y / 2.0f));

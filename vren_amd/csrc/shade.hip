@@ -50,9 +50,16 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
     v[0] = v[0] * 2.0 - 1.0;
     v[1] = v[1] * 2.0 - 1.0;
     double r[4];
-    for (int i = 0; i < 2; i++)
-        r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) + fd.inv_proj_d[2 * 4 + i] * v[2]) +
-               fd.inv_proj_d[3 * 4 + i] * v[3];
+    if (fd.proj_xy_sparse) {
+        // one multiply-add each: the NDC transform folded into the row's two nonzero coefficients (the x/y of the
+        // world position differ from the unfolded order by an ulp of double)
+        r[0] = fd.rec_xy[0] * (double)gx + fd.rec_xy[1];
+        r[1] = fd.rec_xy[2] * (double)gy + fd.rec_xy[3];
+    } else {
+        for (int i = 0; i < 2; i++)
+            r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
+                    fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
+    }
     double w, z;
     {
         // the view depth and the background test round exactly as the oracle's (no fused multiply-add here; this
