@@ -352,6 +352,17 @@ __global__ __launch_bounds__(kClusterWaves * 64, 8) void clusters_kernel(const F
                 if (key[q] == m) rem &= ~(1u << q);
         }
         __builtin_amdgcn_wave_barrier();
+    }
+    // the group's aggregate is published as soon as its tiles' unique counts are known, so that the next groups'
+    // walks find it while this group still searches its pixels' indices (8 % of the kernel against publishing
+    // at the walk)
+    if (lane == 0) s_u[wave] = u;
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t agg = wave_sum(lane < kClusterWaves ? s_u[lane] : 0u);
+        if (lane == 0) wave_lookback_publish(tile_status + fd.tile_base / VREN_TILE_GROUP, t0 / VREN_TILE_GROUP, agg);
+    }
+    if (live) {
         // each pixel's index in the tile's unique list (binary search), found before the look-back so that it
         // overlaps the wait for the previous groups
 #pragma unroll
@@ -365,13 +376,11 @@ __global__ __launch_bounds__(kClusterWaves * 64, 8) void clusters_kernel(const F
             key[i] = lo;
         }
     }
-    if (lane == 0) s_u[wave] = u;
-    __syncthreads();
     // decoupled look-back over the frame's tile groups (row-major order), one walk per workgroup: a round trip of
     // the walk covers 64 groups = 512 tiles
     if (wave == 0) {
         const uint32_t agg = wave_sum(lane < kClusterWaves ? s_u[lane] : 0u);
-        const uint32_t prefix = wave_lookback(tile_status + fd.tile_base / VREN_TILE_GROUP, t0 / VREN_TILE_GROUP, agg);
+        const uint32_t prefix = wave_lookback_walk(tile_status + fd.tile_base / VREN_TILE_GROUP, t0 / VREN_TILE_GROUP, agg);
         if (lane == 0) s_prefix = prefix;
     }
     __syncthreads();

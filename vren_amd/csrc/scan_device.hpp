@@ -27,18 +27,20 @@ __device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* s_
     return before + incl - v;
 }
 
-// Decoupled look-back, run by ONE full wave. `status` points at the first status word of the segment; tile
-// `t` publishes `agg` and returns the exclusive prefix of the segment's tiles before it, then publishes its
-// inclusive value. Tile 0 of a segment never waits. Status words must be VREN_LB_EMPTY before the launch.
-// Forward progress: tiles are taken from a ticket counter, so every predecessor is already running.
-__device__ __forceinline__ uint32_t wave_lookback(uint64_t* status, uint32_t t, uint32_t agg)
+// Decoupled look-back in two steps, for a caller with work to do between them: wave_lookback_publish (any one
+// lane) publishes tile t's aggregate (tile 0: its inclusive value); wave_lookback_walk (ONE full wave) returns the
+// exclusive prefix of the segment's tiles before t and publishes t's inclusive value. `status` points at the first
+// status word of the segment; status words must be VREN_LB_EMPTY before the launch. Tile 0 of a segment never
+// waits. Forward progress: tiles are taken from a ticket counter, so every predecessor is already running.
+__device__ __forceinline__ void wave_lookback_publish(uint64_t* status, uint32_t t, uint32_t agg)
+{
+    st_agent_u64(&status[t], (t == 0 ? VREN_LB_INCL : VREN_LB_AGG) | agg);
+}
+
+__device__ __forceinline__ uint32_t wave_lookback_walk(uint64_t* status, uint32_t t, uint32_t agg)
 {
     const uint32_t lane = lane_id();
-    if (t == 0) {
-        if (lane == 0) st_agent_u64(&status[0], VREN_LB_INCL | agg);
-        return 0u;
-    }
-    if (lane == 0) st_agent_u64(&status[t], VREN_LB_AGG | agg);
+    if (t == 0) return 0u;
     uint32_t prefix = 0;
     int64_t base = (int64_t)t - 1;
     while (true) {
@@ -63,4 +65,11 @@ __device__ __forceinline__ uint32_t wave_lookback(uint64_t* status, uint32_t t, 
     }
     if (lane == 0) st_agent_u64(&status[t], VREN_LB_INCL | (uint32_t)(prefix + agg));
     return prefix;
+}
+
+// both steps at once (nothing between publishing the aggregate and walking)
+__device__ __forceinline__ uint32_t wave_lookback(uint64_t* status, uint32_t t, uint32_t agg)
+{
+    if (lane_id() == 0) wave_lookback_publish(status, t, agg);
+    return wave_lookback_walk(status, t, agg);
 }
