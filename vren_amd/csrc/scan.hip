@@ -31,6 +31,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
                                                         uint64_t* status, uint32_t* counter)
 {
     __shared__ uint32_t s_part[kRows * kWaves]; // (row, wave) segment totals, then their exclusive prefixes
+    __shared__ uint32_t s_wsum[kWaves];          // each wave's total over its rows
     __shared__ uint32_t s_ticket;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
     if (tid == 0) s_ticket = atomic_add_agent(counter, 1u);
@@ -60,6 +61,22 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
             v[j].w = e + 3 < cnt ? src[e + 3] : 0u;
         }
     }
+    // the tile's total first, published before the row scans, so that the next tiles' walks find this tile's
+    // aggregate while it still scans its rows
+    {
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int j = 0; j < kRows; j++) tsum += (v[j].x + v[j].y) + (v[j].z + v[j].w);
+        tsum = wave_sum(tsum);
+        if (lane == 0) s_wsum[wave] = tsum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t agg = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) agg += s_wsum[w];
+        wave_lookback_publish(status + (uint64_t)seg * tiles_per_seg, t, agg);
+    }
     uint32_t excl[kRows]; // exclusive prefix of the thread's 4 elements within its wave's segment of each row
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
@@ -73,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
         const uint32_t x = s_part[lane];
         const uint32_t xi = wave_inclusive_sum(x);
         const uint32_t agg = shfl(xi, VREN_WAVE - 1);
-        const uint32_t prefix = wave_lookback(status + (uint64_t)seg * tiles_per_seg, t, agg);
+        const uint32_t prefix = wave_lookback_walk(status + (uint64_t)seg * tiles_per_seg, t, agg);
         s_part[lane] = prefix + xi - x;
     }
     __syncthreads();
