@@ -656,7 +656,8 @@ def main():
         # the chain autotuner: every launch shape timed on this batch, the fastest kept (setup, not timed)
         shape, tms = batch.autotune(reps=3)
         chain_tune = {"shade_pixels_per_thread": shape[0], "assign_waves_per_cu": shape[1],
-                      "light_sort_bits": shape[2], "tuned_ms_per_batched_run": round(tms, 4)}
+                      "light_sort_bits": shape[2], "light_stage_concurrent": shape[3],
+                      "tuned_ms_per_batched_run": round(tms, 4)}
         print("[bench] chain shape %s (%.3f ms per batched run)" % (shape, tms), file=sys.stderr, flush=True)
 
     # batched: one graph replay per step

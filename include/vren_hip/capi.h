@@ -405,14 +405,15 @@ typedef struct vren_chain_shape {
     uint32_t shade_pixels_per_thread; /* 1 or 2: 32x8 strips per 256-thread shade workgroup (default 1) */
     uint32_t assign_waves_per_cu;     /* 8, 16, 24 or 32: persistent light-assignment waves per CU (default 32) */
     uint32_t light_sort_bits;         /* 4 or 8: digit width of each frame's Morton-code sort (default 8) */
-    uint32_t reserved;
+    uint32_t light_stage_concurrent;  /* 0 or 1: batched runs build the light BVH on a second stream beside the
+                                         cluster stage (a fork and join in the captured graph; default 0) */
 } vren_chain_shape;
 /* the batch's current shape (the context's tuned shape for the batch's size class when one exists, else the
  * defaults) and a forced one (invalid fields: VREN_EINVAL); a new shape drops the captured graph */
 VREN_EXPORT int vren_batch_get_shape(vren_batch* batch, vren_chain_shape* shape);
 VREN_EXPORT int vren_batch_set_shape(vren_batch* batch, const vren_chain_shape* shape);
 /* times the batched chain (VREN_RUN_BATCHED, `reps` timed runs after one warm-up) with every shape of the grid
- * {1, 2} x {8, 16, 24, 32} x {4, 8}, keeps the fastest for the batch and caches it in the context under the batch's
+ * {1, 2} x {8, 16, 24, 32} x {4, 8} x {0, 1}, keeps the fastest for the batch and caches it in the context under the batch's
  * size class (log2 of its pixels, log2 of its largest light count, log2 of its frame count), where batches created
  * later in that class find it; vren_autotune_save / _load keep the chain shapes beside the radix configurations.
  * Synchronises the stream; the outputs hold the last run's results. */

@@ -576,11 +576,11 @@ def test_batch_chain_shapes_identical_and_autotuned(vctx, co, tmp_path):
     specs = [(1280, 720, 5000, 9300), (640, 360, 300, 9301), (333, 200, 1, 9302)]
     descs, keep = make_frames_batch(vctx, co, specs)
     b = vren_amd.Batch(vctx, descs, ol.cfg())
-    assert b.shape() == (0, 0, 0)
+    assert b.shape() == (0, 0, 0, 0)
     b.run(b.BATCHED)
     torch.cuda.synchronize()
     want = [k["o"].clone() for k in keep]
-    for shape in [(1, 8, 4), (2, 16, 8), (2, 32, 4), (1, 24, 8), (2, 8, 8)]:
+    for shape in [(1, 8, 4, 0), (2, 16, 8, 1), (2, 32, 4, 0), (1, 24, 8, 1), (2, 8, 8, 1), (1, 32, 8, 1)]:
         b.set_shape(*shape)
         assert b.shape() == shape
         for mode in (b.BATCHED, b.GRAPH, b.GRAPH):
@@ -592,8 +592,10 @@ def test_batch_chain_shapes_identical_and_autotuned(vctx, co, tmp_path):
                 assert torch.equal(a, k["o"]), (shape, mode, i)
     with pytest.raises(vren_amd.VrenError):
         b.set_shape(3, 8, 4)
+    with pytest.raises(vren_amd.VrenError):
+        b.set_shape(1, 8, 4, 2)
     best, ms = b.autotune(reps=1)
-    assert best[0] in (1, 2) and best[1] in (8, 16, 24, 32) and best[2] in (4, 8) and ms > 0
+    assert best[0] in (1, 2) and best[1] in (8, 16, 24, 32) and best[2] in (4, 8) and best[3] in (0, 1) and ms > 0
     assert b.shape() == best
     for k in keep:
         k["o"].fill_(0xFF)

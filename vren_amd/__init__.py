@@ -249,13 +249,14 @@ class Batch:
         return o
 
     def shape(self):
-        """(shade pixels per thread, assignment waves per CU, light-sort digit bits); 0 = the default."""
+        """(shade pixels per thread, assignment waves per CU, light-sort digit bits, concurrent light stage);
+        0 = the default."""
         s = ChainShape()
         self.ctx.check(self.ctx.lib.vren_batch_get_shape(self.h, ctypes.byref(s)))
-        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits)
+        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits, s.light_stage_concurrent)
 
-    def set_shape(self, shade_ppt, assign_wpc, light_bits):
-        s = ChainShape(shade_ppt, assign_wpc, light_bits, 0)
+    def set_shape(self, shade_ppt, assign_wpc, light_bits, light_concurrent=0):
+        s = ChainShape(shade_ppt, assign_wpc, light_bits, light_concurrent)
         return self.ctx.check(self.ctx.lib.vren_batch_set_shape(self.h, ctypes.byref(s)))
 
     def autotune(self, reps=3):
@@ -263,7 +264,8 @@ class Batch:
         s, ms = ChainShape(), ctypes.c_float()
         self.ctx.check(self.ctx.lib.vren_batch_autotune(self.h, self.ctx.stream(), reps, ctypes.byref(s),
                                                         ctypes.byref(ms)))
-        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits), ms.value
+        return (s.shade_pixels_per_thread, s.assign_waves_per_cu, s.light_sort_bits,
+                s.light_stage_concurrent), ms.value
 
     def close(self):
         if self.h:

@@ -62,7 +62,7 @@ class FrameDesc(ctypes.Structure):
 
 class ChainShape(ctypes.Structure):
     _fields_ = [("shade_pixels_per_thread", c_uint32), ("assign_waves_per_cu", c_uint32),
-                ("light_sort_bits", c_uint32), ("reserved", c_uint32)]
+                ("light_sort_bits", c_uint32), ("light_stage_concurrent", c_uint32)]
 
 
 class FrameOutputs(ctypes.Structure):

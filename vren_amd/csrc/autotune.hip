@@ -150,8 +150,8 @@ VREN_EXPORT int vren_autotune_save(vren_ctx* ctx, const char* path)
         fprintf(f, "radix %llu %d %d %d %d\n", (unsigned long long)kv.first, b, k, w, a);
     }
     for (auto& kv : ctx->chain_tuned)
-        fprintf(f, "chain %llu %u %u %u\n", (unsigned long long)kv.first, kv.second.shade_pixels_per_thread,
-                kv.second.assign_waves_per_cu, kv.second.light_sort_bits);
+        fprintf(f, "chain %llu %u %u %u %u\n", (unsigned long long)kv.first, kv.second.shade_pixels_per_thread,
+                kv.second.assign_waves_per_cu, kv.second.light_sort_bits, kv.second.light_stage_concurrent);
     fclose(f);
     return VREN_OK;
 }
@@ -165,17 +165,17 @@ VREN_EXPORT int vren_autotune_load(vren_ctx* ctx, const char* path)
     while (fgets(line, sizeof(line), f)) {
         unsigned long long key;
         int b, k, w, a;
-        unsigned p, q, r;
+        unsigned p, q, r, c = 0;
         if (sscanf(line, " radix %llu %d %d %d %d", &key, &b, &k, &w, &a) == 5) {
             for (int i = 0; i < vren_radix_config_count(); i++) {
                 int b2, k2, w2, a2;
                 vren_radix_config_info(i, &b2, &k2, &w2, &a2);
                 if (b == b2 && k == k2 && w == w2 && a == a2) ctx->radix_tuned[(uint64_t)key] = i;
             }
-        } else if (sscanf(line, " chain %llu %u %u %u", &key, &p, &q, &r) == 4) {
+        } else if (sscanf(line, " chain %llu %u %u %u %u", &key, &p, &q, &r, &c) >= 4) {
             // only shapes a batch accepts (the same rules as vren_batch_set_shape)
-            if ((p == 1 || p == 2) && (q == 8 || q == 16 || q == 24 || q == 32) && (r == 4 || r == 8))
-                ctx->chain_tuned[(uint64_t)key] = vren_chain_shape{p, q, r, 0u};
+            if ((p == 1 || p == 2) && (q == 8 || q == 16 || q == 24 || q == 32) && (r == 4 || r == 8) && c <= 1)
+                ctx->chain_tuned[(uint64_t)key] = vren_chain_shape{p, q, r, c};
         }
     }
     fclose(f);

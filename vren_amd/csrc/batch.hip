@@ -34,6 +34,9 @@ struct vren_batch {
     hipStream_t capture = nullptr;
     uint64_t size_class = 0;        // chain autotuner key (batch_size_class)
     vren_chain_shape shape{};       // the chain's launch shape (zero fields: defaults)
+    // the second stream of a concurrent light stage, with its fork and join events (created with the batch)
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -62,22 +65,24 @@ bool valid_shape(const vren_chain_shape& s)
     return (s.shade_pixels_per_thread == 0 || s.shade_pixels_per_thread == 1 || s.shade_pixels_per_thread == 2) &&
            (s.assign_waves_per_cu == 0 || s.assign_waves_per_cu == 8 || s.assign_waves_per_cu == 16 ||
             s.assign_waves_per_cu == 24 || s.assign_waves_per_cu == 32) &&
-           (s.light_sort_bits == 0 || s.light_sort_bits == 4 || s.light_sort_bits == 8);
+           (s.light_sort_bits == 0 || s.light_sort_bits == 4 || s.light_sort_bits == 8) &&
+           (s.light_stage_concurrent == 0 || s.light_stage_concurrent == 1);
 }
 
-void apply_shape(ChainWork& w, const vren_chain_shape& s)
+void apply_shape(ChainWork& w, const vren_chain_shape& s, bool batched)
 {
     w.shade_ppt = s.shade_pixels_per_thread;
     w.assign_wpc = s.assign_waves_per_cu;
     w.light_bits = s.light_sort_bits;
+    w.light_concurrent = batched ? s.light_stage_concurrent : 0u; // the per-frame path stays one stream
 }
 
 // a new shape invalidates the captured graph (its kernels and grids were captured with the old one)
 void set_batch_shape(vren_batch* b, const vren_chain_shape& s)
 {
     b->shape = s;
-    apply_shape(b->w, s);
-    for (ChainWork& wf : b->w_frame) apply_shape(wf, s);
+    apply_shape(b->w, s, true);
+    for (ChainWork& wf : b->w_frame) apply_shape(wf, s, false);
     if (b->exec) vren_teardown(hipGraphExecDestroy(b->exec));
     if (b->graph) vren_teardown(hipGraphDestroy(b->graph));
     b->exec = nullptr;
@@ -85,15 +90,29 @@ void set_batch_shape(vren_batch* b, const vren_chain_shape& s)
 }
 
 // live_slots: the leaf-kernel grid over the frames' light-holding leaf groups (their padding groups were written
-// when the batch was created)
+// when the batch was created). With a concurrent light stage (and no per-kernel timing), the lights and their BVH
+// run on the batch's second stream between a fork and a join while `s` keys the clusters: the two stages share no
+// buffer, and light assignment, the first consumer of both, waits for the join. The join is recorded even when a
+// light-stage launch fails, so a capture always ends with both streams joined.
 int run_chain(vren_batch* b, hipStream_t s, const FrameDev* table, uint32_t F, const ChainWork& w,
               uint32_t live_slots)
 {
     vren_ctx* ctx = b->ctx;
     int st = launch_chain_init(ctx, s, w);
-    if (!st) st = launch_lights(ctx, s, table, F, w);
-    if (!st && live_slots) st = launch_bvh(ctx, s, table, F, live_slots, true, &w, kLeavesLive);
-    if (!st) st = launch_clusters(ctx, s, table, F, w);
+    if (!st && w.light_concurrent && !w.events && b->side) {
+        VREN_HIP(ctx, hipEventRecord(b->fork, s));
+        VREN_HIP(ctx, hipStreamWaitEvent(b->side, b->fork, 0));
+        int sl = launch_lights(ctx, b->side, table, F, w);
+        if (!sl && live_slots) sl = launch_bvh(ctx, b->side, table, F, live_slots, true, &w, kLeavesLive);
+        VREN_HIP(ctx, hipEventRecord(b->join, b->side));
+        st = launch_clusters(ctx, s, table, F, w);
+        VREN_HIP(ctx, hipStreamWaitEvent(s, b->join, 0));
+        if (!st) st = sl;
+    } else {
+        if (!st) st = launch_lights(ctx, s, table, F, w);
+        if (!st && live_slots) st = launch_bvh(ctx, s, table, F, live_slots, true, &w, kLeavesLive);
+        if (!st) st = launch_clusters(ctx, s, table, F, w);
+    }
     if (!st) st = launch_assign(ctx, s, table, F, w);
     if (!st) st = launch_shade(ctx, s, table, F, w);
     return st;
@@ -271,7 +290,12 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         set_batch_shape(b, it != ctx->chain_tuned.end() ? it->second : vren_chain_shape{});
     }
     hipStream_t s = nullptr;
-    st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);
+    // the second stream and events of a concurrent light stage (default priority, like every stream here)
+    if (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+        st = vren_fail(ctx, VREN_EHIP, "batch: second stream or its events could not be created");
+    if (!st) st = vren_upload_frames(ctx, s, b->d_table, b->host.data(), n_frames);
     if (!st) st = vren_upload_frames(ctx, s, b->d_single, single.data(), n_frames);
     // the BVH leaf groups past each frame's last light, written once: every run of the chain skips them
     if (!st && leaf_base) st = launch_bvh(ctx, s, b->d_table, n_frames, leaf_base, true, nullptr, kLeavesPadding);
@@ -403,8 +427,9 @@ VREN_EXPORT int vren_batch_autotune(vren_batch* b, vren_stream_t stream, uint32_
     const uint32_t ppts[2] = {1, 2}, wpcs[4] = {8, 16, 24, 32}, bits[2] = {4, 8};
     for (uint32_t a = 0; a < 2 && !st; a++)
         for (uint32_t c = 0; c < 4 && !st; c++)
-            for (uint32_t d = 0; d < 2 && !st; d++) {
-                const vren_chain_shape cand{ppts[a], wpcs[c], bits[d], 0u};
+            for (uint32_t d = 0; d < 2 && !st; d++)
+            for (uint32_t e = 0; e < 2 && !st; e++) {
+                const vren_chain_shape cand{ppts[a], wpcs[c], bits[d], e};
                 set_batch_shape(b, cand);
                 st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots); // warm-up
                 if (!st && hipEventRecord(e0, s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch_autotune: record");
@@ -439,6 +464,9 @@ VREN_EXPORT void vren_batch_destroy(vren_batch* b)
     if (b->exec) vren_teardown(hipGraphExecDestroy(b->exec));
     if (b->graph) vren_teardown(hipGraphDestroy(b->graph));
     if (b->capture) vren_teardown(hipStreamDestroy(b->capture));
+    if (b->side) vren_teardown(hipStreamDestroy(b->side));
+    if (b->fork) vren_teardown(hipEventDestroy(b->fork));
+    if (b->join) vren_teardown(hipEventDestroy(b->join));
     if (b->arena) vren_teardown(hipFree(b->arena));
     if (b->work) vren_teardown(hipFree(b->work));
     if (b->d_table) vren_teardown(hipFree(b->d_table));

@@ -36,6 +36,7 @@ struct ChainWork {
     uint32_t shade_ppt = 0;
     uint32_t assign_wpc = 0;
     uint32_t light_bits = 0;
+    uint32_t light_concurrent = 0; // batched runs: the light BVH on a second stream beside the cluster stage
     // optional per-kernel timing: an event is recorded after every kernel of the chain
     hipEvent_t* events = nullptr;
     int* n_events = nullptr;
