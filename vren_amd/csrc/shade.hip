@@ -331,7 +331,8 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     o.z = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(acc[2]) * gexp);
     o.w = 1.0f;
     const vf4 ov = {o.x, o.y, o.z, o.w};
-    *((__attribute__((address_space(1))) vf4*)fd.out + p) = ov;
+    // a streaming store: the radiance is not read again by the chain (1.4 % of the kernel against a cached store)
+    __builtin_nontemporal_store(ov, (__attribute__((address_space(1))) vf4*)fd.out + p);
 }
 
 // ---- material resolve (shade.comp:89-96) -------------------------------------------------------------------------
