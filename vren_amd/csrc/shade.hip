@@ -44,11 +44,18 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
 {
     PixelIn o;
     // view depth and the background test exactly as the oracle evaluates them (shade.comp:75-86): the projection's
-    // inverse has zero x/y coefficients in its z and w rows, so r[2] and r[3] do not depend on the pixel's x/y
-    double v[4] = {(double)gx * fd.inv_W, (double)gy * fd.inv_H, (double)depth, 1.0};
-    v[1] = 1.0 - v[1];
-    v[0] = v[0] * 2.0 - 1.0;
-    v[1] = v[1] * 2.0 - 1.0;
+    // inverse has zero x/y coefficients in its z and w rows, so r[2] and r[3] do not depend on the pixel's x/y.
+    // The NDC x/y (v[0], v[1]) are formed only on the paths that read them.
+    const double dz = (double)depth;
+    auto ndc = [&](double v[4]) {
+        v[0] = (double)gx * fd.inv_W;
+        v[1] = (double)gy * fd.inv_H;
+        v[2] = dz;
+        v[3] = 1.0;
+        v[1] = 1.0 - v[1];
+        v[0] = v[0] * 2.0 - 1.0;
+        v[1] = v[1] * 2.0 - 1.0;
+    };
     double r[4];
     if (fd.proj_xy_sparse) {
         // one multiply-add each: the NDC transform folded into the row's two nonzero coefficients (the x/y of the
@@ -56,6 +63,8 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
         r[0] = fd.rec_xy[0] * (double)gx + fd.rec_xy[1];
         r[1] = fd.rec_xy[2] * (double)gy + fd.rec_xy[3];
     } else {
+        double v[4];
+        ndc(v);
         for (int i = 0; i < 2; i++)
             r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
                     fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
@@ -67,9 +76,11 @@ __device__ __forceinline__ PixelIn reconstruct(const FrameDev& fd, uint32_t gx, 
 #pragma clang fp contract(off)
         if (fd.proj_zw_xy_zero) {
             // ((0 x + 0 y) + m2 depth) + m3 is m2 depth + m3 exactly (up to the sign of a zero)
-            r[2] = fd.inv_proj_d[2 * 4 + 2] * v[2] + fd.inv_proj_d[3 * 4 + 2];
-            r[3] = fd.inv_proj_d[2 * 4 + 3] * v[2] + fd.inv_proj_d[3 * 4 + 3];
+            r[2] = fd.inv_proj_d[2 * 4 + 2] * dz + fd.inv_proj_d[3 * 4 + 2];
+            r[3] = fd.inv_proj_d[2 * 4 + 3] * dz + fd.inv_proj_d[3 * 4 + 3];
         } else {
+            double v[4];
+            ndc(v);
             for (int i = 2; i < 4; i++)
                 r[i] = ((fd.inv_proj_d[0 * 4 + i] * v[0] + fd.inv_proj_d[1 * 4 + i] * v[1]) +
                         fd.inv_proj_d[2 * 4 + i] * v[2]) + fd.inv_proj_d[3 * 4 + i] * v[3];
@@ -148,7 +159,8 @@ __device__ __forceinline__ void apply_light(const Surface& s, const float L[3], 
 // per-pixel constants of the forward shade, arranged for shade_light
 struct ShadeSurface {
     float V[3], N[3];
-    float A[3];   // albedo (1 - metallic) / PI: the diffuse term of a light is (1 - F) A
+    float GA[3];  // (1 - F0) albedo (1 - metallic) / PI: the diffuse term of a light is (1 - f5) GA
+    float G[3];   // 1 - F0
     float F0[3];
     float k;      // roughness, the Schlick-GGX k (pbr.glsl:26-40)
     float nn;     // 1 - |N|^2
@@ -159,7 +171,8 @@ struct ShadeSurface {
 
 // pbr_apply_light (pbr.glsl:42-73) for a light direction L (from the light towards the surface), the light's
 // colour and its attenuation: acc += max(((1 - F) A + F spec) colour att n.l, 0) per channel, with 1 - F taken as
-// (1 - F0)(1 - f5), which it equals (F = F0 + (1 - F0) f5), without the cancellation of 1 - F
+// (1 - F0)(1 - f5), which it equals (F = F0 + (1 - F0) f5), without the cancellation of 1 - F. The weight
+// w = att n.l is positive (both factors are tested), so max(x w, 0) = max(x, 0) w and w leaves the clamp.
 __device__ __forceinline__ void shade_light(const ShadeSurface& s, const float L[3], const float col[3], float att,
                                             float acc[3])
 {
@@ -188,9 +201,8 @@ __device__ __forceinline__ void shade_light(const ShadeSurface& s, const float L
     const float w = att * NdotL;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float G = 1.0f - s.F0[c];
-        const float F = s.F0[c] + G * f5;
-        acc[c] += fmaxf(((G * om5) * s.A[c] + F * spec) * (col[c] * w), 0.0f);
+        const float F = s.F0[c] + s.G[c] * f5;
+        acc[c] += fmaxf((s.GA[c] * om5 + F * spec) * col[c], 0.0f) * w;
     }
 }
 
@@ -295,7 +307,8 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     const float albedo[3] = {ar.x, ar.y, ar.z};
     for (int c = 0; c < 3; c++) {
         s.F0[c] = 0.04f * km + albedo[c] * metal;
-        s.A[c] = albedo[c] * km * kInvPi;
+        s.G[c] = 1.0f - s.F0[c];
+        s.GA[c] = s.G[c] * (albedo[c] * km * kInvPi);
     }
     const float NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
     s.a2ggxV = s.a2 * (NdotV * frcp(NdotV * (1.0f - rough) + rough));
