@@ -7,7 +7,8 @@
 // holds a handful), and the unique counts of a group of 8 tiles are chained to the previous groups of the same frame
 // by a decoupled look-back. This replaces the reference's atomicAdd allocation (whose order across tiles is nondeterministic,
 // find_unique_clusters.comp:100-103) by the canonical row-major tile order, and needs no separate scan dispatch.
-// Each pixel finds its key among the tile's unique keys by binary search and stores the frame-wide cluster index.
+// Each pixel keeps the index of its key in the tile's list from the round that emitted the key and stores the
+// frame-wide cluster index.
 //
 // Defined behaviour where the reference has none: pixels of partial tiles sample (x mod W, y mod H) like its
 // REPEAT sampler (gbuffer.cpp:28-34) and do not store a reference; a view depth below near gives slice 0; keys
@@ -336,45 +337,33 @@ __global__ __launch_bounds__(kClusterWaves * 64, 8) void clusters_kernel(const F
                 }
             }
         }
-        // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key); lanes
-        // with no key left offer 0xFFFFFFFF, which only wins when some lane still holds that very key
-        uint32_t rem = 0xFFFFu;
-        while (ballot(rem != 0u)) {
-            uint32_t mine = 0xFFFFFFFFu;
+        // the tile's unique keys in ascending order, by repeated wave minima (one round per distinct key). Every
+        // key of the tile has the tile's ids in its low 16 bits, so the keys compare by their high halves; a
+        // pixel whose key has been emitted takes 0x10000 + the key's index in the list, above every high half
+        // still pending, and keeps it as its index (no search afterwards)
+        const uint32_t low = key[0] & 0xFFFFu;
 #pragma unroll
-            for (int q = 0; q < 16; q++)
-                if ((rem >> q) & 1u) mine = min(mine, key[q]);
+        for (int q = 0; q < 16; q++) key[q] >>= 16;
+        for (;;) {
+            uint32_t mine = key[0];
+#pragma unroll
+            for (int q = 1; q < 16; q++) mine = min(mine, key[q]);
             const uint32_t m = wave_min_u32(mine);
-            if (lane == 0) uq[u] = m;
-            u++;
+            if (m > 0xFFFFu) break; // every key emitted (wave-uniform)
+            if (lane == 0) uq[u] = (m << 16) | low;
 #pragma unroll
-            for (int q = 0; q < 16; q++)
-                if (key[q] == m) rem &= ~(1u << q);
+            for (int q = 0; q < 16; q++) key[q] = key[q] == m ? 0x10000u + u : key[q];
+            u++;
         }
         __builtin_amdgcn_wave_barrier();
     }
-    // the group's aggregate is published as soon as its tiles' unique counts are known, so that the next groups'
-    // walks find it while this group still searches its pixels' indices (8 % of the kernel against publishing
-    // at the walk)
+    // the group's aggregate is published as soon as its tiles' unique counts are known, before this group's own
+    // walk, so that the next groups' walks can sum it without waiting for the walk
     if (lane == 0) s_u[wave] = u;
     __syncthreads();
     if (wave == 0) {
         const uint32_t agg = wave_sum(lane < kClusterWaves ? s_u[lane] : 0u);
         if (lane == 0) wave_lookback_publish(tile_status + fd.tile_base / VREN_TILE_GROUP, t0 / VREN_TILE_GROUP, agg);
-    }
-    if (live) {
-        // each pixel's index in the tile's unique list (binary search), found before the look-back so that it
-        // overlaps the wait for the previous groups
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            uint32_t lo = 0, hi = u;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (uq[mid] <= key[i]) lo = mid;
-                else hi = mid;
-            }
-            key[i] = lo;
-        }
     }
     // decoupled look-back over the frame's tile groups (row-major order), one walk per workgroup: a round trip of
     // the walk covers 64 groups = 512 tiles
@@ -403,7 +392,7 @@ __global__ __launch_bounds__(kClusterWaves * 64, 8) void clusters_kernel(const F
         if (y >= fd.H) continue;
         uint32_t ref[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) ref[q] = prefix + key[4 * j + q];
+        for (int q = 0; q < 4; q++) ref[q] = (prefix - 0x10000u) + key[4 * j + q]; // key = 0x10000 + index
         uint32_t* row = fd.refs + (uint64_t)y * fd.W;
         if (x0 + 3 < fd.W && ((reinterpret_cast<uintptr_t>(row + x0) & 15) == 0)) {
             *reinterpret_cast<uint4*>(row + x0) = make_uint4(ref[0], ref[1], ref[2], ref[3]);
