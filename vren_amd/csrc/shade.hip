@@ -710,7 +710,6 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     __shared__ float s_g[kStage * 7];
     __shared__ uint32_t s_li[kStage];
     __shared__ uint32_t s_min[4], s_max[4];
-    __shared__ uint32_t s_start, s_n, s_staged;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
     const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
     const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
@@ -727,44 +726,46 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     float4 ar = make_float4(0.f, 0.f, 0.f, 0.f);
     float metal = 0.0f;
     float4 go = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 gar = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t cl_raw = 0xFFFFFFFFu;
+    // every per-pixel load is issued before any of them is used: the reference and the pixel's running material
+    // gradient along with the G-buffer, not after the reconstruction and the light loops
     if (in) {
         const float depth = fd.depth[p];
         nb = reinterpret_cast<const uint2*>(fd.normals)[p];
         ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
         metal = fd.metallic[p];
         go = reinterpret_cast<const float4*>(grad_out)[p];
+        cl_raw = fd.refs[p];
+        gar = reinterpret_cast<const float4*>(g_ar)[p];
         px = reconstruct(fd, gx, gy, depth);
     }
-    const uint32_t cl = px.shaded ? fd.refs[p] : 0xFFFFFFFFu;
-    const bool has_list = px.shaded && fd.L > 0 && cl < fd.key_cap;
-    uint32_t cmin = has_list ? cl : 0xFFFFFFFFu, cmax = has_list ? cl : 0u;
+    const bool has_list = px.shaded && fd.L > 0 && cl_raw < fd.key_cap;
+    // the pixel's list [off, end) (clamped to the index capacity), then the workgroup's range of list entries
+    // reduced from its pixels' lists, so that every list lies inside the staged range
+    uint32_t off = 0, end = 0;
+    if (has_list) {
+        const uint32_t o = fd.offsets[cl_raw];
+        off = min(o, fd.index_cap);
+        const uint64_t e = (uint64_t)o + fd.counts[cl_raw];
+        end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
+    }
+    uint32_t rmin = has_list ? off : 0xFFFFFFFFu, rmax = has_list ? end : 0u;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
-        cmin = min(cmin, shfl_xor(cmin, m));
-        cmax = max(cmax, shfl_xor(cmax, m));
+        rmin = min(rmin, shfl_xor(rmin, m));
+        rmax = max(rmax, shfl_xor(rmax, m));
     }
     if (lane == 0) {
-        s_min[wave] = cmin;
-        s_max[wave] = cmax;
+        s_min[wave] = rmin;
+        s_max[wave] = rmax;
     }
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t a = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
-        const uint32_t b = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
-        uint32_t start = 0, n = 0;
-        if (a != 0xFFFFFFFFu) {
-            start = min(fd.offsets[a], fd.index_cap);
-            const uint64_t e = (uint64_t)fd.offsets[b] + fd.counts[b];
-            const uint32_t end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
-            n = end > start ? end - start : 0;
-        }
-        s_start = start;
-        s_n = n;
-        s_staged = n <= kStage ? 1u : 0u;
-    }
-    __syncthreads();
-    const uint32_t start = s_start, nstage = s_n;
-    const bool staged = s_staged != 0;
+    const uint32_t ra = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+    const uint32_t rb = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+    const uint32_t start = ra == 0xFFFFFFFFu ? 0u : ra;
+    const uint32_t nstage = (ra != 0xFFFFFFFFu && rb > ra) ? rb - ra : 0u;
+    const bool staged = nstage <= kStage;
     const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
     const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
     if (staged) {
@@ -779,7 +780,7 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     }
     __syncthreads();
     Surface s;
-    uint32_t off = 0, end = 0, len = 0;
+    uint32_t len = 0;
     float i_gv = 0.0f;
     float gT[3] = {0.0f, 0.0f, 0.0f};
     float gp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -803,11 +804,6 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
         s.NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
         i_gv = frcp(s.NdotV * (1.0f - s.rough) + s.rough);
         s.ggxV = s.NdotV * i_gv;
-        if (has_list) {
-            off = fd.offsets[cl];
-            const uint64_t e = (uint64_t)off + fd.counts[cl];
-            end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
-        }
         // pass 1: the pixel's colour (the gamma derivative needs it)
         float acc[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t e = off; e < end; e++) {
@@ -894,13 +890,12 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
         }
     }
     if (px.shaded) {
-        float4* pa = reinterpret_cast<float4*>(g_ar) + p;
-        float4 cur = *pa;
-        cur.x += gp[0];
-        cur.y += gp[1];
-        cur.z += gp[2];
-        cur.w += gp[3];
-        *pa = cur;
+        // only this thread touches pixel p's material gradient, so the value loaded with the pixel is current
+        gar.x += gp[0];
+        gar.y += gp[1];
+        gar.z += gp[2];
+        gar.w += gp[3];
+        reinterpret_cast<float4*>(g_ar)[p] = gar;
     }
     __syncthreads();
     if (staged) {
