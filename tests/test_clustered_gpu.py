@@ -81,6 +81,31 @@ def test_light_bvh_c2(vctx, co, c3, mode):
     assert np.array_equal(bvh.view(np.uint32).reshape(-1, 8)[:, :7], obvh.view(np.uint32).reshape(-1, 8)[:, :7])
 
 
+def test_bucket_sort_light_morton_codes_c2(vctx, co, c3):
+    # configuration 2's BucketSort of the lights into 64 buckets, on the lights' own 30-bit Morton codes (scene AABB)
+    # rather than random keys: (code, light index) pairs in light order, bucket = the code's top 6 bits
+    cam = c3["cam"]
+    cf = ol.cfg(morton_bits=10, morton_aabb_mode=1, aabb=((-50, -50, 0), (50, 50, 100)))
+    _, codes, _ = gpu_light_bvh(vctx, co, c3["pos"], c3["lig"], cam, cf)
+    L = codes.shape[0]
+    pairs = np.zeros((L, 2), np.uint32)
+    pairs[codes[:, 1]] = codes                       # back to light order
+    assert np.array_equal(pairs[:, 1], np.arange(L, dtype=np.uint32))
+    assert int(pairs[:, 0].max()) < (1 << 30)
+    out = empty_dev(8 * L)
+    offs = empty_dev(4 * 64)
+    vctx.bucket_sort(to_dev(pairs), L, out, offs, 64, 24)
+    opairs, ooffs = co.o.bucket_sort(pairs, 64, 24)
+    got, goffs = from_dev(out, np.uint32, (L, 2)), from_dev(offs, np.uint32, (64,))
+    assert np.array_equal(got, opairs) and np.array_equal(goffs, ooffs)
+    # the buckets partition the Morton order: bucket b holds exactly the codes with top bits b, in light order
+    for b in range(64):
+        lo, hi = int(goffs[b]), int(goffs[b + 1]) if b < 63 else L
+        seg = got[lo:hi]
+        assert ((seg[:, 0] >> 24) == b).all() and (np.diff(seg[:, 1].astype(np.int64)) > 0).all()
+    assert len(np.unique(pairs[:, 0] >> 24)) > 8   # the codes spread over many buckets
+
+
 @pytest.mark.parametrize("L", [1, 31, 32, 33, 1000, 4097, 40000])
 def test_light_bvh_sizes(vctx, co, L):
     cam = ol.camera(640, 360)
