@@ -425,25 +425,25 @@ VREN_EXPORT int vren_batch_autotune(vren_batch* b, vren_stream_t stream, uint32_
     float top_ms = -1.0f;
     int st = VREN_OK;
     const uint32_t ppts[2] = {1, 2}, wpcs[4] = {8, 16, 24, 32}, bits[2] = {4, 8};
-    for (uint32_t a = 0; a < 2 && !st; a++)
-        for (uint32_t c = 0; c < 4 && !st; c++)
-            for (uint32_t d = 0; d < 2 && !st; d++)
-            for (uint32_t e = 0; e < 2 && !st; e++) {
-                const vren_chain_shape cand{ppts[a], wpcs[c], bits[d], e};
-                set_batch_shape(b, cand);
-                st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots); // warm-up
-                if (!st && hipEventRecord(e0, s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch_autotune: record");
-                for (uint32_t r = 0; r < reps && !st; r++) st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots);
-                float ms = 0.0f;
-                if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    st = vren_fail(ctx, VREN_EHIP, "batch_autotune: timing a shape failed");
-                ms /= (float)reps;
-                if (!st && (top_ms < 0.0f || ms < top_ms)) {
-                    top = cand;
-                    top_ms = ms;
-                }
-            }
+    // every shape of the grid: {1, 2} pixels per thread x {8, 16, 24, 32} waves per CU x {4, 8} bits x light stage
+    // in line or concurrent, as index i = ((a * 4 + c) * 2 + d) * 2 + e
+    for (uint32_t i = 0; i < 2 * 4 * 2 * 2 && !st; i++) {
+        const uint32_t a = i / 16, c = (i / 4) % 4, d = (i / 2) % 2, e = i % 2;
+        const vren_chain_shape cand{ppts[a], wpcs[c], bits[d], e};
+        set_batch_shape(b, cand);
+        st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots); // warm-up
+        if (!st && hipEventRecord(e0, s) != hipSuccess) st = vren_fail(ctx, VREN_EHIP, "batch_autotune: record");
+        for (uint32_t r = 0; r < reps && !st; r++) st = run_chain(b, s, b->d_table, b->F, b->w, b->total_live_slots);
+        float ms = 0.0f;
+        if (!st && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            st = vren_fail(ctx, VREN_EHIP, "batch_autotune: timing a shape failed");
+        ms /= (float)reps;
+        if (!st && (top_ms < 0.0f || ms < top_ms)) {
+            top = cand;
+            top_ms = ms;
+        }
+    }
     vren_teardown(hipEventDestroy(e0));
     vren_teardown(hipEventDestroy(e1));
     if (st) {
