@@ -44,6 +44,28 @@ def test_reduce_u32_multi_block_and_in_place(vctx, oracle, c1_keys):
         assert np.array_equal(from_dev(out, np.uint32, (3 * p,)), oracle.reduce_tree_u32(x, op, blocks=3)), n
 
 
+def test_reduce_u32_two_passes_in_sequence(vctx, oracle, c1_keys):
+    # one segment of 16 384 < p <= 2^26 elements (two up-sweep passes, the second over the first's compact tile
+    # totals): calls in a row with the scalar total in between (both use the context's scratch), a partial last
+    # tile, in place, and 2^24 elements
+    for n, op in ((1 << 20, 0), (100000, 2), ((1 << 16) + 3, 1), (1 << 20, 1)):
+        x = c1_keys[:n]
+        p = 1 << (n - 1).bit_length()
+        out = empty_dev(4 * p, fill=0xEF)
+        vctx.reduce_u32(to_dev(x), n, out, op)
+        assert np.array_equal(from_dev(out, np.uint32, (p,)), oracle.reduce_tree_u32(x, op)), (n, op)
+        tot = empty_dev(4)
+        vctx.reduce_total_u32(to_dev(x), n, tot, op)
+        assert from_dev(tot, np.uint32)[0] == oracle.reduce_total_u32(x, op)
+    d = to_dev(c1_keys)
+    vctx.reduce_u32(d, N_C1, d, 0)
+    assert np.array_equal(from_dev(d, np.uint32, (N_C1,)), oracle.reduce_tree_u32(c1_keys, 0))
+    big = oracle.pcg32(7, 54, 1 << 24)
+    out = empty_dev(4 << 24)
+    vctx.reduce_u32(to_dev(big), 1 << 24, out, 0)
+    assert np.array_equal(from_dev(out, np.uint32, (1 << 24,)), oracle.reduce_tree_u32(big, 0))
+
+
 @pytest.mark.parametrize("op", [0, 1, 2])
 def test_reduce_vec4_tree_bit_exact(vctx, oracle, op):
     # random (non-integer) floats: bit-exact only if the combination tree is the reference's
