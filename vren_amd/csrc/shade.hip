@@ -436,14 +436,19 @@ template <int PPT, bool MAT>
 // at most 64 VGPRs: eight waves per SIMD (the split light loops would otherwise take 66 and seven)
 __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restrict__ frames,
                                                     const uint32_t* __restrict__ tile_frame,
-                                                    const uint2* __restrict__ tile_range)
+                                                    const uint2* __restrict__ tile_range, uint32_t total_tiles)
 {
     constexpr uint32_t BPT = 4 / PPT; // workgroups per tile
     static_assert(kFwdStage == 256, "one staged entry per thread");
     __shared__ StagedLight s_light[kFwdStage];
     __shared__ uint32_t s_min[4], s_max[4];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
-    const uint32_t g = blockIdx.x / BPT, sub = blockIdx.x % BPT;
+    // the BPT workgroups of a tile run on one XCD (workgroup b runs on XCD b mod 8) and consecutive tiles on
+    // consecutive XCDs, so that a tile's list range, indices and light records are fetched into one L2; the grid is
+    // rounded up to whole rounds of 8 tiles
+    const uint32_t k = blockIdx.x / 8;
+    const uint32_t g = (k / BPT) * 8 + blockIdx.x % 8, sub = k % BPT;
+    if (g >= total_tiles) return;
     const uint32_t f = frame_of_tile(tile_frame, g);
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
@@ -921,14 +926,16 @@ int launch_shade(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32_
         VREN_LAUNCH_CHECK(ctx);
     }
     const uint32_t* tf = F > 1 ? w.tile_frame : nullptr;
+    const uint32_t rounds = (w.total_tiles + 7) / 8; // rounds of 8 tiles, one per XCD
     if (w.shade_ppt == 2) {
-        const uint32_t grid = w.total_tiles * 2;
-        if (w.any_materials) shade_kernel<2, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
-        else shade_kernel<2, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+        const uint32_t grid = rounds * 8 * 2;
+        if (w.any_materials) shade_kernel<2, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range, w.total_tiles);
+        else shade_kernel<2, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range, w.total_tiles);
     } else {
-        const uint32_t grid = w.total_tiles * (4 / kShadePPT);
-        if (w.any_materials) shade_kernel<kShadePPT, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
-        else shade_kernel<kShadePPT, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range);
+        const uint32_t grid = rounds * 8 * (4 / kShadePPT);
+        if (w.any_materials)
+            shade_kernel<kShadePPT, true><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range, w.total_tiles);
+        else shade_kernel<kShadePPT, false><<<grid, 256, 0, s>>>(d_frames, tf, w.tile_range, w.total_tiles);
     }
     VREN_LAUNCH_CHECK(ctx);
     chain_mark(w, s);
