@@ -707,7 +707,8 @@ __device__ __forceinline__ float wave_sum_components(const float g[7])
     return w;
 }
 
-__global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __restrict__ frames, uint32_t F, const float* grad_out,
+__global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __restrict__ frames, uint32_t F,
+                                                             uint32_t total_tiles, const float* grad_out,
                                                              float* g_pos, float* g_light, float* g_ar)
 {
     __shared__ float4 s_lpos[kStage];
@@ -716,7 +717,11 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     __shared__ uint32_t s_li[kStage];
     __shared__ uint32_t s_min[4], s_max[4];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / VREN_WAVE;
-    const uint32_t g = blockIdx.x / 4, sub = blockIdx.x % 4;
+    // a tile's four strips on one XCD, consecutive tiles on consecutive XCDs (as the forward shade), the grid rounded
+    // up to whole rounds of 8 tiles
+    const uint32_t k = blockIdx.x / 8;
+    const uint32_t g = (k / 4) * 8 + blockIdx.x % 8, sub = k % 4;
+    if (g >= total_tiles) return;
     const uint32_t f = find_frame_wave(frames, F, g, [](const FrameDev& d) { return d.tile_base; });
     const FrameDev& fd = frames[f];
     const uint32_t t = g - fd.tile_base;
@@ -1093,8 +1098,9 @@ extern "C" VREN_EXPORT int vren_shade_backward(vren_ctx* ctx, vren_stream_t stre
     FrameDev* slot = vren_next_frame_slot(ctx);
     st = vren_upload_frames(ctx, s, slot, &f, 1);
     if (st) return st;
-    shade_backward_kernel<<<f.Sx * f.Sy * 4, 256, 0, s>>>(slot, 1, grad_output, grad_light_positions, grad_point_lights,
-                                                          grad_albedo_roughness);
+    const uint32_t tiles = f.Sx * f.Sy;
+    shade_backward_kernel<<<(tiles + 7) / 8 * 32, 256, 0, s>>>(slot, 1, tiles, grad_output, grad_light_positions,
+                                                               grad_point_lights, grad_albedo_roughness);
     VREN_LAUNCH_CHECK(ctx);
     return VREN_OK;
 }
