@@ -338,7 +338,7 @@ __device__ __forceinline__ void for_each_cluster_chunk(const FrameDev* __restric
 }
 
 // for_each_cluster_chunk for the paired count pass: pre(fd, c, frame) runs per lane (lane i: cluster i of the
-// chunk), then fn(k, n, cA, cB) for k = 0, 2, 4, ... with cA, cB the frame-local indices of clusters k and k + 1
+// chunk), then fn(k, n, cA, cB) for k = 0, 2, 4, ... with cA, cB the frame-local indices of clusters k and k + 1.
 template <typename Pre, typename Fn>
 __device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restrict__ frames, uint32_t F, uint32_t gw,
                                                         uint32_t stride, Pre pre, Fn fn)
@@ -356,7 +356,17 @@ __device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restri
         const uint32_t total = shfl(incl, VREN_WAVE - 1);
         const uint32_t cs = chunk_size(total, stride);
         const uint32_t chunks = (uint32_t)(((uint64_t)total + cs - 1) / cs);
-        for (uint32_t chunk = gw; chunk < chunks; chunk += stride) {
+        // the chunks in eight contiguous blocks, one per XCD (workgroup b runs on XCD b mod 8), each block strided
+        // over that XCD's waves: an L2 then holds the BVHs of an eighth of the frames (a grid that is not a
+        // multiple of 8 workgroups keeps the plain stride)
+        uint32_t first = gw, end = chunks, step = stride;
+        if (gridDim.x % 8u == 0u) {
+            const uint32_t xcd = blockIdx.x % 8u, wpb = blockDim.x / VREN_WAVE;
+            first = (uint32_t)((uint64_t)chunks * xcd / 8u) + (blockIdx.x / 8u) * wpb + gw % wpb;
+            end = (uint32_t)((uint64_t)chunks * (xcd + 1u) / 8u);
+            step = stride / 8u;
+        }
+        for (uint32_t chunk = first; chunk < end; chunk += step) {
             const uint32_t c0 = chunk * cs;
             const uint32_t c = c0 + lane;
             uint32_t pos = 0;
