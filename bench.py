@@ -42,15 +42,16 @@ SHADE_OPS_PER_PAIR = 102.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def profiled_traffic():
-    """HBM bytes per launch of each kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+def profiled_traffic(label="traffic-bytes-per-launch"):
+    """HBM bytes per launch of each kernel (label "traffic-bytes-per-launch") or per call of each row's entry point
+    (label "row-traffic-bytes-per-call"), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     (profiles/hbm_traffic_pmc.md; counters scaled by a 1 GiB copy-kernel calibration)."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic_pmc.md")) as f:
             text = f.read()
     except OSError:
         return {}
-    m = re.search(r"traffic-bytes-per-launch: (.*)", text)
+    m = re.search(label + r": (.*)", text)
     if not m:
         return {}
     out = {}
@@ -187,11 +188,13 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     sort_traffic = None
     if log2n == 26:
         if algo == 0:  # histogram of every pass + onesweep passes
-            parts = [traffic.get("rs_hist_kernel<%d>" % bits)] + \
-                    [traffic.get("rs_onesweep_kernel<%d, %d, %d, true, false>" % (bits, kpt, waves))] * passes
+            parts = [traffic.get("rs_hist_kernel<%d, false>" % bits)] + \
+                    [traffic.get("rs_onesweep_kernel<%d, %d, %d, true, false, false, false, false>"
+                                 % (bits, kpt, waves))] * passes
         else:          # per pass: tile counts, scan, scatter
-            parts = [traffic.get("rs_upsweep_kernel<%d, %d, %d>" % (bits, kpt, waves)), traffic.get("scan_kernel"),
-                     traffic.get("rs_onesweep_kernel<%d, %d, %d, true, true>" % (bits, kpt, waves))] * passes
+            parts = [traffic.get("rs_upsweep_kernel<%d, %d, %d>" % (bits, kpt, waves)), traffic.get("rs_tile_scan"),
+                     traffic.get("rs_onesweep_kernel<%d, %d, %d, true, true, false, false, false>"
+                                 % (bits, kpt, waves))] * passes
         if all(x is not None for x in parts):
             sort_traffic = sum(parts)
     return {
@@ -277,6 +280,10 @@ def bench_primitives(ctx, vren_amd, reps, log2n=26, bucket_log2=24):
     del lpos, ldir
     res.update(bench_shade_frame(ctx, vren_amd, reps))
     res.update(bench_pyramid(ctx, vren_amd, reps))
+    row_traffic = profiled_traffic("row-traffic-bytes-per-call")
+    for name, r in res.items():
+        if "bound" in r:
+            r["traffic"] = row_traffic.get(name)
     return res
 
 
