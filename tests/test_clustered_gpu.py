@@ -520,6 +520,44 @@ def test_batch_config4_64_heterogeneous_frames(vctx, co):
     b.close()
 
 
+def test_batch_long_lists_and_many_clusters(vctx, co):
+    """The two ways a batch's writing pass gets a light list: copied from the slot the counting pass kept (lists of
+    at most 192 lights, for the first 32 768 clusters of a frame) or traversed again. Frame 0 (640x360, 3000
+    lights with five times their radius) has lists of 65-192 and of more than 192 lights; frame 1 (1920x1080 with
+    its depth and normals shuffled over the pixels, 2000 lights) has more than 32 768 unique clusters, those past
+    32 768 with lights. Counts, offsets and lists bit for bit against the oracle after per-frame and after graph
+    launches."""
+    import torch
+    import vren_amd
+    specs = [(640, 360, 3000, 5100), (1920, 1080, 2000, 5200)]
+    descs, keep = make_frames_batch(vctx, co, specs)
+    keep[0]["l"].view(torch.float32).view(3000, 4)[:, 3] *= 5.0
+    W, H = 1920, 1080
+    perm = torch.from_numpy(np.random.default_rng(5).permutation(W * H)).cuda()
+    d = keep[1]["d"].view(torch.float32)
+    d.copy_(d[perm])
+    n = keep[1]["n"].view(torch.int16).view(W * H, 4)
+    n.copy_(n[perm])
+    torch.cuda.synchronize()
+    b = vren_amd.Batch(vctx, descs, ol.cfg())
+    for mode in (b.PER_FRAME, b.GRAPH):
+        for k in keep:
+            k["o"].fill_(0xFF)
+        b.run(mode)
+        torch.cuda.synchronize()
+        vctx.sync_status()
+        f0, f1 = _batch_host_frame(b, keep, 0), _batch_host_frame(b, keep, 1)
+        c0 = f0["counts"]
+        assert ((c0 > 64) & (c0 <= 192)).sum() > 0 and (c0 > 192).sum() > 0, c0.max()
+        assert f1["U"] > 32768 and f1["counts"][32768:].sum() > 0, f1["U"]
+        for i, f in ((0, f0), (1, f1)):
+            _batch_check_frame(co, ol.cfg(), i, f, n_shade=512)
+        print("\n[long lists, mode %d] frame 0: %d clusters, %d of 65-192 lights, %d over 192 (max %d); frame 1: %d "
+              "clusters" % (mode, f0["U"], ((c0 > 64) & (c0 <= 192)).sum(), (c0 > 192).sum(), c0.max(), f1["U"]),
+              flush=True)
+    b.close()
+
+
 @pytest.mark.parametrize("aabb_mode", [0, 1])
 def test_batch_sliced_encoder(vctx, co, aabb_mode):
     """The configuration's own key encoder (32 log-depth slices x 6 cube faces) through the batched chain: 1080p

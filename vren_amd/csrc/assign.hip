@@ -7,8 +7,9 @@
 // Here: persistent kernels over the clusters of every frame (their number is read on the device; it never returns
 // to the host), taken in chunks of up to 64 whose per-cluster work (the cluster AABB, the list bounds) is done one
 // cluster per lane. COUNT runs two traversals per wave, one per 32-lane half (the BVH's fan-out), each from its own
-// LDS stack, and keeps lists of at most 64 lights, sorted by rank, in per-cluster slots; one workgroup per frame
-// scans the counts into offsets; WRITE copies the kept lists and traverses again only for longer ones, with the
+// LDS stack, and keeps lists of at most VREN_LIST_SLOT (192) lights, sorted by rank, in per-cluster slots; one
+// workgroup per frame scans the counts into offsets; WRITE copies the kept lists and traverses again only for longer
+// ones, with the
 // whole wave testing two BVH blocks per step, one bit per hit light in a per-wave LDS bitmask, emitted in ascending
 // light index (the canonical order) by a popcount prefix over the bitmask words.
 // The cluster AABB and both tests are restated operation for operation (including the inverted y range of
@@ -532,8 +533,8 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
             const uint32_t out = __builtin_amdgcn_readlane(mine.offset, k);
             if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
                 // the counting pass kept this list
-                if (lane < cnt && (uint64_t)out + lane < fd.index_cap)
-                    fd.indices[out + lane] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + lane];
+                for (uint32_t i = lane; i < cnt && (uint64_t)out + i < fd.index_cap; i += VREN_WAVE)
+                    fd.indices[out + i] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + i];
                 return;
             }
             const Box box = cluster_aabb(fd, fd.keys[c]);
@@ -565,7 +566,8 @@ int launch_assign(vren_ctx* ctx, hipStream_t s, const FrameDev* d_frames, uint32
     while (wpb > 1 && per_wave * wpb > 150 * 1024) wpb--;
     if (per_wave > 150 * 1024) return vren_fail(ctx, VREN_EINVAL, "assign_lights: too many lights for the LDS bitmask");
     // two stacks and two lists per wave (one per half), the chunk's boxes and cluster frames
-    const size_t count_per_wave = (size_t)2 * (kStackHalf + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
+    constexpr size_t count_per_wave = (size_t)2 * (kStackHalf + VREN_LIST_SLOT) * 4 + VREN_WAVE * (sizeof(float) * 6 + 4);
+    static_assert(count_per_wave * 4 <= 160 * 1024 / 8, "assign_count: 8 workgroups of 4 waves must fit a CU's LDS");
     assign_count_kernel<<<persistent_blocks(ctx, 4, w.assign_wpc), 4 * VREN_WAVE, count_per_wave * 4, s>>>(d_frames, F,
                                                                                                     mask_words);
     VREN_LAUNCH_CHECK(ctx);

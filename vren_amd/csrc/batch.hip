@@ -204,7 +204,8 @@ VREN_EXPORT int vren_batch_create(vren_ctx* ctx, const vren_frame_desc* frames, 
         live_base += live;
         offs_base += (key_cap + VREN_SCAN_TILE - 1) / VREN_SCAN_TILE;
         max_L = std::max(max_L, L);
-        const uint32_t slot_cap = std::min<uint32_t>(key_cap, 1u << 16);
+        // list slots for the first 32 768 clusters of the frame (768 B each); clusters past them are traversed again
+        const uint32_t slot_cap = std::min<uint32_t>(key_cap, 1u << 15);
         f.slot_cap = L ? slot_cap : 0u;
         size_t* o = &off[i * 14];
         o[0] = arena; arena += a256((size_t)L * 16);                 // view_pos

@@ -8,8 +8,12 @@
 
 #include "common.hpp"
 
-// light lists of at most this many entries are kept by the counting traversal and copied by the writing pass
-#define VREN_LIST_SLOT 64u
+// light lists of at most this many entries are kept by the counting traversal and copied by the writing pass; a
+// longer list is traversed again by the writing pass. 192 entries keep every list of a 65 536-light 4K frame (the
+// longest lists of the heterogeneous 64-frame batch hold 147 lights; 0.27 % of its clusters hold more than 64, and
+// traversing those again cost 0.2 ms per batch) while the counting kernel's per-half LDS lists still let 8
+// workgroups share a CU
+#define VREN_LIST_SLOT 192u
 
 struct FrameDev {
     // sizes
