@@ -9,9 +9,8 @@
 // cluster per lane. COUNT runs two traversals per wave, one per 32-lane half (the BVH's fan-out), each from its own
 // LDS stack, and keeps lists of at most VREN_LIST_SLOT (192) lights, sorted by rank, in per-cluster slots; one
 // workgroup per frame scans the counts into offsets; WRITE copies the kept lists and traverses again only for longer
-// ones, with the
-// whole wave testing two BVH blocks per step, one bit per hit light in a per-wave LDS bitmask, emitted in ascending
-// light index (the canonical order) by a popcount prefix over the bitmask words.
+// ones, with the whole wave testing two BVH blocks per step, one bit per hit light in a per-wave LDS bitmask, emitted
+// in ascending light index (the canonical order) by a popcount prefix over the bitmask words.
 // The cluster AABB and both tests are restated operation for operation (including the inverted y range of
 // the two-corner AABB); padding leaves and invalid nodes (empty AABB) never match.
 #include "chain.hpp"
@@ -291,7 +290,8 @@ __device__ uint32_t traverse_halves(const TravFrame& tf, const Box& box, bool li
 // (one per lane).
 __device__ __forceinline__ uint32_t chunk_size(uint32_t total, uint32_t waves)
 {
-    return max(1u, min(VREN_WAVE, (uint32_t)(((uint64_t)total + 4ull * waves - 1) / (4ull * waves))));
+    // 32-bit arithmetic: total <= 64 frames x 2^20 clusters and waves < 2^20, so nothing here wraps
+    return max(1u, min(VREN_WAVE, (total + 4u * waves - 1u) / (4u * waves)));
 }
 
 // Clusters of every frame of the table are processed as one flattened range, in chunks of at most 64 consecutive
@@ -313,9 +313,9 @@ __device__ __forceinline__ void for_each_cluster_chunk(const FrameDev* __restric
         }
         const uint32_t incl = wave_inclusive_sum(nf);
         const uint32_t excl = incl - nf;
-        const uint32_t total = shfl(incl, VREN_WAVE - 1);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, VREN_WAVE - 1);
                 const uint32_t cs = chunk_size(total, stride);
-        const uint32_t chunks = (uint32_t)(((uint64_t)total + cs - 1) / cs);
+        const uint32_t chunks = (total + cs - 1u) / cs;
         for (uint32_t chunk = gw; chunk < chunks; chunk += stride) {
             const uint32_t c0 = chunk * cs;
             const uint32_t c = c0 + lane;
@@ -354,9 +354,9 @@ __device__ __forceinline__ void for_each_cluster_chunk2(const FrameDev* __restri
         }
         const uint32_t incl = wave_inclusive_sum(nf);
         const uint32_t excl = incl - nf;
-        const uint32_t total = shfl(incl, VREN_WAVE - 1);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, VREN_WAVE - 1);
         const uint32_t cs = chunk_size(total, stride);
-        const uint32_t chunks = (uint32_t)(((uint64_t)total + cs - 1) / cs);
+        const uint32_t chunks = (total + cs - 1u) / cs;
         // the chunks in eight contiguous blocks, one per XCD (workgroup b runs on XCD b mod 8), each block strided
         // over that XCD's waves: an L2 then holds the BVHs of an eighth of the frames (a grid that is not a
         // multiple of 8 workgroups keeps the plain stride)
@@ -417,7 +417,7 @@ __device__ __forceinline__ uint32_t emit_bitmask(uint32_t* bitmask, uint32_t* su
                 }
             }
         }
-        out += shfl(incl, VREN_WAVE - 1);
+        out += __builtin_amdgcn_readlane(incl, VREN_WAVE - 1);
     }
     return out;
 }
@@ -533,8 +533,12 @@ __global__ __launch_bounds__(256, 8) void assign_write_kernel(const FrameDev* __
             const uint32_t out = __builtin_amdgcn_readlane(mine.offset, k);
             if (fd.list_slots && c < fd.slot_cap && cnt <= VREN_LIST_SLOT) {
                 // the counting pass kept this list
-                for (uint32_t i = lane; i < cnt && (uint64_t)out + i < fd.index_cap; i += VREN_WAVE)
-                    fd.indices[out + i] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + i];
+#pragma unroll
+                for (uint32_t r = 0; r < VREN_LIST_SLOT / VREN_WAVE; r++) {
+                    const uint32_t i = lane + r * VREN_WAVE;
+                    if (i < cnt && (uint64_t)out + i < fd.index_cap)
+                        fd.indices[out + i] = fd.list_slots[(size_t)c * VREN_LIST_SLOT + i];
+                }
                 return;
             }
             const Box box = cluster_aabb(fd, fd.keys[c]);

@@ -52,15 +52,25 @@ __device__ __forceinline__ T shfl_up(T v, unsigned d) { return __shfl_up(v, d, V
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int m) { return __shfl_xor(v, m, VREN_WAVE); }
 
-// inclusive prefix sum over the wave
+// one DPP move of v (source lanes outside the row, or rows that row_mask leaves out, give 0)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp0(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
+}
+
+// inclusive prefix sum over the wave on the DPP network, with no lane-address registers (a shuffle-based scan keeps
+// six per-lane permute addresses live, which the register-capped persistent kernels spilled to scratch):
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes; row_bcast:15 adds lane 15's sum to row 1 and lane 47's to row 3,
+// row_bcast:31 adds lane 31's to rows 2 and 3
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 {
-    uint32_t lane = lane_id();
-#pragma unroll
-    for (unsigned d = 1; d < VREN_WAVE; d <<= 1) {
-        uint32_t o = shfl_up(v, d);
-        if (lane >= d) v += o;
-    }
+    v += dpp0<0x111, 0xF>(v); // row_shr:1
+    v += dpp0<0x112, 0xF>(v); // row_shr:2
+    v += dpp0<0x114, 0xF>(v); // row_shr:4
+    v += dpp0<0x118, 0xF>(v); // row_shr:8
+    v += dpp0<0x142, 0xA>(v); // row_bcast:15
+    v += dpp0<0x143, 0xC>(v); // row_bcast:31
     return v;
 }
 

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
     if (wave == 0) {
         const uint32_t x = s_part[lane];
         const uint32_t xi = wave_inclusive_sum(x);
-        const uint32_t agg = shfl(xi, VREN_WAVE - 1);
+        const uint32_t agg = __builtin_amdgcn_readlane(xi, VREN_WAVE - 1);
         const uint32_t prefix = wave_lookback_walk(status + (uint64_t)seg * tiles_per_seg, t, agg);
         s_part[lane] = prefix + xi - x;
     }
