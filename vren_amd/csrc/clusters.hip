@@ -185,13 +185,6 @@ __global__ void chain_init_kernel(uint64_t* status, uint32_t n_status, uint32_t*
     if (blockIdx.x == 0 && threadIdx.x < 4) tickets[threadIdx.x] = 0u;
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = min(v, shfl_xor(v, m));
-    return v;
-}
-
 constexpr uint32_t kClusterWaves = 8;   // waves per workgroup, one 32x32 tile per wave (16 measured slower)
 
 // The depths and normals of lane `lane` of a tile's rows 2 h and 2 h + 1 (pixels i = 4 j + q with j = 2 h + jj,

@@ -74,11 +74,38 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
     return v;
 }
 
+// one DPP move of v; lanes of rows that ROW_MASK leaves out get `fill` (every source lane of these patterns is
+// inside its row)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_fill(uint32_t v, uint32_t fill)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Wave reduction on the DPP network, returned to every lane as a wave-uniform value: quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror and row_mirror leave every lane with its row's result; row_bcast:15 folds row 0 into
+// row 1 and row 2 into row 3, row_bcast:31 folds lane 31 (rows 0-1) into row 3; lane 63 then holds the wave's.
+// `id` is the operation's identity (what the rows left out by a row mask contribute).
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_dpp(uint32_t v, uint32_t id, Op op)
+{
+    v = op(v, dpp_fill<0xB1, 0xF>(v, id));  // quad_perm [1,0,3,2]
+    v = op(v, dpp_fill<0x4E, 0xF>(v, id));  // quad_perm [2,3,0,1]
+    v = op(v, dpp_fill<0x141, 0xF>(v, id)); // row_half_mirror
+    v = op(v, dpp_fill<0x140, 0xF>(v, id)); // row_mirror
+    v = op(v, dpp_fill<0x142, 0xA>(v, id)); // row_bcast:15
+    v = op(v, dpp_fill<0x143, 0xC>(v, id)); // row_bcast:31
+    return __builtin_amdgcn_readlane(v, VREN_WAVE - 1);
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
-    return v;
+    return wave_reduce_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+    return wave_reduce_dpp(v, 0xFFFFFFFFu, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
