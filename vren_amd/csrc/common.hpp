@@ -62,7 +62,7 @@ __device__ __forceinline__ uint32_t dpp0(uint32_t v)
 // inclusive prefix sum over the wave on the DPP network, with no lane-address registers (a shuffle-based scan keeps
 // six per-lane permute addresses live, which the register-capped persistent kernels spilled to scratch):
 // row_shr 1, 2, 4, 8 scan each row of 16 lanes; row_bcast:15 adds lane 15's sum to row 1 and lane 47's to row 3,
-// row_bcast:31 adds lane 31's to rows 2 and 3
+// row_bcast:31 adds lane 31's to rows 2 and 3. Every lane of the wave must be active.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 {
     v += dpp0<0x111, 0xF>(v); // row_shr:1
@@ -85,7 +85,8 @@ __device__ __forceinline__ uint32_t dpp_fill(uint32_t v, uint32_t fill)
 // Wave reduction on the DPP network, returned to every lane as a wave-uniform value: quad_perm [1,0,3,2] and
 // [2,3,0,1], row_half_mirror and row_mirror leave every lane with its row's result; row_bcast:15 folds row 0 into
 // row 1 and row 2 into row 3, row_bcast:31 folds lane 31 (rows 0-1) into row 3; lane 63 then holds the wave's.
-// `id` is the operation's identity (what the rows left out by a row mask contribute).
+// `id` is the operation's identity (what the rows left out by a row mask contribute). Every lane of the wave must be
+// active.
 template <typename Op>
 __device__ __forceinline__ uint32_t wave_reduce_dpp(uint32_t v, uint32_t id, Op op)
 {
