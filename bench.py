@@ -156,7 +156,7 @@ def bench_sort(ctx, log2n, steps, warmup, tune):
     import torch
     n = 1 << log2n
     keys0 = torch.empty(n, dtype=torch.int32, device="cuda")
-    ctx.synth_keys(42, 54, n, keys0)  # configuration 1's generator: pcg32(initstate 42, initseq 54)
+    ctx.synth_keys(42, 54, n, keys0)  # the sort's key generator: pcg32(initstate 42, initseq 54)
     vals0 = torch.arange(n, dtype=torch.int32, device="cuda")
     keys, vals = torch.empty_like(keys0), torch.empty_like(vals0)
     scratch = torch.empty(ctx.radix_sort_scratch_bytes(n, True), dtype=torch.uint8, device="cuda")
@@ -363,9 +363,9 @@ def bench_material_path(ctx, vren_amd, keep, args, plain_steps):
             "shade": _hbm(shade_ms, 42 * px, algorithmic_bytes_per_pixel=42)}
 
 
-def bench_config4(ctx, vren_amd, args):
+def bench_heterogeneous(ctx, vren_amd, args):
     """The batched configuration: 64 frames, resolutions drawn from {640x360, 1280x720, 1920x1080, 2560x1440,
-    3840x2160}, light counts log-uniform in [256, 65536] (the same draw as the configuration-4 parity test),
+    3840x2160}, light counts log-uniform in [256, 65536] (the same draw as the 64-frame heterogeneous parity test),
     batched (one graph replay) and plain (per frame launches + stream sync) frames/s."""
     import numpy as np
     import torch
@@ -420,7 +420,7 @@ def bench_config4(ctx, vren_amd, args):
     b.close()
     fps = 64 * steps / (ms_b * 1e-3)
     plain = 64 * 2 / (wall_p * 1e-3)
-    print("[bench] configuration 4: %.3f ms/step batched, %.1f frames/s plain" % (ms_b / steps, plain),
+    print("[bench] 64 heterogeneous frames: %.3f ms/step batched, %.1f frames/s plain" % (ms_b / steps, plain),
           file=sys.stderr, flush=True)
     return {"frames": 64, "pixels": total_px, "frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / steps, 4),
             "plain_frames_per_s": round(plain, 2), "batched_over_plain": round(fps / plain, 3),
@@ -632,7 +632,7 @@ def main():
     ap.add_argument("--no-rows", action="store_true", help="skip reduce, scan, bucket sort and the shade adjoint")
     ap.add_argument("--no-key-variant", action="store_true",
                     help="skip the second batch with the 32-slice x 6-bin cluster-key encoder")
-    ap.add_argument("--no-config4", action="store_true", help="skip the 64-frame heterogeneous batch")
+    ap.add_argument("--no-heterogeneous", action="store_true", help="skip the 64-frame heterogeneous batch")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB with one plain device copy kernel (scales HBM counters)")
     args = ap.parse_args()
@@ -717,7 +717,7 @@ def main():
                   "pairs_per_pixel": round(pairs / pixels, 3), "ops_per_pair": SHADE_OPS_PER_PAIR,
                   "launch_ms": kernel_ms["shade"]}
 
-    # the same frames with the configuration's cluster-key encoder (32 log-depth slices x 6 normal-cone bins)
+    # the same frames with the 32x32 tile x 32 log-depth slice x 6 normal-cone bin cluster-key encoder
     # instead of the reference's (1024 log-depth slices x 64 normal codes)
     sliced = None
     if not args.no_key_variant:
@@ -726,10 +726,10 @@ def main():
     material = None
     if not args.no_key_variant:
         material = bench_material_path(ctx, vren_amd, keep, args, plain_steps)
-    config4 = None
-    if not args.no_config4:
-        print("[bench] configuration 4 (64 heterogeneous frames)", file=sys.stderr, flush=True)
-        config4 = bench_config4(ctx, vren_amd, args)
+    hetero = None
+    if not args.no_heterogeneous:
+        print("[bench] 64 heterogeneous frames", file=sys.stderr, flush=True)
+        hetero = bench_heterogeneous(ctx, vren_amd, args)
 
     sort = None
     if not args.no_sort:
@@ -760,7 +760,7 @@ def main():
         "batched_over_plain": round(value / plain_fps, 3),
         "config_key_encoder": sliced,
         "config_material_path": material,
-        "config4_heterogeneous": config4,
+        "heterogeneous_64_frames": hetero,
         "chain_shape": chain_tune,
         "kernel_ms": kernel_ms,
         "roofline": roofline,

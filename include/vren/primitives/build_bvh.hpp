@@ -47,8 +47,8 @@ namespace vren
     private:
         vren::context const* m_context;
     };
-    // Karras LBVH over sorted (Morton code, light index) pairs (no reference counterpart; configuration 2's binary
-    // BVH). Node layout: vren_lbvh_node; internal nodes 0 .. n-2, leaf i at n - 1 + i (capi.h, vren_build_lbvh).
+    // Karras LBVH over sorted (Morton code, light index) pairs (no reference counterpart; a binary BVH
+    // of the lights beside the reference's 32-ary one). Node layout: vren_lbvh_node; internal nodes 0 .. n-2, leaf i at n - 1 + i (capi.h, vren_build_lbvh).
     class build_lbvh
     {
     public:

@@ -198,7 +198,7 @@ VREN_EXPORT uint32_t vren_calc_bvh_level_count(uint32_t leaf_count);
 VREN_EXPORT int vren_build_bvh(vren_ctx* ctx, vren_stream_t stream, vren_bvh_node* nodes,
                                uint32_t padded_leaf_count);
 
-/* Karras binary radix-tree BVH (LBVH; no reference counterpart: configuration 2 asks for it beside the reference's
+/* Karras binary radix-tree BVH (LBVH; no reference counterpart: a binary BVH of the lights, beside the reference's
  * 32-ary tree). Leaves are the n >= 1 sorted (Morton code, light index) pairs light_codes[n] (uint2; the 64-bit keys
  * code << 32 | index must be strictly increasing, as vren_construct_point_light_bvh emits them); leaf i's AABB is
  * view_positions[light index] -+ radius (point_lights[.].w), as in the 32-ary leaves.
@@ -459,7 +459,7 @@ VREN_EXPORT int vren_visualize_bvh(vren_ctx* ctx, vren_stream_t stream, const vr
  * Synthetic inputs on the device (include/vren_hip/synth.h; no reference counterpart: the reference draws its
  * inputs from a rasterised glTF scene). initstate = 42 + 1000 * config (+ frame).
  * ------------------------------------------------------------------------------------------------------- */
-/* n uint32 PCG32 draws of stream (initstate, initseq) (configuration 1's keys: initstate 42, initseq 54) */
+/* n uint32 PCG32 draws of stream (initstate, initseq) (the sort workload's keys: initstate 42, initseq 54) */
 VREN_EXPORT int vren_synth_keys(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint64_t initseq, uint64_t n,
                                 uint32_t* out);
 VREN_EXPORT int vren_synth_lights(vren_ctx* ctx, vren_stream_t stream, uint64_t initstate, uint32_t light_count,

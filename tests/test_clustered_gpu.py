@@ -12,8 +12,8 @@ from gpu_util import empty_dev, from_dev, to_dev
 
 pytestmark = pytest.mark.gpu
 
-W3, H3 = 1920, 1080
-L2 = 4096
+W_FRAME, H_FRAME = 1920, 1080
+L_FRAME = 4096
 SEED_LIGHTS = 42 + 1000 * 1
 SEED_FRAME = 42 + 1000 * 2
 SHADE_RTOL, SHADE_ATOL = 1e-5, 1e-6
@@ -25,11 +25,11 @@ def co(oracle):
 
 
 @pytest.fixture(scope="module")
-def c3(co):
-    cam = ol.camera(W3, H3)
-    pos, lig = co.synth_lights(SEED_LIGHTS, L2)
-    depth, normals = co.synth_frame(SEED_FRAME, W3, H3, cam)
-    ar, me = co.synth_materials(SEED_FRAME, W3 * H3)
+def f1080(co):
+    cam = ol.camera(W_FRAME, H_FRAME)
+    pos, lig = co.synth_lights(SEED_LIGHTS, L_FRAME)
+    depth, normals = co.synth_frame(SEED_FRAME, W_FRAME, H_FRAME, cam)
+    ar, me = co.synth_materials(SEED_FRAME, W_FRAME * H_FRAME)
     return dict(cam=cam, pos=pos, lig=lig, depth=depth, normals=normals, ar=ar, me=me)
 
 
@@ -44,49 +44,49 @@ def gpu_light_bvh(vctx, co, pos, lig, cam, cf):
             from_dev(bvh, ol.BVH_DTYPE, (nb,)))
 
 
-def test_synth_generators_match_oracle(vctx, co, c3):
-    cam = c3["cam"]
-    p, l = empty_dev(16 * L2), empty_dev(16 * L2)
-    vctx.synth_lights(SEED_LIGHTS, L2, p, l)
-    assert np.array_equal(from_dev(p, np.uint32, (L2, 4)), c3["pos"].view(np.uint32))
-    assert np.array_equal(from_dev(l, np.uint32, (L2, 4)), c3["lig"].view(np.uint32))
-    d, n = empty_dev(4 * W3 * H3), empty_dev(8 * W3 * H3)
-    vctx.synth_frame(SEED_FRAME, W3, H3, cam, 64, d, n)
-    assert np.array_equal(from_dev(d, np.uint32, (W3 * H3,)), c3["depth"].view(np.uint32))
-    assert np.array_equal(from_dev(n, np.uint16, (W3 * H3, 4)), c3["normals"])
-    a, m = empty_dev(16 * W3 * H3), empty_dev(4 * W3 * H3)
-    vctx.synth_materials(SEED_FRAME, W3 * H3, a, m)
-    assert np.array_equal(from_dev(a, np.uint32, (W3 * H3, 4)), c3["ar"].view(np.uint32))
-    assert np.array_equal(from_dev(m, np.uint32, (W3 * H3,)), c3["me"].view(np.uint32))
+def test_synth_generators_match_oracle(vctx, co, f1080):
+    cam = f1080["cam"]
+    p, l = empty_dev(16 * L_FRAME), empty_dev(16 * L_FRAME)
+    vctx.synth_lights(SEED_LIGHTS, L_FRAME, p, l)
+    assert np.array_equal(from_dev(p, np.uint32, (L_FRAME, 4)), f1080["pos"].view(np.uint32))
+    assert np.array_equal(from_dev(l, np.uint32, (L_FRAME, 4)), f1080["lig"].view(np.uint32))
+    d, n = empty_dev(4 * W_FRAME * H_FRAME), empty_dev(8 * W_FRAME * H_FRAME)
+    vctx.synth_frame(SEED_FRAME, W_FRAME, H_FRAME, cam, 64, d, n)
+    assert np.array_equal(from_dev(d, np.uint32, (W_FRAME * H_FRAME,)), f1080["depth"].view(np.uint32))
+    assert np.array_equal(from_dev(n, np.uint16, (W_FRAME * H_FRAME, 4)), f1080["normals"])
+    a, m = empty_dev(16 * W_FRAME * H_FRAME), empty_dev(4 * W_FRAME * H_FRAME)
+    vctx.synth_materials(SEED_FRAME, W_FRAME * H_FRAME, a, m)
+    assert np.array_equal(from_dev(a, np.uint32, (W_FRAME * H_FRAME, 4)), f1080["ar"].view(np.uint32))
+    assert np.array_equal(from_dev(m, np.uint32, (W_FRAME * H_FRAME,)), f1080["me"].view(np.uint32))
 
 
 def test_camera_matrices_match_oracle(vctx, co):
-    for cam in (ol.camera(W3, H3), ol.camera(640, 360, fov_deg=60.0, pos=(3.0, -2.0, 7.5))):
+    for cam in (ol.camera(W_FRAME, H_FRAME), ol.camera(640, 360, fov_deg=60.0, pos=(3.0, -2.0, 7.5))):
         v1, p1 = vctx.camera_matrices(cam)
         v2, p2 = co.camera_matrices(cam)
         assert np.array_equal(v1, v2) and np.array_equal(p1, p2)
 
 
 @pytest.mark.parametrize("mode", ["reference15", "scene30"])
-def test_light_bvh_c2(vctx, co, c3, mode):
-    cam = c3["cam"]
+def test_light_bvh_4096_lights(vctx, co, f1080, mode):
+    cam = f1080["cam"]
     if mode == "reference15":
         cf = ol.cfg()
-    else:  # config 2: 30-bit codes on the scene AABB ([-50,50]^3 world = view z in [0,100] for this camera)
+    else:  # 30-bit codes on the scene AABB ([-50,50]^3 world = view z in [0,100] for this camera)
         cf = ol.cfg(morton_bits=10, morton_aabb_mode=1, aabb=((-50, -50, 0), (50, 50, 100)))
-    vp, codes, bvh = gpu_light_bvh(vctx, co, c3["pos"], c3["lig"], cam, cf)
-    ovp, ocodes, obvh = co.light_bvh(c3["pos"], c3["lig"], cam, cf)
+    vp, codes, bvh = gpu_light_bvh(vctx, co, f1080["pos"], f1080["lig"], cam, cf)
+    ovp, ocodes, obvh = co.light_bvh(f1080["pos"], f1080["lig"], cam, cf)
     assert np.array_equal(vp.view(np.uint32), ovp.view(np.uint32))
     assert np.array_equal(codes, ocodes)
     assert np.array_equal(bvh.view(np.uint32).reshape(-1, 8)[:, :7], obvh.view(np.uint32).reshape(-1, 8)[:, :7])
 
 
-def test_bucket_sort_light_morton_codes_c2(vctx, co, c3):
-    # configuration 2's BucketSort of the lights into 64 buckets, on the lights' own 30-bit Morton codes (scene AABB)
+def test_bucket_sort_light_morton_codes(vctx, co, f1080):
+    # the BucketSort of 4096 lights into 64 buckets, on the lights' own 30-bit Morton codes (scene AABB)
     # rather than random keys: (code, light index) pairs in light order, bucket = the code's top 6 bits
-    cam = c3["cam"]
+    cam = f1080["cam"]
     cf = ol.cfg(morton_bits=10, morton_aabb_mode=1, aabb=((-50, -50, 0), (50, 50, 100)))
-    _, codes, _ = gpu_light_bvh(vctx, co, c3["pos"], c3["lig"], cam, cf)
+    _, codes, _ = gpu_light_bvh(vctx, co, f1080["pos"], f1080["lig"], cam, cf)
     L = codes.shape[0]
     pairs = np.zeros((L, 2), np.uint32)
     pairs[codes[:, 1]] = codes                       # back to light order
@@ -119,7 +119,7 @@ def test_light_bvh_sizes(vctx, co, L):
 @pytest.mark.parametrize("L,bits", [(1, 5), (2, 5), (3, 5), (33, 5), (1000, 5), (4096, 5), (4096, 10),
                                     (40000, 10)])
 def test_lbvh_matches_oracle(vctx, co, L, bits):
-    """Karras LBVH (configuration 2): topology, parents and refitted AABBs bit-exact against the oracle."""
+    """Karras LBVH over sorted light codes: topology, parents and refitted AABBs bit-exact against the oracle."""
     cam = ol.camera(64, 64)
     pos, lig = co.synth_lights(5000 + L, L)
     cf = ol.cfg(morton_bits=bits)
@@ -141,11 +141,11 @@ def gpu_clusters(vctx, W, H, cam, cf, depth, normals, key_cap=1 << 20):
     return keys, from_dev(keys, np.uint32, (min(U, key_cap),)), U, from_dev(refs, np.uint32, (W * H,)), cnt, refs
 
 
-@pytest.mark.parametrize("case", ["c3_reference", "odd_sliced", "odd_reference"])
-def test_unique_clusters(vctx, co, c3, case):
-    if case == "c3_reference":
-        W, H, cam, cf = W3, H3, c3["cam"], ol.cfg()
-        depth, normals = c3["depth"], c3["normals"]
+@pytest.mark.parametrize("case", ["1080p_reference", "odd_sliced", "odd_reference"])
+def test_unique_clusters(vctx, co, f1080, case):
+    if case == "1080p_reference":
+        W, H, cam, cf = W_FRAME, H_FRAME, f1080["cam"], ol.cfg()
+        depth, normals = f1080["depth"], f1080["normals"]
     else:
         W, H = 1000, 700
         cam = ol.camera(W, H, fov_deg=75.0)
@@ -223,10 +223,10 @@ def test_unique_clusters_every_f16_normal_component(vctx, co, mode):
     assert np.array_equal(refs, orefs)
 
 
-def test_unique_clusters_capacity_flag(vctx, co, c3):
+def test_unique_clusters_capacity_flag(vctx, co, f1080):
     from vren_amd import VrenError
-    _, keys, U, refs, _, _ = gpu_clusters(vctx, W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)
-    okeys, oU, orefs = co.unique_clusters(W3, H3, c3["cam"], ol.cfg(), c3["depth"], c3["normals"], key_cap=100)
+    _, keys, U, refs, _, _ = gpu_clusters(vctx, W_FRAME, H_FRAME, f1080["cam"], ol.cfg(), f1080["depth"], f1080["normals"], key_cap=100)
+    okeys, oU, orefs = co.unique_clusters(W_FRAME, H_FRAME, f1080["cam"], ol.cfg(), f1080["depth"], f1080["normals"], key_cap=100)
     assert U == oU and U > 100 and np.array_equal(keys, okeys) and np.array_equal(refs, orefs)
     with pytest.raises(VrenError) as e:
         vctx.sync_status()
@@ -255,26 +255,26 @@ def run_assign(vctx, co, W, H, cam, cf, depth, normals, pos, lig, key_cap=1 << 2
                 bvh=from_dev(bvh, ol.BVH_DTYPE, (nb,)))
 
 
-C3_ENCODERS = {"reference": dict(), "sliced32": dict(key_mode=1, depth_slices=32)}
+ENCODERS_1080P = {"reference": dict(), "sliced32": dict(key_mode=1, depth_slices=32)}
 
 
 @pytest.mark.parametrize("encoder", ["reference", "sliced32"])
 @pytest.mark.parametrize("aabb_mode", [0, 1])
-def test_assign_lights_c3(vctx, co, c3, aabb_mode, encoder):
-    """1920x1080 with 4096 lights, with the reference key encoder and with the configuration's own (32x32 tiles x 32
+def test_assign_lights_1080p(vctx, co, f1080, aabb_mode, encoder):
+    """1920x1080 with 4096 lights, with the reference key encoder and with the sliced one (32x32 tiles x 32
     log-depth slices over [near, far] x 6 cube faces): counts, offsets and lists bit for bit, both AABB modes."""
-    cam = c3["cam"]
-    cf = ol.cfg(cluster_aabb_mode=aabb_mode, **C3_ENCODERS[encoder])
-    g = run_assign(vctx, co, W3, H3, cam, cf, c3["depth"], c3["normals"], c3["pos"], c3["lig"])
-    ovp, ocodes, obvh = co.light_bvh(c3["pos"], c3["lig"], cam, cf)
-    okeys, oU, orefs = co.unique_clusters(W3, H3, cam, cf, c3["depth"], c3["normals"])
-    oc, oo, oi, ot = co.assign(W3, H3, cam, cf, okeys, oU, obvh, ocodes, ovp)
+    cam = f1080["cam"]
+    cf = ol.cfg(cluster_aabb_mode=aabb_mode, **ENCODERS_1080P[encoder])
+    g = run_assign(vctx, co, W_FRAME, H_FRAME, cam, cf, f1080["depth"], f1080["normals"], f1080["pos"], f1080["lig"])
+    ovp, ocodes, obvh = co.light_bvh(f1080["pos"], f1080["lig"], cam, cf)
+    okeys, oU, orefs = co.unique_clusters(W_FRAME, H_FRAME, cam, cf, f1080["depth"], f1080["normals"])
+    oc, oo, oi, ot = co.assign(W_FRAME, H_FRAME, cam, cf, okeys, oU, obvh, ocodes, ovp)
     assert g["U"] == oU and np.array_equal(g["keys"], okeys) and np.array_equal(g["refs"], orefs)
     assert g["total"] == ot
     assert np.array_equal(g["counts"], oc)
     assert np.array_equal(g["offsets"], oo)
     assert np.array_equal(g["idx"], oi)
-    print("\n[c3 %s aabb=%d] %d clusters, %d assignments" % (encoder, aabb_mode, oU, ot))
+    print("\n[f1080 %s aabb=%d] %d clusters, %d assignments" % (encoder, aabb_mode, oU, ot))
 
 
 def test_assign_lights_many_lights(vctx, co):
@@ -299,23 +299,23 @@ def shade_error_report(got, ref, mask):
 
 
 @pytest.mark.parametrize("atten,encoder", [(0, "reference"), (1, "reference"), (0, "sliced32"), (1, "sliced32")])
-def test_shade_c3_vs_fp64(vctx, co, c3, atten, encoder):
-    cam = c3["cam"]
-    cf = ol.cfg(attenuation=atten, **C3_ENCODERS[encoder])
-    g = run_assign(vctx, co, W3, H3, cam, ol.cfg(**C3_ENCODERS[encoder]), c3["depth"], c3["normals"], c3["pos"],
-                   c3["lig"])
-    out = empty_dev(16 * W3 * H3, fill=0xFF)  # NaN sentinel: background pixels must stay untouched
+def test_shade_1080p_vs_fp64(vctx, co, f1080, atten, encoder):
+    cam = f1080["cam"]
+    cf = ol.cfg(attenuation=atten, **ENCODERS_1080P[encoder])
+    g = run_assign(vctx, co, W_FRAME, H_FRAME, cam, ol.cfg(**ENCODERS_1080P[encoder]), f1080["depth"], f1080["normals"], f1080["pos"],
+                   f1080["lig"])
+    out = empty_dev(16 * W_FRAME * H_FRAME, fill=0xFF)  # NaN sentinel: background pixels must stay untouched
     key_cap = 1 << 20
-    vctx.shade(W3, H3, cam, cf, to_dev(c3["depth"]), to_dev(c3["normals"]), to_dev(c3["ar"]), to_dev(c3["me"]),
+    vctx.shade(W_FRAME, H_FRAME, cam, cf, to_dev(f1080["depth"]), to_dev(f1080["normals"]), to_dev(f1080["ar"]), to_dev(f1080["me"]),
                g["dev"]["refs"], key_cap, g["dev"]["counts"], g["dev"]["offsets"], g["dev"]["idx"], 1 << 22,
-               g["dev"]["pos"], g["dev"]["lig"], L2, out)
-    got = from_dev(out, np.float32, (W3 * H3, 4))
+               g["dev"]["pos"], g["dev"]["lig"], L_FRAME, out)
+    got = from_dev(out, np.float32, (W_FRAME * H_FRAME, 4))
     cap_counts = np.zeros(key_cap, np.uint32)
     cap_offs = np.zeros(key_cap, np.uint32)
     cap_counts[:g["counts"].size] = g["counts"]
     cap_offs[:g["offsets"].size] = g["offsets"]
-    ref, mask = co.shade(W3, H3, cam, cf, c3["depth"], c3["normals"], c3["ar"], c3["me"], g["refs"], key_cap,
-                         cap_counts, cap_offs, g["idx"], c3["pos"], c3["lig"])
+    ref, mask = co.shade(W_FRAME, H_FRAME, cam, cf, f1080["depth"], f1080["normals"], f1080["ar"], f1080["me"], g["refs"], key_cap,
+                         cap_counts, cap_offs, g["idx"], f1080["pos"], f1080["lig"])
     assert np.all(np.isnan(got[~mask]))               # background untouched (shade.comp:86)
     assert np.all(got[mask, 3] == 1.0)
     err = shade_error_report(got, ref, mask)
@@ -495,7 +495,7 @@ def _batch_all_modes_identical(b, keep):
                 assert torch.equal(a, k["o"]), (mode, i)
 
 
-def test_batch_config4_64_heterogeneous_frames(vctx, co):
+def test_batch_64_heterogeneous_frames(vctx, co):
     """The batched configuration at full size: 64 frames, resolutions drawn from {640x360, 1280x720, 1920x1080,
     2560x1440, 3840x2160}, light counts log-uniform in [256, 65536]. Per-frame, batched and graph launches give the
     same bytes for every frame (compared on the device). Every frame is checked against the oracle: keys,
@@ -560,7 +560,7 @@ def test_batch_long_lists_and_many_clusters(vctx, co):
 
 @pytest.mark.parametrize("aabb_mode", [0, 1])
 def test_batch_sliced_encoder(vctx, co, aabb_mode):
-    """The configuration's own key encoder (32 log-depth slices x 6 cube faces) through the batched chain: 1080p
+    """The sliced key encoder (32 log-depth slices x 6 cube faces) through the batched chain: 1080p
     with 4096 lights and three other frames; every launch mode gives the same bytes and every frame matches the
     oracle (keys, references, lists; shade samples within 1e-5 of FP64)."""
     import vren_amd

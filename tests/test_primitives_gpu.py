@@ -10,15 +10,15 @@ N_C1 = 1 << 20
 
 
 @pytest.fixture(scope="module")
-def c1_keys(oracle):
-    # config 1: 2^20 uint32 keys from pcg32(initstate 42, initseq 54)
+def pcg_keys(oracle):
+    # 2^20 uint32 keys from pcg32(initstate 42, initseq 54)
     return oracle.pcg32(42, 54, N_C1)
 
 
 @pytest.mark.parametrize("n", [1, 10, 1000, 10000, 100000, N_C1])
 @pytest.mark.parametrize("op", [0, 1, 2])
-def test_reduce_u32_tree(vctx, oracle, c1_keys, n, op):
-    x = c1_keys[:n] if op != 0 or n != 10000 else np.ones(n, np.uint32)
+def test_reduce_u32_tree(vctx, oracle, pcg_keys, n, op):
+    x = pcg_keys[:n] if op != 0 or n != 10000 else np.ones(n, np.uint32)
     p = 1 << max(0, (n - 1).bit_length())
     out = empty_dev(4 * p, fill=0xAB)
     vctx.reduce_u32(to_dev(x), n, out, op)
@@ -26,30 +26,30 @@ def test_reduce_u32_tree(vctx, oracle, c1_keys, n, op):
     assert np.array_equal(got, oracle.reduce_tree_u32(x, op))
 
 
-def test_reduce_u32_multi_block_and_in_place(vctx, oracle, c1_keys):
-    x = c1_keys[:3 * 1000]
+def test_reduce_u32_multi_block_and_in_place(vctx, oracle, pcg_keys):
+    x = pcg_keys[:3 * 1000]
     out = empty_dev(4 * 3 * 1024)
     vctx.reduce_u32(to_dev(x), 1000, out, 0, blocks=3)
     assert np.array_equal(from_dev(out, np.uint32, (3 * 1024,)), oracle.reduce_tree_u32(x, 0, blocks=3))
-    y = c1_keys[:4096]
+    y = pcg_keys[:4096]
     d = to_dev(y)
     vctx.reduce_u32(d, 4096, d, 0)
     assert np.array_equal(from_dev(d, np.uint32, (4096,)), oracle.reduce_tree_u32(y, 0))
     # segments longer than one 16384-element tile, at input offsets that are not 16-byte aligned (5001 * 4 bytes)
     for n, op in ((20000, 0), (5001, 2), (40001, 1)):
-        x = c1_keys[:3 * n]
+        x = pcg_keys[:3 * n]
         p = 1 << (n - 1).bit_length()
         out = empty_dev(4 * 3 * p, fill=0xCD)
         vctx.reduce_u32(to_dev(x), n, out, op, blocks=3)
         assert np.array_equal(from_dev(out, np.uint32, (3 * p,)), oracle.reduce_tree_u32(x, op, blocks=3)), n
 
 
-def test_reduce_u32_two_passes_in_sequence(vctx, oracle, c1_keys):
+def test_reduce_u32_two_passes_in_sequence(vctx, oracle, pcg_keys):
     # one segment of 16 384 < p <= 2^26 elements (two up-sweep passes, the second over the first's compact tile
     # totals): calls in a row with the scalar total in between (both use the context's scratch), a partial last
     # tile, in place, and 2^24 elements
     for n, op in ((1 << 20, 0), (100000, 2), ((1 << 16) + 3, 1), (1 << 20, 1)):
-        x = c1_keys[:n]
+        x = pcg_keys[:n]
         p = 1 << (n - 1).bit_length()
         out = empty_dev(4 * p, fill=0xEF)
         vctx.reduce_u32(to_dev(x), n, out, op)
@@ -57,9 +57,9 @@ def test_reduce_u32_two_passes_in_sequence(vctx, oracle, c1_keys):
         tot = empty_dev(4)
         vctx.reduce_total_u32(to_dev(x), n, tot, op)
         assert from_dev(tot, np.uint32)[0] == oracle.reduce_total_u32(x, op)
-    d = to_dev(c1_keys)
+    d = to_dev(pcg_keys)
     vctx.reduce_u32(d, N_C1, d, 0)
-    assert np.array_equal(from_dev(d, np.uint32, (N_C1,)), oracle.reduce_tree_u32(c1_keys, 0))
+    assert np.array_equal(from_dev(d, np.uint32, (N_C1,)), oracle.reduce_tree_u32(pcg_keys, 0))
     big = oracle.pcg32(7, 54, 1 << 24)
     out = empty_dev(4 << 24)
     vctx.reduce_u32(to_dev(big), 1 << 24, out, 0)
@@ -80,28 +80,28 @@ def test_reduce_vec4_tree_bit_exact(vctx, oracle, op):
 
 
 @pytest.mark.parametrize("op", [0, 1, 2])
-def test_reduce_total(vctx, oracle, c1_keys, op):
+def test_reduce_total(vctx, oracle, pcg_keys, op):
     for n in (1, 5, 4099, N_C1):
         out = empty_dev(4)
-        vctx.reduce_total_u32(to_dev(c1_keys[:n]), n, out, op)
-        assert from_dev(out, np.uint32)[0] == oracle.reduce_total_u32(c1_keys[:n], op)
+        vctx.reduce_total_u32(to_dev(pcg_keys[:n]), n, out, op)
+        assert from_dev(out, np.uint32)[0] == oracle.reduce_total_u32(pcg_keys[:n], op)
 
 
 @pytest.mark.parametrize("n", [1, 7, 4096, 4097, 100003, N_C1])
-def test_exclusive_scan(vctx, oracle, c1_keys, n):
-    x = c1_keys[:n]
+def test_exclusive_scan(vctx, oracle, pcg_keys, n):
+    x = pcg_keys[:n]
     out = empty_dev(4 * n)
     vctx.exclusive_scan_u32(to_dev(x), out, n)
     assert np.array_equal(from_dev(out, np.uint32, (n,)), oracle.exclusive_scan_u32(x))
 
 
-def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, c1_keys):
-    x = c1_keys[:3 * 5000]
+def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, pcg_keys):
+    x = pcg_keys[:3 * 5000]
     d = to_dev(x)
     vctx.exclusive_scan_u32(d, d, 5000, blocks=3)
     assert np.array_equal(from_dev(d, np.uint32, (15000,)), oracle.exclusive_scan_u32(x, blocks=3))
     # several 16384-key tiles per segment, segments at offsets that are not 16-byte aligned (40001 * 4 bytes)
-    x = c1_keys[:3 * 40001]
+    x = pcg_keys[:3 * 40001]
     o = empty_dev(4 * 3 * 40001)
     vctx.exclusive_scan_u32(to_dev(x), o, 40001, blocks=3)
     assert np.array_equal(from_dev(o, np.uint32, (3 * 40001,)), oracle.exclusive_scan_u32(x, blocks=3))
@@ -114,11 +114,11 @@ def test_exclusive_scan_segments_in_place_and_ones(vctx, oracle, c1_keys):
 
 @pytest.mark.parametrize("blocks", [1, 16])
 @pytest.mark.parametrize("length", [16, 1024, 1 << 12, 1 << 17, 1 << 20])
-def test_reduce_then_downsweep_is_exclusive_scan(vctx, oracle, c1_keys, length, blocks):
+def test_reduce_then_downsweep_is_exclusive_scan(vctx, oracle, pcg_keys, length, blocks):
     """The reference's radix-sort use (radix_sort.cpp:230-289): reduce (add, in place) over `blocks` segments, then
     downsweep(clear_last = true): every segment holds std::exclusive_scan of its keys."""
     n = length * blocks
-    x = np.resize(c1_keys, n)
+    x = np.resize(pcg_keys, n)
     d = to_dev(x)
     vctx.reduce_u32(d, length, d, 0, blocks)
     vctx.blelloch_downsweep_u32(d, length, blocks, True)
@@ -126,12 +126,12 @@ def test_reduce_then_downsweep_is_exclusive_scan(vctx, oracle, c1_keys, length, 
 
 
 @pytest.mark.parametrize("length", [1, 2, 4, 8, 16, 32, 512, 1024, 2048, 4096, 8192, 1 << 16, 1 << 22])
-def test_downsweep_arbitrary_input_vs_oracle(vctx, oracle, c1_keys, length):
+def test_downsweep_arbitrary_input_vs_oracle(vctx, oracle, pcg_keys, length):
     """On arbitrary input (not an up-sweep tree), both clear_last settings, 1 and 3 segments (the second and third
     not 16-byte aligned when length < 4): bit for bit against the oracle's level-by-level restatement."""
     for blocks in (1, 3):
         n = length * blocks
-        x = np.resize(c1_keys[::-1], n)
+        x = np.resize(pcg_keys[::-1], n)
         for clear in (True, False):
             d = to_dev(x)
             vctx.blelloch_downsweep_u32(d, length, blocks, clear)
@@ -150,14 +150,14 @@ def _sort_gpu(vctx, keys, vals, begin=0, end=32):
     return k, v
 
 
-def test_radix_sort_c1_kv(vctx, oracle, c1_keys):
+def test_radix_sort_pcg_keys_kv(vctx, oracle, pcg_keys):
     vals = np.arange(N_C1, dtype=np.uint32)
-    k, v = _sort_gpu(vctx, c1_keys, vals)
-    ok, ov = oracle.stable_sort_kv(c1_keys, vals)
+    k, v = _sort_gpu(vctx, pcg_keys, vals)
+    ok, ov = oracle.stable_sort_kv(pcg_keys, vals)
     assert np.array_equal(k, ok) and np.array_equal(v, ov)
 
 
-def test_radix_sort_reference_case_and_edges(vctx, oracle, c1_keys):
+def test_radix_sort_reference_case_and_edges(vctx, oracle, pcg_keys):
     # radix_sort.main: N = 1024, reversed iota, vs std::sort
     k, _ = _sort_gpu(vctx, np.arange(1024, dtype=np.uint32)[::-1].copy(), None)
     assert np.array_equal(k, np.arange(1024, dtype=np.uint32))
@@ -169,15 +169,15 @@ def test_radix_sort_reference_case_and_edges(vctx, oracle, c1_keys):
         k, v = _sort_gpu(vctx, keys, vals)
         ok, ov = oracle.stable_sort_kv(keys, vals)
         assert np.array_equal(k, ok) and np.array_equal(v, ov), n
-    keys = c1_keys[:70000]
+    keys = pcg_keys[:70000]
     for b, e in ((0, 15), (4, 12), (16, 32), (0, 30)):
         k, v = _sort_gpu(vctx, keys, np.arange(keys.size, dtype=np.uint32), b, e)
         ok, ov = oracle.stable_sort_kv(keys, np.arange(keys.size, dtype=np.uint32), b, e)
         assert np.array_equal(v, ov), (b, e)
 
 
-def test_radix_sort_every_config_identical(vctx, oracle, c1_keys):
-    keys = c1_keys[:100003]
+def test_radix_sort_every_config_identical(vctx, oracle, pcg_keys):
+    keys = pcg_keys[:100003]
     vals = np.arange(keys.size, dtype=np.uint32)
     ok, ov = oracle.stable_sort_kv(keys, vals)
     configs = vctx.radix_configs()

@@ -2,7 +2,7 @@
 checked against FP64 central differences of the oracle's loss (h = 1e-4, 256 sampled parameters) and against the
 oracle's FP64 analytic adjoint on every parameter.
 
-Configuration 5: 1920x1080 G-buffer (albedo U[0,1]^3, roughness U[0.05,1], metallic Bernoulli(0.3), depth/normals
+The workload: a 1920x1080 G-buffer (albedo U[0,1]^3, roughness U[0.05,1], metallic Bernoulli(0.3), depth/normals
 of the procedural scene), 4096 lights; smooth windowed attenuation; L2 loss against a target rendered from lights
 perturbed by N(0, 0.5) in position and 10% in intensity, with the culled light lists held fixed.
 """
@@ -14,8 +14,8 @@ from gpu_util import empty_dev, from_dev, to_dev
 
 pytestmark = pytest.mark.gpu
 
-W5, H5, L5 = 1920, 1080, 4096
-SEED5 = 42 + 1000 * 4
+W_GRAD, H_GRAD, L_GRAD = 1920, 1080, 4096
+SEED_GRAD = 42 + 1000 * 4
 H_FD = 1e-4
 N_PARAMS = 256
 # tolerances (stated per check below)
@@ -24,66 +24,66 @@ ADJ_REL_L2 = 1e-3       # ||g_gpu - g_fp64|| / ||g_fp64|| per parameter group
 
 
 @pytest.fixture(scope="module")
-def c5(oracle, vctx):
+def grad_frame(oracle, vctx):
     import torch
     import vren_amd
     from vren_amd.autograd import ShadeFrame
 
     co = ol.ClusteredOracle(oracle)
-    cam = ol.camera(W5, H5)
-    pos, lig = co.synth_lights(SEED5, L5)
-    depth, normals = co.synth_frame(SEED5, W5, H5, cam)
-    ar, me = co.synth_materials(SEED5, W5 * H5)
+    cam = ol.camera(W_GRAD, H_GRAD)
+    pos, lig = co.synth_lights(SEED_GRAD, L_GRAD)
+    depth, normals = co.synth_frame(SEED_GRAD, W_GRAD, H_GRAD, cam)
+    ar, me = co.synth_materials(SEED_GRAD, W_GRAD * H_GRAD)
     cf_lists = ol.cfg()
     cf = ol.cfg(attenuation=1)
     # light lists from the GPU chain (held fixed from here on)
     key_cap, index_cap = 1 << 20, 1 << 22
     dpos, dlig = to_dev(pos), to_dev(lig)
-    vp, codes = empty_dev(16 * L5), empty_dev(8 * L5)
-    bvh = empty_dev(32 * oracle.calc("buffer_length", L5))
-    vctx.construct_point_light_bvh(dpos, dlig, L5, cam, cf_lists, vp, codes, bvh)
-    keys, cnt, refs = empty_dev(4 * key_cap), empty_dev(4), empty_dev(4 * W5 * H5)
+    vp, codes = empty_dev(16 * L_GRAD), empty_dev(8 * L_GRAD)
+    bvh = empty_dev(32 * oracle.calc("buffer_length", L_GRAD))
+    vctx.construct_point_light_bvh(dpos, dlig, L_GRAD, cam, cf_lists, vp, codes, bvh)
+    keys, cnt, refs = empty_dev(4 * key_cap), empty_dev(4), empty_dev(4 * W_GRAD * H_GRAD)
     ddepth, dnorm = to_dev(depth), to_dev(normals)
-    vctx.find_unique_cluster_list(W5, H5, cam, cf_lists, ddepth, dnorm, keys, key_cap, cnt, refs)
+    vctx.find_unique_cluster_list(W_GRAD, H_GRAD, cam, cf_lists, ddepth, dnorm, keys, key_cap, cnt, refs)
     counts, offsets, idx, tot = empty_dev(4 * key_cap), empty_dev(4 * key_cap), empty_dev(4 * index_cap), empty_dev(4)
-    vctx.assign_lights(W5, H5, cam, cf_lists, keys, cnt, key_cap, bvh, L5, codes, vp, counts, offsets, idx, index_cap,
+    vctx.assign_lights(W_GRAD, H_GRAD, cam, cf_lists, keys, cnt, key_cap, bvh, L_GRAD, codes, vp, counts, offsets, idx, index_cap,
                        tot)
     total = int(from_dev(tot, np.uint32)[0])
-    frame = ShadeFrame(vctx, W5, H5, cam, cf, ddepth, dnorm, to_dev(me), refs, key_cap, counts, offsets, idx,
+    frame = ShadeFrame(vctx, W_GRAD, H_GRAD, cam, cf, ddepth, dnorm, to_dev(me), refs, key_cap, counts, offsets, idx,
                        index_cap)
-    host = dict(U=int(from_dev(cnt, np.uint32)[0]), refs=from_dev(refs, np.uint32, (W5 * H5,)),
+    host = dict(U=int(from_dev(cnt, np.uint32)[0]), refs=from_dev(refs, np.uint32, (W_GRAD * H_GRAD,)),
                 counts=from_dev(counts, np.uint32, (key_cap,)),
                 offsets=from_dev(offsets, np.uint32, (key_cap,)), idx=from_dev(idx, np.uint32, (total,)))
     # target: lights perturbed by N(0, 0.5) in position and x(1 + 0.1 N(0, 1)) in intensity
     rng = np.random.default_rng(2024)
     pos_t = pos.copy()
-    pos_t[:, :3] += rng.normal(0.0, 0.5, (L5, 3)).astype(np.float32)
+    pos_t[:, :3] += rng.normal(0.0, 0.5, (L_GRAD, 3)).astype(np.float32)
     lig_t = lig.copy()
-    lig_t[:, :3] *= (1.0 + 0.1 * rng.normal(0.0, 1.0, (L5, 1))).astype(np.float32)
-    target = torch.zeros((W5 * H5, 4), dtype=torch.float32, device="cuda")
-    vctx.shade(W5, H5, cam, cf, ddepth, dnorm, to_dev(ar), frame.metallic, refs, key_cap, counts,
-               offsets, idx, index_cap, to_dev(pos_t), to_dev(lig_t), L5, target)
+    lig_t[:, :3] *= (1.0 + 0.1 * rng.normal(0.0, 1.0, (L_GRAD, 1))).astype(np.float32)
+    target = torch.zeros((W_GRAD * H_GRAD, 4), dtype=torch.float32, device="cuda")
+    vctx.shade(W_GRAD, H_GRAD, cam, cf, ddepth, dnorm, to_dev(ar), frame.metallic, refs, key_cap, counts,
+               offsets, idx, index_cap, to_dev(pos_t), to_dev(lig_t), L_GRAD, target)
     torch.cuda.synchronize()
     return dict(co=co, cam=cam, cf=cf, pos=pos, lig=lig, depth=depth, normals=normals, ar=ar, me=me, frame=frame,
                 host=host, target=target, key_cap=key_cap, vren_amd=vren_amd)
 
 
-def gpu_grads(c5):
+def gpu_grads(grad_frame):
     import torch
     from vren_amd.autograd import shade
-    L = L5
-    lp = torch.from_numpy(c5["pos"]).cuda().requires_grad_(True)
-    colour = torch.from_numpy(c5["lig"][:, :3].copy()).cuda()
+    L = L_GRAD
+    lp = torch.from_numpy(grad_frame["pos"]).cuda().requires_grad_(True)
+    colour = torch.from_numpy(grad_frame["lig"][:, :3].copy()).cuda()
     intensity = colour.max(dim=1, keepdim=True).values.clamp_min(1e-6)
     colour = (colour / intensity).detach().requires_grad_(True)
     intensity = intensity.detach().requires_grad_(True)
-    radius = torch.from_numpy(c5["lig"][:, 3:].copy()).cuda().requires_grad_(True)
+    radius = torch.from_numpy(grad_frame["lig"][:, 3:].copy()).cuda().requires_grad_(True)
     lights = torch.cat([colour * intensity, radius], dim=1)
     lights.retain_grad()
-    ar = torch.from_numpy(c5["ar"]).cuda().requires_grad_(True)
-    out = shade(lp, lights, ar, c5["frame"])
+    ar = torch.from_numpy(grad_frame["ar"]).cuda().requires_grad_(True)
+    out = shade(lp, lights, ar, grad_frame["frame"])
     shaded = (out[:, 3] == 1.0)
-    diff = (out[:, :3] - c5["target"][:, :3]) * shaded[:, None]
+    diff = (out[:, :3] - grad_frame["target"][:, :3]) * shaded[:, None]
     loss = (diff * diff).sum()
     loss.backward()
     torch.cuda.synchronize()
@@ -93,9 +93,9 @@ def gpu_grads(c5):
                 shaded=shaded.cpu().numpy(), lights_value=lights.detach().cpu().numpy())
 
 
-def test_colour_intensity_chain_rule(vctx, c5):
+def test_colour_intensity_chain_rule(vctx, grad_frame):
     # d/dcolour and d/dintensity follow from d/d(rgb = colour * intensity) by the chain rule torch applies
-    g = gpu_grads(c5)
+    g = gpu_grads(grad_frame)
     lv = g["lights_value"]
     intensity = np.maximum(lv[:, :3].max(axis=1, keepdims=True), 1e-6)
     colour = lv[:, :3] / intensity
@@ -103,18 +103,18 @@ def test_colour_intensity_chain_rule(vctx, c5):
     np.testing.assert_allclose(g["intensity"][:, 0], (g["lights"][:, :3] * colour).sum(axis=1), rtol=1e-4, atol=1e-5)
 
 
-def test_adjoint_vs_fp64_adjoint(vctx, c5):
-    co, host = c5["co"], c5["host"]
-    g = gpu_grads(c5)
+def test_adjoint_vs_fp64_adjoint(vctx, grad_frame):
+    co, host = grad_frame["co"], grad_frame["host"]
+    g = gpu_grads(grad_frame)
     # FP64 forward and adjoint of the same loss (dLoss/dout from the FP64 forward)
-    ref, mask = co.shade(W5, H5, c5["cam"], c5["cf"], c5["depth"], c5["normals"], c5["ar"], c5["me"], host["refs"],
-                         c5["key_cap"], host["counts"], host["offsets"], host["idx"], c5["pos"], c5["lig"])
-    target = c5["target"].cpu().numpy()
-    gout = np.zeros((W5 * H5, 4), np.float32)
+    ref, mask = co.shade(W_GRAD, H_GRAD, grad_frame["cam"], grad_frame["cf"], grad_frame["depth"], grad_frame["normals"], grad_frame["ar"], grad_frame["me"], host["refs"],
+                         grad_frame["key_cap"], host["counts"], host["offsets"], host["idx"], grad_frame["pos"], grad_frame["lig"])
+    target = grad_frame["target"].cpu().numpy()
+    gout = np.zeros((W_GRAD * H_GRAD, 4), np.float32)
     gout[mask, :3] = (2.0 * (ref[mask, :3] - target[mask, :3])).astype(np.float32)
-    gp, gl, gar = co.shade_backward(W5, H5, c5["cam"], c5["cf"], c5["depth"], c5["normals"], c5["ar"], c5["me"],
-                                    host["refs"], c5["key_cap"], host["counts"], host["offsets"], host["idx"],
-                                    c5["pos"].astype(np.float64), c5["lig"].astype(np.float64), gout)
+    gp, gl, gar = co.shade_backward(W_GRAD, H_GRAD, grad_frame["cam"], grad_frame["cf"], grad_frame["depth"], grad_frame["normals"], grad_frame["ar"], grad_frame["me"],
+                                    host["refs"], grad_frame["key_cap"], host["counts"], host["offsets"], host["idx"],
+                                    grad_frame["pos"].astype(np.float64), grad_frame["lig"].astype(np.float64), gout)
     report = {}
     for name, a, b in (("position", g["pos"][:, :3], gp[:, :3]), ("rgb", g["lights"][:, :3], gl[:, :3]),
                        ("radius", g["lights"][:, 3], gl[:, 3]), ("albedo", g["ar"][:, :3], gar[:, :3]),
@@ -127,50 +127,50 @@ def test_adjoint_vs_fp64_adjoint(vctx, c5):
         assert v < ADJ_REL_L2, (k, v)
 
 
-def fp64_residual_gradient(c5):
+def fp64_residual_gradient(grad_frame):
     """dLoss/dout of the FP64 forward (the function the central differences differentiate)."""
-    co, host = c5["co"], c5["host"]
-    ref, mask = co.shade(W5, H5, c5["cam"], c5["cf"], c5["depth"], c5["normals"], c5["ar"], c5["me"], host["refs"],
-                         c5["key_cap"], host["counts"], host["offsets"], host["idx"], c5["pos"], c5["lig"])
-    target = c5["target"].cpu().numpy()
-    gout = np.zeros((W5 * H5, 4), np.float32)
+    co, host = grad_frame["co"], grad_frame["host"]
+    ref, mask = co.shade(W_GRAD, H_GRAD, grad_frame["cam"], grad_frame["cf"], grad_frame["depth"], grad_frame["normals"], grad_frame["ar"], grad_frame["me"], host["refs"],
+                         grad_frame["key_cap"], host["counts"], host["offsets"], host["idx"], grad_frame["pos"], grad_frame["lig"])
+    target = grad_frame["target"].cpu().numpy()
+    gout = np.zeros((W_GRAD * H_GRAD, 4), np.float32)
     gout[mask, :3] = (2.0 * (ref[mask, :3] - target[mask, :3])).astype(np.float32)
     return gout, mask
 
 
-def gpu_backward(c5, gout):
+def gpu_backward(grad_frame, gout):
     """vren_shade_backward for a given dLoss/dout (isolates the adjoint from the forward's rounding)."""
     import torch
-    f = c5["frame"]
-    lp = torch.from_numpy(c5["pos"]).cuda()
-    pl = torch.from_numpy(c5["lig"]).cuda()
-    ar = torch.from_numpy(c5["ar"]).cuda()
+    f = grad_frame["frame"]
+    lp = torch.from_numpy(grad_frame["pos"]).cuda()
+    pl = torch.from_numpy(grad_frame["lig"]).cuda()
+    ar = torch.from_numpy(grad_frame["ar"]).cuda()
     g_pos, g_pl, g_ar = torch.zeros_like(lp), torch.zeros_like(pl), torch.zeros_like(ar)
     f.ctx.shade_backward(f.W, f.H, f.cam, f.cfg, f.depth, f.normals, ar, f.metallic, f.refs, f.key_capacity, f.counts,
-                         f.offsets, f.indices, f.index_capacity, lp, pl, L5, torch.from_numpy(gout).cuda(), g_pos, g_pl,
+                         f.offsets, f.indices, f.index_capacity, lp, pl, L_GRAD, torch.from_numpy(gout).cuda(), g_pos, g_pl,
                          g_ar)
     torch.cuda.synchronize()
     return g_pos.cpu().numpy(), g_pl.cpu().numpy(), g_ar.cpu().numpy()
 
 
-def test_adjoint_vs_central_differences(vctx, c5):
-    co, host = c5["co"], c5["host"]
-    gout, mask = fp64_residual_gradient(c5)
-    gpos, glig, gar = gpu_backward(c5, gout)
-    target = c5["target"].cpu().numpy()
+def test_adjoint_vs_central_differences(vctx, grad_frame):
+    co, host = grad_frame["co"], grad_frame["host"]
+    gout, mask = fp64_residual_gradient(grad_frame)
+    gpos, glig, gar = gpu_backward(grad_frame, gout)
+    target = grad_frame["target"].cpu().numpy()
     refs, counts, offsets, idx = host["refs"], host["counts"], host["offsets"], host["idx"]
     U = host["U"]
     cluster_of_entry = np.repeat(np.arange(U, dtype=np.int64), counts[:U])
     assert cluster_of_entry.size == idx.size
     shaded_px = np.nonzero(mask)[0]
-    pos_d = c5["pos"].astype(np.float64)
-    lig_d = c5["lig"].astype(np.float64)
+    pos_d = grad_frame["pos"].astype(np.float64)
+    lig_d = grad_frame["lig"].astype(np.float64)
     rng = np.random.default_rng(77)
     used_lights = np.unique(idx)
     lights_pick = rng.choice(used_lights, size=N_PARAMS // 2, replace=False)
     px_with_lists = shaded_px[counts[refs[shaded_px]] > 0]
     pixels_pick = rng.choice(px_with_lists, size=N_PARAMS - N_PARAMS // 2, replace=False)
-    args = (W5, H5, c5["cam"], c5["cf"], c5["depth"], c5["normals"], c5["ar"], c5["me"], refs, c5["key_cap"],
+    args = (W_GRAD, H_GRAD, grad_frame["cam"], grad_frame["cf"], grad_frame["depth"], grad_frame["normals"], grad_frame["ar"], grad_frame["me"], refs, grad_frame["key_cap"],
             counts, offsets, idx)
     errs = []
 
@@ -202,7 +202,7 @@ def test_adjoint_vs_central_differences(vctx, c5):
         comp = n % 4  # albedo r, g, b, roughness
 
         def ev(delta, p=p, comp=comp):
-            par = c5["ar"][p].astype(np.float64).copy()
+            par = grad_frame["ar"][p].astype(np.float64).copy()
             par[comp] += delta
             return co.shade_loss(*args, pos_d, lig_d, target, np.array([p], np.uint32), par[None, :])
         fd, trunc = central(ev)

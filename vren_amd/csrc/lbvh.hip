@@ -1,5 +1,5 @@
-// lbvh.hip -- Karras binary radix-tree BVH over sorted light codes (no reference counterpart; configuration 2's
-// "Karras LBVH with float32 AABB refit"). Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
+// lbvh.hip -- Karras binary radix-tree BVH over sorted light codes, with a float32 AABB refit (no reference
+// counterpart: a binary BVH of the 4096-light workload, beside the reference's 32-ary one). Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
 // and k-d Trees", HPG 2012: every internal node finds its key range and split independently (figure 4), so the
 // topology is built with one thread per internal node; the AABBs are refitted bottom-up, each leaf climbing
 // until it is the first of its parent's two children to arrive (an agent-scope acquire-release counter per
