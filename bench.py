@@ -415,11 +415,12 @@ def bench_heterogeneous(ctx, vren_amd, args):
         acc = [a + m for a, m in zip(acc, b.run_profiled())]
     kernel_ms = {k: round(a / 3, 5) for k, a in zip(KERNELS, acc)}
     b.run(b.PER_FRAME)
-    _, wall_p = time_events(lambda: b.run(b.PER_FRAME), 2)
+    # as many plain runs as batched ones (the plain path's time varies more from run to run)
+    _, wall_p = time_events(lambda: b.run(b.PER_FRAME), steps)
     ctx.sync_status()
     b.close()
     fps = 64 * steps / (ms_b * 1e-3)
-    plain = 64 * 2 / (wall_p * 1e-3)
+    plain = 64 * steps / (wall_p * 1e-3)
     print("[bench] 64 heterogeneous frames: %.3f ms/step batched, %.1f frames/s plain" % (ms_b / steps, plain),
           file=sys.stderr, flush=True)
     return {"frames": 64, "pixels": total_px, "frames_per_s": round(fps, 2), "ms_per_step": round(ms_b / steps, 4),
