@@ -249,7 +249,7 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
         if constexpr (STAGED) {
             sl = s_light[e - start];
         } else {
-            const uint32_t li = fd.indices[e];
+            const uint32_t li = *((guint*)fd.indices + e);
             if (li >= fd.L) continue;
             sl = stage_light(fd, li);
         }
@@ -741,13 +741,18 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     // every per-pixel load is issued before any of them is used: the reference and the pixel's running material
     // gradient along with the G-buffer, not after the reconstruction and the light loops
     if (in) {
-        const float depth = fd.depth[p];
-        nb = reinterpret_cast<const uint2*>(fd.normals)[p];
-        ar = reinterpret_cast<const float4*>(fd.albedo_rough)[p];
-        metal = fd.metallic[p];
-        go = reinterpret_cast<const float4*>(grad_out)[p];
-        cl_raw = fd.refs[p];
-        gar = reinterpret_cast<const float4*>(g_ar)[p];
+        // global-address-space loads, as in the forward kernel
+        const float depth = *((gfloat*)fd.depth + p);
+        const vu2 nv = *((gvu2*)fd.normals + p);
+        nb = make_uint2(nv.x, nv.y);
+        const vf4 av = *((gvf4*)fd.albedo_rough + p);
+        ar = make_float4(av.x, av.y, av.z, av.w);
+        metal = *((gfloat*)fd.metallic + p);
+        const vf4 gv = *((gvf4*)grad_out + p);
+        go = make_float4(gv.x, gv.y, gv.z, gv.w);
+        cl_raw = *((guint*)fd.refs + p);
+        const vf4 gav = *((gvf4*)g_ar + p);
+        gar = make_float4(gav.x, gav.y, gav.z, gav.w);
         px = reconstruct(fd, gx, gy, depth);
     }
     const bool has_list = px.shaded && fd.L > 0 && cl_raw < fd.key_cap;
@@ -755,9 +760,9 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     // reduced from its pixels' lists, so that every list lies inside the staged range
     uint32_t off = 0, end = 0;
     if (has_list) {
-        const uint32_t o = fd.offsets[cl_raw];
+        const uint32_t o = *((guint*)fd.offsets + cl_raw);
         off = min(o, fd.index_cap);
-        const uint64_t e = (uint64_t)o + fd.counts[cl_raw];
+        const uint64_t e = (uint64_t)o + *((guint*)fd.counts + cl_raw);
         end = e > fd.index_cap ? fd.index_cap : (uint32_t)e;
     }
     uint32_t rmin = has_list ? off : 0xFFFFFFFFu, rmax = has_list ? end : 0u;
@@ -776,15 +781,23 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
     const uint32_t start = ra == 0xFFFFFFFFu ? 0u : ra;
     const uint32_t nstage = (ra != 0xFFFFFFFFu && rb > ra) ? rb - ra : 0u;
     const bool staged = nstage <= kStage;
-    const float4* lpos = reinterpret_cast<const float4*>(fd.light_pos);
-    const float4* lcol = reinterpret_cast<const float4*>(fd.lights);
+    // light records and list entries through global-address-space loads (flat loads would also wait on the LDS
+    // counter)
+    const auto lpos = [&](uint32_t li) {
+        const vf4 v = *((gvf4*)fd.light_pos + li);
+        return make_float4(v.x, v.y, v.z, v.w);
+    };
+    const auto lcol = [&](uint32_t li) {
+        const vf4 v = *((gvf4*)fd.lights + li);
+        return make_float4(v.x, v.y, v.z, v.w);
+    };
     if (staged) {
         for (uint32_t i = tid; i < nstage; i += 256) {
-            const uint32_t li = fd.indices[start + i];
+            const uint32_t li = *((guint*)fd.indices + start + i);
             s_li[i] = li;
             const bool ok = li < fd.L;
-            s_lpos[i] = ok ? lpos[li] : make_float4(0.f, 0.f, 0.f, 0.f);
-            s_lcol[i] = ok ? lcol[li] : make_float4(0.f, 0.f, 0.f, 0.f);
+            s_lpos[i] = ok ? lpos(li) : make_float4(0.f, 0.f, 0.f, 0.f);
+            s_lcol[i] = ok ? lcol(li) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         for (uint32_t i = tid; i < nstage * 7; i += 256) s_g[i] = 0.0f;
     }
@@ -823,10 +836,10 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
                 lp = s_lpos[e - start];
                 lc = s_lcol[e - start];
             } else {
-                li = fd.indices[e];
+                li = *((guint*)fd.indices + e);
                 if (li >= fd.L) continue;
-                lp = lpos[li];
-                lc = lcol[li];
+                lp = lpos(li);
+                lc = lcol(li);
             }
             const LightTerm lt = light_term(px.P, lp, lc, fd.attenuation);
             if (!lt.live) continue;
@@ -863,11 +876,11 @@ __global__ __launch_bounds__(256) void shade_backward_kernel(const FrameDev* __r
                 lp = s_lpos[e - start];
                 lc = s_lcol[e - start];
             } else {
-                li = fd.indices[e];
+                li = *((guint*)fd.indices + e);
                 ok = li < fd.L;
                 if (ok) {
-                    lp = lpos[li];
-                    lc = lcol[li];
+                    lp = lpos(li);
+                    lc = lcol(li);
                 }
             }
             if (ok) {
