@@ -285,11 +285,10 @@ __device__ __forceinline__ void light_loop(const FrameDev& fd, const PixelIn& px
     }
 }
 
-// Shades one pixel whose inputs are loaded: its list entries [off, end) come from the LDS stage (when `staged`,
-// entry e at e - start) or from global memory.
-__device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar, float metal,
-                                            bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
-                                            const StagedLight* s_light, uint64_t p)
+// The per-pixel constants of the forward shade and the radiance accumulator's start, the ambient term
+// (shade.comp:133), to which the lights are added.
+__device__ __forceinline__ ShadeSurface shade_surface(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar,
+                                                      float metal, float acc[3])
 {
     ShadeSurface s;
     {
@@ -313,21 +312,25 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     const float NdotV = fmaxf(dot3(s.N, s.V), 0.0f);
     s.a2ggxV = s.a2 * (NdotV * frcp(NdotV * (1.0f - rough) + rough));
     s.NdotV4 = 4.0f * NdotV;
+    for (int c = 0; c < 3; c++) acc[c] = 0.03f * albedo[c];
+    return s;
+}
 
-    // the ambient term (shade.comp:133) first, the lights added to it
-    float acc[3] = {0.03f * albedo[0], 0.03f * albedo[1], 0.03f * albedo[2]};
-    if (has_list) {
-        // one loop per (staging, attenuation) pair: both are uniform over the workgroup, so no entry pays for the
-        // other branch's registers or its test
-        const bool smooth = fd.attenuation == VREN_ATTEN_SMOOTH;
-        if (staged) {
-            if (smooth) light_loop<true, true>(fd, px, s, off, end, start, s_light, acc);
-            else light_loop<true, false>(fd, px, s, off, end, start, s_light, acc);
-        } else {
-            if (smooth) light_loop<false, true>(fd, px, s, off, end, start, s_light, acc);
-            else light_loop<false, false>(fd, px, s, off, end, start, s_light, acc);
-        }
-    }
+// The pixel's list entries [off, end) from the LDS stage (STAGED, entry e at e - start) or from global memory; one
+// loop per (staging, attenuation) pair: both are uniform over the workgroup, so no entry pays for the other branch's
+// registers or its test.
+template <bool STAGED>
+__device__ __forceinline__ void shade_lights(const FrameDev& fd, const PixelIn& px, const ShadeSurface& s,
+                                             uint32_t off, uint32_t end, uint32_t start, const StagedLight* s_light,
+                                             float acc[3])
+{
+    if (fd.attenuation == VREN_ATTEN_SMOOTH) light_loop<STAGED, true>(fd, px, s, off, end, start, s_light, acc);
+    else light_loop<STAGED, false>(fd, px, s, off, end, start, s_light, acc);
+}
+
+// The directional lights, the gamma and the pixel's store.
+__device__ __forceinline__ void shade_finish(const FrameDev& fd, const ShadeSurface& s, float acc[3], uint64_t p)
+{
     for (uint32_t i = 0; i < fd.n_dir; i++) {
         const float4* D = reinterpret_cast<const float4*>(fd.dir_lights) + 2 * i;
         const float4 dir = D[0], col = D[1];
@@ -346,6 +349,21 @@ __device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& p
     const vf4 ov = {o.x, o.y, o.z, o.w};
     // a streaming store: the radiance is not read again by the chain (1.4 % of the kernel against a cached store)
     __builtin_nontemporal_store(ov, (__attribute__((address_space(1))) vf4*)fd.out + p);
+}
+
+// Shades one pixel whose inputs are loaded: its list entries [off, end) come from the LDS stage (when `staged`,
+// entry e at e - start) or from global memory.
+__device__ __forceinline__ void shade_pixel(const FrameDev& fd, const PixelIn& px, uint2 nb, float4 ar, float metal,
+                                            bool has_list, uint32_t off, uint32_t end, bool staged, uint32_t start,
+                                            const StagedLight* s_light, uint64_t p)
+{
+    float acc[3];
+    const ShadeSurface s = shade_surface(fd, px, nb, ar, metal, acc);
+    if (has_list) {
+        if (staged) shade_lights<true>(fd, px, s, off, end, start, s_light, acc);
+        else shade_lights<false>(fd, px, s, off, end, start, s_light, acc);
+    }
+    shade_finish(fd, s, acc, p);
 }
 
 // ---- material resolve (shade.comp:89-96) -------------------------------------------------------------------------
@@ -548,14 +566,36 @@ __global__ __launch_bounds__(256, 8) void shade_kernel(const FrameDev* __restric
         nstage = (a != 0xFFFFFFFFu && b > a) ? b - a : 0u;
         if (nstage <= kFwdStage && tid < nstage) li = *((guint*)fd.indices + start + tid);
     }
-    const bool staged = nstage <= kFwdStage;
-    if (staged && tid < nstage) s_light[tid] = stage_light(fd, li);
-    __syncthreads();
+    if (nstage <= kFwdStage) {
+        if (tid < nstage) s_light[tid] = stage_light(fd, li);
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PPT; q++)
-        if (px[q].shaded)
-            shade_pixel(fd, px[q], nb[q], ar[q], metal[q], has_list[q], my_off[q], my_end[q], staged, start, s_light,
-                        p[q]);
+        for (int q = 0; q < PPT; q++)
+            if (px[q].shaded)
+                shade_pixel(fd, px[q], nb[q], ar[q], metal[q], has_list[q], my_off[q], my_end[q], true, start, s_light,
+                            p[q]);
+        return;
+    }
+    // a range longer than the stage (a few tiles of frames with tens of thousands of lights): the range in windows
+    // of kFwdStage entries, each staged in turn, every pixel shading the part of its list inside the window (a loop
+    // uniform over the workgroup, so the barriers are reached by every thread)
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        ShadeSurface sf;
+        if (px[q].shaded) sf = shade_surface(fd, px[q], nb[q], ar[q], metal[q], acc);
+        for (uint32_t w0 = start; w0 - start < nstage; w0 += kFwdStage) {
+            const uint32_t n = min(kFwdStage, nstage - (w0 - start));
+            __syncthreads(); // the previous window's readers are done
+            if (tid < n) s_light[tid] = stage_light(fd, *((guint*)fd.indices + w0 + tid));
+            __syncthreads();
+            if (px[q].shaded && has_list[q]) {
+                const uint32_t a = max(my_off[q], w0), b = min(my_end[q], w0 + n);
+                if (a < b) shade_lights<true>(fd, px[q], sf, a, b, w0, s_light, acc);
+            }
+        }
+        if (px[q].shaded) shade_finish(fd, sf, acc, p[q]);
+    }
 }
 
 
