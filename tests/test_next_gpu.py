@@ -213,6 +213,51 @@ def test_batch_mixes_material_and_resolved_frames(vctx, co):
     b.close()
 
 
+def test_batch_material_path_long_lists(vctx, co):
+    """The material path when a tile's light-list range is longer than the shade's 256-entry stage: 640x360 with 3000
+    lights at five times their radius (most tiles stage their range in windows). Every shaded pixel against the FP64
+    oracle of the material path."""
+    import torch
+    import vren_amd
+    from test_clustered_gpu import _batch_host_frame, make_frames_batch
+    specs = [(640, 360, 3000, 5100)]
+    descs, keep = make_frames_batch(vctx, co, specs)
+    keep[0]["l"].view(torch.float32).view(3000, 4)[:, 3] *= 5.0
+    textures, mats = make_materials(23)
+    W, H = 640, 360
+    tc, mi = synth_texcoords(33, W, H, len(mats))
+    dm = DeviceMaterials(vren_amd, tc, mi, textures, mats)
+    descs[0].materials = ctypes.pointer(dm.set)
+    descs[0].albedo_roughness, descs[0].metallic = None, None
+    torch.cuda.synchronize()
+    cf = ol.cfg()
+    b = vren_amd.Batch(vctx, descs, cf)
+    for mode in (b.PER_FRAME, b.GRAPH):
+        keep[0]["o"].fill_(0xFF)
+        b.run(mode)
+        torch.cuda.synchronize()
+        vctx.sync_status()
+        f = _batch_host_frame(b, keep, 0)
+        okeys, oU, orefs = co.unique_clusters(W, H, f["cam"], cf, f["depth"], f["normals"])
+        ovp, ocodes, obvh = co.light_bvh(f["pos"], f["lig"], f["cam"], cf)
+        oc, oo, oi, ot = co.assign(W, H, f["cam"], cf, okeys, oU, obvh, ocodes, ovp)
+        assert f["U"] == oU and np.array_equal(f["idx"], oi)
+        tile_sums = np.bincount(np.unique(okeys & 0xFFFF, return_inverse=True)[1], weights=oc.astype(np.int64))
+        assert (tile_sums > 256).sum() > 0
+        cc = np.zeros(f["key_capacity"], np.uint32)
+        cco = np.zeros(f["key_capacity"], np.uint32)
+        cc[:oU], cco[:oU] = oc, oo
+        ref, mask = co.shade_materials(W, H, f["cam"], cf, f["depth"], f["normals"], tc, mi, mats, textures, orefs,
+                                       f["key_capacity"], cc, cco, oi, f["pos"], f["lig"])
+        got = f["out"]
+        assert np.all(np.isnan(got[~mask]))
+        r = ref[mask, :3]
+        fin = ~np.isnan(r).any(axis=1)
+        err = _err(got[mask, :3][fin], r[fin])
+        assert fin.mean() > 0.99 and err.max() <= SHADE_RTOL, (mode, err.max())
+    b.close()
+
+
 # ---- depth pyramid -------------------------------------------------------------------------------------------------
 
 @pytest.mark.parametrize("W,H,images", [(1, 1, 1), (2, 1, 3), (5, 100, 2), (64, 64, 1), (333, 200, 4),
